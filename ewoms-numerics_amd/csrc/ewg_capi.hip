@@ -1,0 +1,712 @@
+// ewg_capi.hip -- host side of libewg: context, C ABI, BiCGStab / Newton / time-loop
+// drivers. The numerical work runs in the kernels of ewg_linearize.hip and
+// ewg_linalg.hip on the context's stream; the host keeps only the control flow that
+// NewtonMethod::apply, BiCGStabSolver::apply and FvBaseProblem::timeIntegration define.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/ewg_capi.h"
+#include "ewg_internal.h"
+
+using namespace ewg;
+
+struct ewg_ctx {
+    Grid g{};
+    ewg_grid_desc grid{};
+    ewg_fluid_desc fluid{};
+    ewg_params prm{};
+    int law = 0;
+    int nregions = 0;
+    long long B = 0;
+    long long side_off[7] = {0};
+    hipStream_t stream = nullptr;
+    // device state
+    double *x0 = nullptr, *x1 = nullptr, *xcur = nullptr, *sto = nullptr;
+    double *R = nullptr, *J = nullptr;
+    double *K = nullptr, *phi = nullptr, *pe = nullptr;
+    int *region = nullptr;
+    MatReg *regs = nullptr;
+    int *bc_type = nullptr;
+    double *bc_val = nullptr;
+    // preconditioner
+    double *L64 = nullptr, *D64 = nullptr;
+    float *L32 = nullptr, *D32 = nullptr, *U32 = nullptr, *w32 = nullptr;
+    // Krylov vectors
+    double *kr = nullptr, *kp = nullptr, *kv = nullptr, *ky = nullptr, *kz = nullptr,
+           *kx = nullptr;
+    double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr;
+    double *dscalar = nullptr;
+    int *anyNonzero = nullptr, *ilu_err = nullptr;
+    KrylovState *ks = nullptr;
+    // pinned host mirrors
+    KrylovState *ks_host = nullptr;
+    double *scal_host = nullptr;
+    int *err_host = nullptr;
+    // host copies of the BCRS pattern (rows ascending columns) and slot mapping
+    std::vector<int32_t> rowptr, colidx;
+    std::vector<int64_t> bcrs_src;   // BCRS block -> slot * N + row
+    // Newton state (newtonmethod.hh:197 starts error_ at 1e100)
+    double error = 1e100, last_error = 1e100;
+    int num_iterations = 0;
+    int last_linear_iterations = 0;
+    bool static_set = false;
+    bool have_system = false;
+    ewg_stats stats{};
+    std::string err;
+};
+
+namespace {
+
+int set_err(ewg_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return set_err(ctx, EWG_DEVICE_ERROR,                                         \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+    return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+void free_all(ewg_ctx *c) {
+    void *ptrs[] = {c->x0, c->x1, c->xcur, c->sto, c->R, c->J, c->K, c->phi, c->pe, c->region,
+                    c->regs, c->bc_type, c->bc_val, c->L64, c->D64, c->L32, c->D32, c->U32,
+                    c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
+                    c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->ks_host) (void)hipHostFree(c->ks_host);
+    if (c->scal_host) (void)hipHostFree(c->scal_host);
+    if (c->err_host) (void)hipHostFree(c->err_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+LinArgs lin_args(ewg_ctx *c, double dt, int newton_iter) {
+    LinArgs a{};
+    a.g = c->g;
+    for (int d = 0; d < 3; ++d) a.grav[d] = c->fluid.gravity[d];
+    a.rho[0] = c->fluid.rho[0];
+    a.rho[1] = c->fluid.rho[1];
+    a.mu[0] = c->fluid.mu[0];
+    a.mu[1] = c->fluid.mu[1];
+    a.x = c->x0;
+    a.x1 = c->x1;
+    a.sto = c->sto;
+    a.K = c->K;
+    a.phi = c->phi;
+    a.region = c->nregions > 1 ? c->region : nullptr;
+    a.pe = c->pe;
+    a.regs = c->regs;
+    a.bc_type = c->bc_type;
+    a.bc_val = c->bc_val;
+    for (int s = 0; s < 7; ++s) a.side_off[s] = c->side_off[s];
+    a.dt = dt;
+    a.newton_iter = newton_iter;
+    a.enable_cache = c->prm.enable_storage_cache;
+    a.recycle = c->prm.recycle_first_iteration_storage;
+    a.R = c->R;
+    a.J = c->J;
+    return a;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// max |v| over n doubles, to the host
+int device_max_abs(ewg_ctx *c, const double *v, double *out) {
+    launch_max_abs(2 * c->g.N, v, c->part0, c->stream);
+    launch_reduce_max(c->part0, c->dscalar, c->stream);
+    HIP_TRY(c, hipMemcpyAsync(c->scal_host, c->dscalar, sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *out = c->scal_host[0];
+    return EWG_OK;
+}
+
+void precond_factor(ewg_ctx *c) {
+    if (c->prm.precond_fp32)
+        launch_ilu_factor32(c->g, c->J, c->L32, c->D32, c->U32, c->ilu_err, c->stream);
+    else
+        launch_ilu_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->stream);
+}
+
+void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status) {
+    if (c->prm.precond_fp32)
+        launch_ilu_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->stream);
+    else
+        launch_ilu_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->stream);
+}
+
+// BiCGStabSolver::apply on J dx = b (b = residual on the device). Returns EWG_OK,
+// EWG_NOT_CONVERGED or EWG_NUMERICAL_ISSUE; *iters as BiCGStab's report.
+int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
+    const int n = 2 * c->g.N;
+    hipStream_t s = c->stream;
+    double absTol = c->prm.linear_abs_tolerance;
+    if (absTol < 0.0) absTol = c->prm.newton_tolerance / 100.0;
+    HIP_TRY(c, hipMemsetAsync(c->ilu_err, 0, sizeof(int), s));
+    HIP_TRY(c, hipMemsetAsync(c->anyNonzero, 0, sizeof(int), s));
+    precond_factor(c);
+    launch_bicg_init(n, b, c->kr, x, c->kp, c->kv, c->part0, c->part1, s);
+    launch_ks_init(c->ks, c->part0, c->part1, c->prm.linear_tolerance, absTol,
+                   c->prm.linear_max_error, s);
+    HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    *iters = 0;
+    if (*c->err_host) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
+    if (c->ks_host->status == KS_CONVERGED) return EWG_OK;
+    const int *st = &c->ks->status;
+    for (int it = 0; it < c->prm.linear_max_iterations; ++it) {
+        launch_ks_beta(c->ks, c->part2, s);
+        launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s);
+        precond_apply(c, c->kp, c->ky, st);
+        launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
+        launch_ks_alpha(c->ks, c->part0, s);
+        launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->anyNonzero, s);
+        launch_ks_crit1(c->ks, c->part1, c->anyNonzero, s);
+        precond_apply(c, c->kr, c->kz, st);
+        launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
+        launch_ks_omega(c->ks, c->part0, c->part1, s);
+        launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->anyNonzero, s);
+        launch_ks_crit2(c->ks, c->anyNonzero, s);
+        HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
+                                  s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        if (c->ks_host->status != KS_RUNNING) break;
+    }
+    *iters = c->ks_host->iterations;
+    switch (c->ks_host->status) {
+        case KS_CONVERGED: return EWG_OK;
+        case KS_BREAKDOWN: return set_err(c, EWG_NUMERICAL_ISSUE, "Breakdown of the BiCGStab solver");
+        default: return set_err(c, EWG_NOT_CONVERGED, "linear solver did not converge");
+    }
+}
+
+int linearize_device(ewg_ctx *c, double dt, int newton_iter) {
+    LinArgs a = lin_args(c, dt, newton_iter);
+    launch_linearize(a, c->stream);
+    HIP_TRY(c, hipGetLastError());
+    c->have_system = true;
+    return EWG_OK;
+}
+
+// FP32 preconditioner storage, allocated on first use
+int ensure_fp32(ewg_ctx *c) {
+    if (c->L32) return EWG_OK;
+    size_t N = c->g.N;
+    HIP_TRY(c, dalloc(&c->L32, (size_t)c->g.dim * N * 4));
+    HIP_TRY(c, dalloc(&c->D32, N * 4));
+    HIP_TRY(c, dalloc(&c->U32, (size_t)c->g.dim * N * 4));
+    HIP_TRY(c, dalloc(&c->w32, 2 * N));
+    return EWG_OK;
+}
+
+bool proceed(const ewg_ctx *c) {  // newtonmethod.hh:866-884
+    if (c->num_iterations < 1) return true;
+    if (c->error <= c->prm.newton_tolerance) return false;
+    if (c->num_iterations >= c->prm.newton_max_iterations) return c->error * 4.0 < c->last_error;
+    return true;
+}
+
+int newton(ewg_ctx *c, double dt) {
+    c->num_iterations = 0;
+    const size_t vbytes = sizeof(double) * 2 * c->g.N;
+    int rc;
+    while (proceed(c)) {
+        c->last_error = c->error;  // beginIteration_
+        HIP_TRY(c, hipMemcpyAsync(c->xcur, c->x0, vbytes, hipMemcpyDeviceToDevice, c->stream));
+        double t0 = now_s();
+        if ((rc = linearize_device(c, dt, c->num_iterations)) != EWG_OK) return rc;
+        c->stats.newton_iterations++;
+        c->last_error = c->error;  // preSolve_
+        double e;
+        if ((rc = device_max_abs(c, c->R, &e)) != EWG_OK) return rc;
+        double t1 = now_s();
+        c->stats.time_linearize_s += t1 - t0;
+        c->error = e;
+        if (c->error > c->prm.newton_max_error) {
+            c->num_iterations = 2 * c->prm.newton_target_iterations;
+            return set_err(c, EWG_NUMERICAL_ISSUE, "Newton: error above NewtonMaxError");
+        }
+        if (!proceed(c)) {
+            ++c->num_iterations;
+            break;
+        }
+        int its = 0;
+        rc = bicgstab(c, c->R, c->kx, &its);
+        double t2 = now_s();
+        c->stats.time_solve_s += t2 - t1;
+        c->last_linear_iterations = its;
+        c->stats.linear_iterations += its;
+        c->stats.linear_solves++;
+        if (rc != EWG_OK) {
+            c->num_iterations = 2 * c->prm.newton_target_iterations;
+            return rc;
+        }
+        launch_newton_update(2 * c->g.N, c->xcur, c->kx, c->x0, c->part0, c->stream);
+        launch_reduce_sum(c->part0, c->dscalar, c->stream);
+        HIP_TRY(c, hipMemcpyAsync(c->scal_host, c->dscalar, sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->stats.time_update_s += now_s() - t2;
+        if (!std::isfinite(c->scal_host[0])) {
+            c->num_iterations = 2 * c->prm.newton_target_iterations;
+            return set_err(c, EWG_NUMERICAL_ISSUE, "Non-finite update!");
+        }
+        ++c->num_iterations;
+    }
+    if (!(c->error <= c->prm.newton_tolerance)) {
+        c->num_iterations = 2 * c->prm.newton_target_iterations;
+        return set_err(c, EWG_NOT_CONVERGED, "Newton did not converge");
+    }
+    return EWG_OK;
+}
+
+double suggest_dt(const ewg_ctx *c, double oldDt) {  // newtonmethod.hh:527-544
+    int tgt = c->prm.newton_target_iterations;
+    if (c->num_iterations > tgt) {
+        double percent = static_cast<double>(c->num_iterations - tgt) / tgt;
+        return std::max(c->prm.min_time_step_size, oldDt / (1.0 + percent));
+    }
+    double percent = static_cast<double>(tgt - c->num_iterations) / tgt;
+    return std::max(c->prm.min_time_step_size, oldDt * (1.0 + percent / 1.2));
+}
+
+}  // namespace
+
+extern "C" {
+
+void ewg_default_params(ewg_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->newton_tolerance = 1e-8;
+    p->newton_max_error = 1e100;
+    p->newton_target_iterations = 10;
+    p->newton_max_iterations = 18;
+    p->linear_tolerance = 1e-3;
+    p->linear_abs_tolerance = -1.0;
+    p->linear_max_error = 1e7;
+    p->linear_max_iterations = 1000;
+    p->enable_storage_cache = 1;
+    p->recycle_first_iteration_storage = 1;
+    p->max_time_step_divisions = 10;
+    p->min_time_step_size = 0.0;
+    p->max_time_step_size = std::numeric_limits<double>::infinity();
+    p->precond_fp32 = 0;
+}
+
+int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
+               const ewg_material_desc *material, const ewg_params *params, ewg_ctx **out) {
+    if (!grid || !fluid || !material || !params || !out) return EWG_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (grid->dim != 2 && grid->dim != 3) return EWG_INVALID_ARGUMENT;
+    if (grid->dim == 2 && grid->nz != 1) return EWG_INVALID_ARGUMENT;
+    // the structured ILU(0) relies on nx >= 3 (and ny >= 3 in 3D): then the only fill
+    // position two pattern offsets can reach is the diagonal
+    if (grid->nx < 3 || grid->ny < (grid->dim == 3 ? 3 : 1) || grid->nz < 1)
+        return EWG_INVALID_ARGUMENT;
+    long long N = (long long)grid->nx * grid->ny * grid->nz;
+    if (N >= (1LL << 31) / 8) return EWG_INVALID_ARGUMENT;
+    if (material->nregions < 1 || !material->region_params) return EWG_INVALID_ARGUMENT;
+    if (material->law != EWG_LAW_VAN_GENUCHTEN && material->law != EWG_LAW_BROOKS_COREY)
+        return EWG_INVALID_ARGUMENT;
+    auto *c = new ewg_ctx();
+    c->grid = *grid;
+    c->fluid = *fluid;
+    c->prm = *params;
+    c->law = material->law;
+    c->nregions = material->nregions;
+    Grid &g = c->g;
+    g.dim = grid->dim;
+    g.nx = grid->nx;
+    g.ny = grid->ny;
+    g.nz = grid->nz;
+    g.N = static_cast<int>(N);
+    g.nxny = g.nx * g.ny;
+    g.nslots = 2 * g.dim + 1;
+    g.diag = g.dim;
+    g.nlevels = g.dim == 3 ? g.nx + g.ny + g.nz - 2 : g.nx + g.ny - 1;
+    g.h[0] = grid->lx / g.nx;
+    g.h[1] = grid->ly / g.ny;
+    g.h[2] = g.dim == 3 ? grid->lz / g.nz : 1.0;
+    if (g.dim == 2) {
+        g.V = g.h[0] * g.h[1];
+        g.area[0] = g.h[1];
+        g.area[1] = g.h[0];
+        g.area[2] = 0.0;
+    } else {
+        g.V = g.h[0] * g.h[1] * g.h[2];
+        g.area[0] = g.h[1] * g.h[2];
+        g.area[1] = g.h[0] * g.h[2];
+        g.area[2] = g.h[0] * g.h[1];
+    }
+    long long side[6];
+    side[0] = side[1] = (long long)g.ny * g.nz;
+    side[2] = side[3] = (long long)g.nx * g.nz;
+    side[4] = side[5] = g.dim == 3 ? (long long)g.nx * g.ny : 0;
+    c->side_off[0] = 0;
+    for (int s = 0; s < 6; ++s) c->side_off[s + 1] = c->side_off[s] + side[s];
+    c->B = c->side_off[6];
+
+    std::vector<MatReg> regs;
+    for (int r = 0; r < material->nregions; ++r)
+        regs.push_back(make_region(material->law, material->region_params + 4 * r));
+
+    auto fail = [&](const char *what, hipError_t e) {
+        std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+        free_all(c);
+        delete c;
+        std::fprintf(stderr, "ewg_create: %s\n", msg.c_str());
+        return EWG_DEVICE_ERROR;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail("stream", e);
+    const size_t n2 = 2 * (size_t)N;
+    const size_t nb = (size_t)g.nslots * N * 4;
+#define ALLOC(p, cnt) \
+    if ((e = dalloc(&(p), (cnt))) != hipSuccess) return fail(#p, e)
+    ALLOC(c->x0, n2);
+    ALLOC(c->x1, n2);
+    ALLOC(c->xcur, n2);
+    ALLOC(c->sto, n2);
+    ALLOC(c->R, n2);
+    ALLOC(c->J, nb);
+    ALLOC(c->K, N);
+    ALLOC(c->phi, N);
+    ALLOC(c->region, N);
+    ALLOC(c->regs, regs.size());
+    ALLOC(c->bc_type, c->B);
+    ALLOC(c->bc_val, 2 * c->B);
+    ALLOC(c->L64, (size_t)g.dim * N * 4);
+    ALLOC(c->D64, (size_t)N * 4);
+    ALLOC(c->kr, n2);
+    ALLOC(c->kp, n2);
+    ALLOC(c->kv, n2);
+    ALLOC(c->ky, n2);
+    ALLOC(c->kz, n2);
+    ALLOC(c->kx, n2);
+    ALLOC(c->part0, kRedBlocks);
+    ALLOC(c->part1, kRedBlocks);
+    ALLOC(c->part2, kRedBlocks);
+    ALLOC(c->dscalar, 8);
+    ALLOC(c->anyNonzero, 1);
+    ALLOC(c->ilu_err, 1);
+    ALLOC(c->ks, 1);
+#undef ALLOC
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->ks_host), sizeof(KrylovState))) != hipSuccess)
+        return fail("ks_host", e);
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->scal_host), 8 * sizeof(double))) != hipSuccess)
+        return fail("scal_host", e);
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->err_host), sizeof(int))) != hipSuccess)
+        return fail("err_host", e);
+    // blocks of absent neighbours stay zero for the whole life of the context
+    if ((e = hipMemset(c->J, 0, nb * sizeof(double))) != hipSuccess) return fail("J", e);
+    if ((e = hipMemset(c->x0, 0, n2 * sizeof(double))) != hipSuccess) return fail("x0", e);
+    if ((e = hipMemset(c->x1, 0, n2 * sizeof(double))) != hipSuccess) return fail("x1", e);
+    if ((e = hipMemset(c->sto, 0, n2 * sizeof(double))) != hipSuccess) return fail("sto", e);
+    if ((e = hipMemset(c->bc_type, 0, c->B * sizeof(int))) != hipSuccess) return fail("bc", e);
+    if ((e = hipMemset(c->bc_val, 0, 2 * c->B * sizeof(double))) != hipSuccess) return fail("bcv", e);
+    if ((e = hipMemcpy(c->regs, regs.data(), regs.size() * sizeof(MatReg), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return fail("regs", e);
+
+    // BCRS pattern on the host (IstlSparseMatrixAdapter::reserve order)
+    c->rowptr.assign(N + 1, 0);
+    c->colidx.reserve((size_t)N * g.nslots);
+    c->bcrs_src.reserve((size_t)N * g.nslots);
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    for (int i = 0; i < g.N; ++i) {
+        int ijk[3] = {i % g.nx, (i / g.nx) % g.ny, i / g.nxny};
+        for (int s = 0; s < g.nslots; ++s) {
+            int col;
+            if (s == g.diag) {
+                col = i;
+            } else if (s < g.diag) {
+                int ax = g.dim - 1 - s;
+                if (ijk[ax] == 0) continue;
+                col = i - (ax == 0 ? 1 : (ax == 1 ? g.nx : g.nxny));
+            } else {
+                int ax = s - g.diag - 1;
+                if (ijk[ax] + 1 >= nn[ax]) continue;
+                col = i + (ax == 0 ? 1 : (ax == 1 ? g.nx : g.nxny));
+            }
+            c->colidx.push_back(col);
+            c->bcrs_src.push_back((int64_t)s * N + i);
+        }
+        c->rowptr[i + 1] = static_cast<int32_t>(c->colidx.size());
+    }
+    if (c->prm.precond_fp32 && ensure_fp32(c) != EWG_OK) {
+        std::fprintf(stderr, "ewg_create: %s\n", c->err.c_str());
+        free_all(c);
+        delete c;
+        return EWG_DEVICE_ERROR;
+    }
+    *out = c;
+    return EWG_OK;
+}
+
+void ewg_destroy(ewg_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+const char *ewg_last_error(const ewg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
+    if (!c || !p) return EWG_INVALID_ARGUMENT;
+    if (p->precond_fp32) {
+        int rc = ensure_fp32(c);
+        if (rc) return rc;
+    }
+    c->prm = *p;
+    return EWG_OK;
+}
+
+int64_t ewg_num_cells(const ewg_ctx *c) { return c ? c->g.N : 0; }
+int64_t ewg_num_blocks(const ewg_ctx *c) { return c ? (int64_t)c->colidx.size() : 0; }
+int64_t ewg_num_boundary_faces(const ewg_ctx *c) { return c ? c->B : 0; }
+
+int ewg_set_static(ewg_ctx *c, const double *K, const double *phi, const int32_t *region,
+                   const double *pe) {
+    if (!c || !K || !phi || !region) return EWG_INVALID_ARGUMENT;
+    const size_t N = c->g.N;
+    for (size_t i = 0; i < N; ++i)
+        if (region[i] < 0 || region[i] >= c->nregions)
+            return set_err(c, EWG_INVALID_ARGUMENT, "region index out of range");
+    if (pe && c->law != EWG_LAW_BROOKS_COREY)
+        return set_err(c, EWG_INVALID_ARGUMENT, "per-cell entry pressure needs Brooks-Corey");
+    HIP_TRY(c, hipMemcpy(c->K, K, N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->phi, phi, N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->region, region, N * sizeof(int), hipMemcpyHostToDevice));
+    if (pe) {
+        if (!c->pe) HIP_TRY(c, dalloc(&c->pe, N));
+        HIP_TRY(c, hipMemcpy(c->pe, pe, N * sizeof(double), hipMemcpyHostToDevice));
+    } else if (c->pe) {
+        HIP_TRY(c, hipFree(c->pe));
+        c->pe = nullptr;
+    }
+    c->static_set = true;
+    return EWG_OK;
+}
+
+int ewg_set_boundary(ewg_ctx *c, const int32_t *type, const double *values) {
+    if (!c || !type || !values) return EWG_INVALID_ARGUMENT;
+    for (long long f = 0; f < c->B; ++f)
+        if (type[f] < EWG_BC_NO_FLOW || type[f] > EWG_BC_MASS_RATE)
+            return set_err(c, EWG_INVALID_ARGUMENT, "unknown boundary type");
+    HIP_TRY(c, hipMemcpy(c->bc_type, type, c->B * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->bc_val, values, 2 * c->B * sizeof(double), hipMemcpyHostToDevice));
+    return EWG_OK;
+}
+
+int ewg_set_solution(ewg_ctx *c, int time_idx, const double *x) {
+    if (!c || !x || (time_idx != 0 && time_idx != 1)) return EWG_INVALID_ARGUMENT;
+    HIP_TRY(c, hipMemcpy(time_idx == 0 ? c->x0 : c->x1, x, 2 * (size_t)c->g.N * sizeof(double),
+                         hipMemcpyHostToDevice));
+    return EWG_OK;
+}
+
+int ewg_get_solution(ewg_ctx *c, int time_idx, double *x) {
+    if (!c || !x || (time_idx != 0 && time_idx != 1)) return EWG_INVALID_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(x, time_idx == 0 ? c->x0 : c->x1, 2 * (size_t)c->g.N * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return EWG_OK;
+}
+
+int ewg_advance_time_level(ewg_ctx *c) {
+    if (!c) return EWG_INVALID_ARGUMENT;
+    HIP_TRY(c, hipMemcpyAsync(c->x1, c->x0, 2 * (size_t)c->g.N * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return EWG_OK;
+}
+
+int ewg_get_pattern(const ewg_ctx *c, int32_t *rowptr, int32_t *colidx) {
+    if (!c || !rowptr || !colidx) return EWG_INVALID_ARGUMENT;
+    std::copy(c->rowptr.begin(), c->rowptr.end(), rowptr);
+    std::copy(c->colidx.begin(), c->colidx.end(), colidx);
+    return EWG_OK;
+}
+
+int ewg_linearize(ewg_ctx *c, double dt, int newton_iter, double *resid, double *jac) {
+    if (!c || !(dt > 0.0) || newton_iter < 0) return EWG_INVALID_ARGUMENT;
+    if (!c->static_set) return set_err(c, EWG_INVALID_STATE, "static fields not set");
+    int rc = linearize_device(c, dt, newton_iter);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t N = c->g.N;
+    if (resid) HIP_TRY(c, hipMemcpy(resid, c->R, 2 * N * sizeof(double), hipMemcpyDeviceToHost));
+    if (jac) {
+        std::vector<double> slots((size_t)c->g.nslots * N * 4);
+        HIP_TRY(c, hipMemcpy(slots.data(), c->J, slots.size() * sizeof(double),
+                             hipMemcpyDeviceToHost));
+        for (size_t b = 0; b < c->bcrs_src.size(); ++b)
+            std::memcpy(jac + 4 * b, slots.data() + 4 * c->bcrs_src[b], 4 * sizeof(double));
+    }
+    return EWG_OK;
+}
+
+int ewg_solve(ewg_ctx *c, double *dx, int32_t *iterations) {
+    if (!c) return EWG_INVALID_ARGUMENT;
+    if (!c->have_system) return set_err(c, EWG_INVALID_STATE, "no linearization");
+    int its = 0;
+    int rc = bicgstab(c, c->R, c->kx, &its);
+    c->last_linear_iterations = its;
+    if (iterations) *iterations = its;
+    if (dx && rc >= 0) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipMemcpy(dx, c->kx, 2 * (size_t)c->g.N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return rc;
+}
+
+int ewg_newton(ewg_ctx *c, double dt, int32_t *iterations, double *final_error) {
+    if (!c || !(dt > 0.0)) return EWG_INVALID_ARGUMENT;
+    if (!c->static_set) return set_err(c, EWG_INVALID_STATE, "static fields not set");
+    int rc = newton(c, dt);
+    if (iterations) *iterations = c->num_iterations;
+    if (final_error) *final_error = c->error;
+    c->stats.last_newton_iterations = c->num_iterations;
+    c->stats.last_newton_error = c->error;
+    return rc;
+}
+
+int ewg_run(ewg_ctx *c, double t0, double t_end, double dt0, int32_t *steps_out) {
+    if (!c || !(dt0 > 0.0)) return EWG_INVALID_ARGUMENT;
+    if (!c->static_set) return set_err(c, EWG_INVALID_STATE, "static fields not set");
+    const size_t vbytes = 2 * (size_t)c->g.N * sizeof(double);
+    double t = t0;
+    auto remaining = [&](double tt) { return std::max(0.0, t_end - tt); };
+    double dt = std::min({dt0, c->prm.max_time_step_size, remaining(t)});
+    int steps = 0;
+    auto finished = [&](double tt, double dtt) {  // simulator.hh:489-496
+        double eps = std::max(std::fabs(tt), dtt) * std::numeric_limits<double>::epsilon() * 1e3;
+        return tt * (1.0 + eps) >= t_end;
+    };
+    while (!finished(t, dt)) {
+        bool ok = false;
+        for (int fails = 0; fails < c->prm.max_time_step_divisions; ++fails) {
+            int rc = newton(c, dt);
+            if (rc == EWG_OK) {
+                ok = true;
+                break;
+            }
+            if (rc < 0) return rc;  // device / API error: not a convergence failure
+            HIP_TRY(c, hipMemcpyAsync(c->x0, c->x1, vbytes, hipMemcpyDeviceToDevice, c->stream));
+            c->stats.failed_steps++;
+            double nextDt = dt / 2.0;
+            if (dt < c->prm.min_time_step_size * (1 + 1e-9))
+                return set_err(c, EWG_NOT_CONVERGED, "time integration failed at minimum step");
+            if (nextDt < c->prm.min_time_step_size) nextDt = c->prm.min_time_step_size;
+            dt = nextDt;
+        }
+        if (!ok) return set_err(c, EWG_NOT_CONVERGED, "Newton did not converge after dt divisions");
+        HIP_TRY(c, hipMemcpyAsync(c->x1, c->x0, vbytes, hipMemcpyDeviceToDevice, c->stream));
+        c->stats.time_steps++;
+        t += dt;
+        ++steps;
+        double dtNext = std::min(c->prm.max_time_step_size, suggest_dt(c, dt));
+        double simMax = remaining(t);
+        if (dtNext < simMax && simMax < dtNext * 2) dtNext = simMax / 2 * 1.01;
+        dt = std::min(simMax, dtNext);
+        if (finished(t, dt)) break;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (steps_out) *steps_out = steps;
+    return EWG_OK;
+}
+
+int ewg_get_stats(const ewg_ctx *c, ewg_stats *s) {
+    if (!c || !s) return EWG_INVALID_ARGUMENT;
+    *s = c->stats;
+    s->last_newton_iterations = c->num_iterations;
+    s->last_linear_iterations = c->last_linear_iterations;
+    s->last_newton_error = c->error;
+    return EWG_OK;
+}
+
+int ewg_reset_stats(ewg_ctx *c) {
+    if (!c) return EWG_INVALID_ARGUMENT;
+    c->stats = ewg_stats{};
+    return EWG_OK;
+}
+
+int ewg_time_kernel(ewg_ctx *c, int which, int reps, double *mean_ms) {
+    if (!c || reps < 1 || !mean_ms) return EWG_INVALID_ARGUMENT;
+    if (!c->static_set) return set_err(c, EWG_INVALID_STATE, "static fields not set");
+    hipStream_t s = c->stream;
+    if (which != EWG_KERNEL_LINEARIZE && !c->have_system) {
+        int rc = linearize_device(c, 1.0, 0);
+        if (rc) return rc;
+    }
+    if (which == EWG_KERNEL_ILU_APPLY) {
+        HIP_TRY(c, hipMemsetAsync(c->ilu_err, 0, sizeof(int), s));
+        precond_factor(c);
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    LinArgs a = lin_args(c, 1.0, 1);
+    HIP_TRY(c, hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+        switch (which) {
+            case EWG_KERNEL_LINEARIZE: launch_linearize(a, s); break;
+            case EWG_KERNEL_SPMV:
+                launch_spmv(c->g, c->J, c->x0, c->ky, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+                break;
+            case EWG_KERNEL_ILU_APPLY: precond_apply(c, c->R, c->ky, nullptr); break;
+            case EWG_KERNEL_ILU_FACTOR: precond_factor(c); break;
+            default:
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return set_err(c, EWG_INVALID_ARGUMENT, "unknown kernel id");
+        }
+    }
+    HIP_TRY(c, hipEventRecord(e1, s));
+    HIP_TRY(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(c, hipGetLastError());
+    *mean_ms = ms / reps;
+    return EWG_OK;
+}
+
+// device name / CU count for run records (no counterpart in the reference)
+int ewg_device_info(char *buf, int len) {
+    if (!buf || len < 1) return EWG_INVALID_ARGUMENT;
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+        std::snprintf(buf, len, "unknown");
+        return EWG_DEVICE_ERROR;
+    }
+    std::snprintf(buf, len, "%s (%s), %d CUs, %.1f GiB", p.name, p.gcnArchName,
+                  p.multiProcessorCount, p.totalGlobalMem / 1073741824.0);
+    return EWG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// ewg_internal.h -- device data layout and kernel entry points of libewg.
+//
+// Layout in HBM (N cells, lexicographic, x fastest):
+//   vectors      AoS double2 per cell [p_w, S_w] / [eq0, eq1]          16 B/cell
+//   Jacobian     structured BCRS, slot-major: J[s][cell][4], s = 0..2*dim in ascending
+//                column order (3D: -z,-y,-x,diag,+x,+y,+z), 2x2 blocks row-major
+//                [eq][pv]; blocks of absent neighbours are zero and never read.
+//                Column indices are implicit (cell + offset[s]).        224 B/cell (3D)
+//   ILU(0)       L[s][cell][4] for the dim lower slots, Dinv[cell][4] (inverted
+//                pivots); U is the Jacobian's upper slots (FP64) or an FP32 copy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "ewg_physics.h"
+
+namespace ewg {
+
+struct Grid {
+    int nx, ny, nz, dim;
+    int N, nxny;
+    int nslots, diag;   // 2*dim+1, dim
+    int nlevels;        // nx+ny+nz-2 (3D) / nx+ny-1 (2D): hyperplanes x+y+z = const
+    double h[3];
+    double V;
+    double area[3];
+};
+
+struct LinArgs {
+    Grid g;
+    double grav[3];
+    double rho[2], mu[2];
+    const double *x;      // solution(0)      [2N]
+    const double *x1;     // solution(1)      [2N]
+    double *sto;          // storage cache of the previous time level [2N]
+    const double *K;
+    const double *phi;
+    const int *region;    // nullptr when there is a single region
+    const double *pe;     // nullptr unless per-cell Brooks-Corey entry pressure
+    const MatReg *regs;
+    const int *bc_type;
+    const double *bc_val;
+    long long side_off[7];
+    double dt;
+    int newton_iter;
+    int enable_cache;
+    int recycle;
+    double *R;            // [2N]
+    double *J;            // [nslots][N][4]
+};
+
+// BiCGStab scalars, kept on the device so that one host read per iteration suffices
+struct KrylovState {
+    double rho, alpha, omega, beta;
+    double rho_i, denom;
+    double resid_err, init_err, last_err;
+    double rel_tol, abs_tol, max_resid;
+    int status;       // 0 running, 1 converged, 2 failed (criterion), 3 breakdown
+    int stagnates;
+    int iterations;
+    int pad;
+};
+enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_FAILED = 2, KS_BREAKDOWN = 3 };
+
+constexpr int kRedBlocks = 1024;   // fixed grid of the reducing vector kernels
+constexpr int kRedThreads = 256;
+
+// linearization
+void launch_linearize(const LinArgs &a, hipStream_t s);
+void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s);
+
+// ILU(0)
+void launch_ilu_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
+                         hipStream_t s);
+void launch_ilu_factor32(const Grid &g, const double *A, float *L, float *Dinv, float *U,
+                         int *err, hipStream_t s);
+void launch_ilu_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
+                        const double *d, double *v, const int *status, hipStream_t s);
+void launch_ilu_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
+                        const double *d, float *w, double *v, const int *status,
+                        hipStream_t s);
+
+// SpMV y = A x; optional fused partial dots (dotA = <a, y>, dotB = <b, y> with a,b
+// given; nullptr to skip), partials per block into part[0..gridDim)
+void launch_spmv(const Grid &g, const double *A, const double *x, double *y,
+                 const double *a, double *partA, const double *b, double *partB,
+                 const int *status, hipStream_t s);
+
+// Krylov helpers (n = 2N doubles)
+void launch_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v,
+                      double *partMax, double *partDot, hipStream_t s);
+void launch_ks_init(KrylovState *ks, const double *partMax, const double *partDot,
+                    double rel_tol, double abs_tol, double max_resid, hipStream_t s);
+void launch_ks_beta(KrylovState *ks, const double *partRho, hipStream_t s);
+void launch_p_update(int n, const KrylovState *ks, const double *r, const double *v, double *p,
+                     hipStream_t s);
+void launch_ks_alpha(KrylovState *ks, const double *partDen, hipStream_t s);
+void launch_hs_update(int n, const KrylovState *ks, const double *y, const double *v, double *x,
+                      double *r, double *partMax, int *anyNonzero, hipStream_t s);
+void launch_ks_crit1(KrylovState *ks, const double *partMax, int *anyNonzero, hipStream_t s);
+void launch_ks_omega(KrylovState *ks, const double *partTT, const double *partTS, hipStream_t s);
+void launch_xr_update(int n, const KrylovState *ks, const double *z, const double *t, double *x,
+                      double *r, const double *r0hat, double *partRho, int *anyNonzero,
+                      hipStream_t s);
+void launch_ks_crit2(KrylovState *ks, int *anyNonzero, hipStream_t s);
+
+// Newton helpers
+void launch_max_abs(int n, const double *x, double *part, hipStream_t s);
+void launch_reduce_max(const double *part, double *out, hipStream_t s);
+void launch_newton_update(int n, const double *cur, const double *dx, double *x, double *part,
+                          hipStream_t s);
+void launch_reduce_sum(const double *part, double *out, hipStream_t s);
+
+}  // namespace ewg

@@ -1,0 +1,600 @@
+// ewg_linalg.hip -- block ILU(0), SpMV and the BiCGStab vector/scalar kernels.
+//
+// ILU(0) in natural (lexicographic) order on the 7-point (5-point in 2D) pattern, as
+// dune-istl's blockILU0Decomposition computes it: L_ij = A_ij inv(U_jj), A_ik -= L_ij A_jk
+// on the pattern, pivots stored inverted. For nx, ny >= 3 the only pattern entry two
+// lower/upper offsets can reach is the diagonal, so U_ij = A_ij (j > i) and
+//   D_i = A_ii - sum_{j in {i-nxny, i-nx, i-1}} L_ij A_ji   (ascending j).
+// Row i depends on rows i-1, i-nx, i-nxny only, so all cells on the hyperplane
+// x + y + z = l are independent: factorization and both triangular sweeps run one
+// hyperplane per launch (exact natural-order ILU(0); the summation order inside each row
+// is the sequential one).
+#include "ewg_internal.h"
+
+namespace ewg {
+
+namespace {
+
+template <class T>
+__device__ __forceinline__ void bmul(const T *a, const T *b, T *out) {
+    T r0 = T(0), r1 = T(0), r2 = T(0), r3 = T(0);
+    r0 += a[0] * b[0]; r0 += a[1] * b[2];
+    r1 += a[0] * b[1]; r1 += a[1] * b[3];
+    r2 += a[2] * b[0]; r2 += a[3] * b[2];
+    r3 += a[2] * b[1]; r3 += a[3] * b[3];
+    out[0] = r0; out[1] = r1; out[2] = r2; out[3] = r3;
+}
+
+template <class T>
+__device__ __forceinline__ void bmmv(const T *a, const T *x, T *y) {  // y -= a x
+    y[0] -= a[0] * x[0];
+    y[0] -= a[1] * x[1];
+    y[1] -= a[2] * x[0];
+    y[1] -= a[3] * x[1];
+}
+
+template <class T>
+__device__ __forceinline__ void bmv(const T *a, const T *x, T *y) {  // y = a x
+    T s0 = T(0), s1 = T(0);
+    s0 += a[0] * x[0]; s0 += a[1] * x[1];
+    s1 += a[2] * x[0]; s1 += a[3] * x[1];
+    y[0] = s0; y[1] = s1;
+}
+
+template <class T>
+__device__ __forceinline__ void load_blk(const T *base, size_t idx, T *o) {
+    const T *p = base + idx * 4;
+    o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = p[3];
+}
+template <class T>
+__device__ __forceinline__ void load_blk_cast(const double *base, size_t idx, T *o) {
+    const double *p = base + idx * 4;
+    o[0] = static_cast<T>(p[0]); o[1] = static_cast<T>(p[1]);
+    o[2] = static_cast<T>(p[2]); o[3] = static_cast<T>(p[3]);
+}
+template <class T>
+__device__ __forceinline__ void store_blk(T *base, size_t idx, const T *o) {
+    T *p = base + idx * 4;
+    p[0] = o[0]; p[1] = o[1]; p[2] = o[2]; p[3] = o[3];
+}
+
+// map thread -> cell of hyperplane `level`; returns -1 if the thread has no cell
+__device__ __forceinline__ int level_cell(const Grid &g, int level, int t, int ijk[3]) {
+    int y, z;
+    if (g.dim == 3) {
+        if (t >= g.ny * g.nz) return -1;
+        y = t % g.ny;
+        z = t / g.ny;
+    } else {
+        if (t >= g.ny) return -1;
+        y = t;
+        z = 0;
+    }
+    int x = level - y - z;
+    if (x < 0 || x >= g.nx) return -1;
+    ijk[0] = x; ijk[1] = y; ijk[2] = z;
+    return x + g.nx * y + g.nxny * z;
+}
+
+__device__ __forceinline__ int lower_axis(const Grid &g, int s) { return g.dim - 1 - s; }
+__device__ __forceinline__ int axis_stride(const Grid &g, int a) {
+    return a == 0 ? 1 : (a == 1 ? g.nx : g.nxny);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+k_ilu_factor_level(Grid g, int level, const double *A, T *L, T *Dinv, T *U, int *err) {
+    int ijk[3];
+    int i = level_cell(g, level, blockIdx.x * blockDim.x + threadIdx.x, ijk);
+    if (i < 0) return;
+    T D[4];
+    load_blk_cast<T>(A, (size_t)g.diag * g.N + i, D);
+    for (int s = 0; s < g.dim; ++s) {  // lower slots, ascending column
+        int ax = lower_axis(g, s);
+        T Lij[4] = {T(0), T(0), T(0), T(0)};
+        if (ijk[ax] > 0) {
+            int j = i - axis_stride(g, ax);
+            T Aij[4], Dj[4], Aji[4], Bm[4];
+            load_blk_cast<T>(A, (size_t)s * g.N + i, Aij);
+            load_blk<T>(Dinv, j, Dj);
+            bmul(Aij, Dj, Lij);
+            load_blk_cast<T>(A, (size_t)(g.diag + 1 + ax) * g.N + j, Aji);
+            bmul(Lij, Aji, Bm);
+            for (int e = 0; e < 4; ++e) D[e] -= Bm[e];
+        }
+        store_blk(L, (size_t)s * g.N + i, Lij);
+    }
+    T det = D[0] * D[3] - D[1] * D[2];
+    if (!(fabs(static_cast<double>(det)) >= 1e-30)) {
+        atomicExch(err, 1);
+        det = T(1);
+    }
+    T det_1 = T(1) / det;
+    T inv[4] = {D[3] * det_1, -D[1] * det_1, -D[2] * det_1, D[0] * det_1};
+    store_blk(Dinv, i, inv);
+    if (U) {
+        for (int s = 0; s < g.dim; ++s) {
+            T Ui[4];
+            load_blk_cast<T>(A, (size_t)(g.diag + 1 + s) * g.N + i, Ui);
+            store_blk(U, (size_t)s * g.N + i, Ui);
+        }
+    }
+}
+
+// forward sweep: w_i = d_i - sum_j L_ij w_j
+template <class T>
+__global__ void __launch_bounds__(256)
+k_ilu_fwd_level(Grid g, int level, const T *L, const double *d, T *w, const int *status) {
+    if (status && *status != KS_RUNNING) return;
+    int ijk[3];
+    int i = level_cell(g, level, blockIdx.x * blockDim.x + threadIdx.x, ijk);
+    if (i < 0) return;
+    T rhs[2] = {static_cast<T>(d[2 * i]), static_cast<T>(d[2 * i + 1])};
+    for (int s = 0; s < g.dim; ++s) {
+        int ax = lower_axis(g, s);
+        if (ijk[ax] == 0) continue;
+        int j = i - axis_stride(g, ax);
+        T Lij[4];
+        load_blk<T>(L, (size_t)s * g.N + i, Lij);
+        T wj[2] = {w[2 * j], w[2 * j + 1]};
+        bmmv(Lij, wj, rhs);
+    }
+    w[2 * i] = rhs[0];
+    w[2 * i + 1] = rhs[1];
+}
+
+// backward sweep: w_i = inv(D_i) (w_i - sum_k U_ik w_k), out = (double) w
+template <class T, class UT>
+__global__ void __launch_bounds__(256)
+k_ilu_bwd_level(Grid g, int level, const UT *U, int uslot0, const T *Dinv, T *w, double *out,
+                const int *status) {
+    if (status && *status != KS_RUNNING) return;
+    int ijk[3];
+    int i = level_cell(g, level, blockIdx.x * blockDim.x + threadIdx.x, ijk);
+    if (i < 0) return;
+    int n[3] = {g.nx, g.ny, g.nz};
+    T rhs[2] = {w[2 * i], w[2 * i + 1]};
+    for (int s = 0; s < g.dim; ++s) {  // upper slots ascending: +x, +y, +z
+        if (ijk[s] + 1 >= n[s]) continue;
+        int k = i + axis_stride(g, s);
+        const UT *up = U + ((size_t)(uslot0 + s) * g.N + i) * 4;
+        T Uik[4] = {static_cast<T>(up[0]), static_cast<T>(up[1]), static_cast<T>(up[2]),
+                    static_cast<T>(up[3])};
+        T wk[2] = {w[2 * k], w[2 * k + 1]};
+        bmmv(Uik, wk, rhs);
+    }
+    T Di[4];
+    load_blk<T>(Dinv, i, Di);
+    T res[2];
+    bmv(Di, rhs, res);
+    w[2 * i] = res[0];
+    w[2 * i + 1] = res[1];
+    if (out != reinterpret_cast<double *>(w)) {
+        out[2 * i] = static_cast<double>(res[0]);
+        out[2 * i + 1] = static_cast<double>(res[1]);
+    }
+}
+
+inline int level_threads(const Grid &g) { return g.dim == 3 ? g.ny * g.nz : g.ny; }
+
+// ----------------------------------------------------------------------------------
+// reductions
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ double warp_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double warp_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    return v;
+}
+// block of kRedThreads (4 wavefronts); result valid in thread 0
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sh[kRedThreads / 64];
+    v = warp_sum(v);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kRedThreads / 64; ++w) r += sh[w];
+    return r;
+}
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double shm[kRedThreads / 64];
+    v = warp_max(v);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) shm[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kRedThreads / 64; ++w) r = fmax(r, shm[w]);
+    return r;
+}
+// sum / max of kRedBlocks partials by one block (fixed order: deterministic)
+__device__ __forceinline__ double reduce_partials_sum(const double *part) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) v += part[i];
+    return block_sum(v);
+}
+__device__ __forceinline__ double reduce_partials_max(const double *part) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) v = fmax(v, part[i]);
+    return block_max(v);
+}
+
+// SpMV over the present slots in ascending column order (BCRSMatrix::mv semantics)
+__global__ void __launch_bounds__(256)
+k_spmv(Grid g, const double *A, const double *x, double *y, const double *a, double *partA,
+       const double *b, double *partB, const int *status) {
+    if (status && *status != KS_RUNNING) return;
+    double accA = 0.0, accB = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += gridDim.x * blockDim.x) {
+        int ijk[3];
+        ijk[0] = i % g.nx;
+        int r = i / g.nx;
+        ijk[1] = r % g.ny;
+        ijk[2] = r / g.ny;
+        int n[3] = {g.nx, g.ny, g.nz};
+        double s0 = 0.0, s1 = 0.0;
+        for (int slot = 0; slot < g.nslots; ++slot) {
+            int off;
+            if (slot == g.diag) {
+                off = 0;
+            } else if (slot < g.diag) {
+                int ax = g.dim - 1 - slot;
+                if (ijk[ax] == 0) continue;
+                off = -(ax == 0 ? 1 : (ax == 1 ? g.nx : g.nxny));
+            } else {
+                int ax = slot - g.diag - 1;
+                if (ijk[ax] + 1 >= n[ax]) continue;
+                off = (ax == 0 ? 1 : (ax == 1 ? g.nx : g.nxny));
+            }
+            const double2 *blk = reinterpret_cast<const double2 *>(A + ((size_t)slot * g.N + i) * 4);
+            double2 r0 = blk[0], r1 = blk[1];
+            double2 xv = reinterpret_cast<const double2 *>(x)[i + off];
+            s0 += r0.x * xv.x;
+            s0 += r0.y * xv.y;
+            s1 += r1.x * xv.x;
+            s1 += r1.y * xv.y;
+        }
+        reinterpret_cast<double2 *>(y)[i] = make_double2(s0, s1);
+        if (partA) {
+            double2 av = reinterpret_cast<const double2 *>(a)[i];
+            accA += av.x * s0 + av.y * s1;
+        }
+        if (partB) {
+            double2 bv = reinterpret_cast<const double2 *>(b)[i];
+            accB += bv.x * s0 + bv.y * s1;
+        }
+    }
+    if (partA) {
+        double t = block_sum(accA);
+        if (threadIdx.x == 0) partA[blockIdx.x] = t;
+    }
+    if (partB) {
+        double t = block_sum(accB);
+        if (threadIdx.x == 0) partB[blockIdx.x] = t;
+    }
+}
+
+// ----------------------------------------------------------------------------------
+// BiCGStab (bicgstabsolver.hh:116-326) with CombinedCriterion (combinedcriterion.hh)
+// ----------------------------------------------------------------------------------
+constexpr double kBreakdownEps = 2.2250738585072014e-308 * 1e10;  // DBL_MIN * 1e10
+
+__global__ void __launch_bounds__(kRedThreads)
+k_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v, double *partMax,
+            double *partDot) {
+    double m = 0.0, d = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double bi = b[i];
+        r[i] = bi;
+        x[i] = 0.0;
+        p[i] = 0.0;
+        v[i] = 0.0;
+        m = fmax(m, fabs(bi));
+        d += bi * bi;
+    }
+    double tm = block_max(m);
+    double td = block_sum(d);
+    if (threadIdx.x == 0) {
+        partMax[blockIdx.x] = tm;
+        partDot[blockIdx.x] = td;
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_ks_init(KrylovState *ks, const double *partMax, const double *partDot, double rel_tol,
+          double abs_tol, double max_resid) {
+    double m = reduce_partials_max(partMax);
+    __syncthreads();
+    double d = reduce_partials_sum(partDot);
+    if (threadIdx.x == 0) {
+        ks->rel_tol = rel_tol;
+        ks->abs_tol = abs_tol;
+        ks->max_resid = max_resid;
+        double e = fmax(m, 2.2250738585072014e-308 * 1e10);
+        ks->resid_err = e;
+        ks->init_err = e;
+        ks->last_err = e;
+        ks->stagnates = 0;
+        ks->rho = 1.0;
+        ks->alpha = 1.0;
+        ks->omega = 1.0;
+        ks->rho_i = d;
+        ks->iterations = 0;
+        double red = e / fmax(1e-20, e);
+        ks->status = (red <= rel_tol || e <= abs_tol) ? KS_CONVERGED : KS_RUNNING;
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_ks_beta(KrylovState *ks, const double *partRho) {
+    if (ks->status != KS_RUNNING) return;
+    double rho_i = ks->iterations == 0 ? ks->rho_i : reduce_partials_sum(partRho);
+    if (threadIdx.x == 0) {
+        if (fabs(ks->rho) <= kBreakdownEps || fabs(ks->omega) <= kBreakdownEps) {
+            ks->status = KS_BREAKDOWN;
+            return;
+        }
+        ks->beta = (rho_i / ks->rho) * (ks->alpha / ks->omega);
+        ks->rho = rho_i;
+        ks->rho_i = rho_i;
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_p_update(int n, const KrylovState *ks, const double *r, const double *v, double *p) {
+    if (ks->status != KS_RUNNING) return;
+    const double omega = ks->omega, beta = ks->beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double tmp = v[i];
+        tmp *= omega;
+        tmp -= p[i];
+        tmp *= -beta;
+        double pi = r[i];
+        pi += tmp;
+        p[i] = pi;
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_ks_alpha(KrylovState *ks, const double *partDen) {
+    if (ks->status != KS_RUNNING) return;
+    double den = reduce_partials_sum(partDen);
+    if (threadIdx.x == 0) {
+        ks->denom = den;
+        if (fabs(den) <= kBreakdownEps) {
+            ks->status = KS_BREAKDOWN;
+            return;
+        }
+        double alpha = ks->rho / den;
+        ks->alpha = alpha;
+        if (fabs(alpha) <= kBreakdownEps) ks->status = KS_BREAKDOWN;
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_hs_update(int n, const KrylovState *ks, const double *y, const double *v, double *x, double *r,
+            double *partMax, int *anyNonzero) {
+    if (ks->status != KS_RUNNING) return;
+    const double alpha = ks->alpha;
+    double m = 0.0;
+    bool nz = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double yi = y[i];
+        double tmp = yi;
+        tmp *= alpha;
+        tmp += x[i];
+        x[i] = tmp;
+        tmp = v[i];
+        tmp *= alpha;
+        double ri = r[i] - tmp;
+        r[i] = ri;
+        m = fmax(m, fabs(ri));
+        nz |= (yi != 0.0);
+    }
+    double tm = block_max(m);
+    if (threadIdx.x == 0) partMax[blockIdx.x] = tm;
+    if (__any(nz) && (threadIdx.x & 63) == 0) atomicOr(anyNonzero, 1);
+}
+
+__device__ __forceinline__ void criterion(KrylovState *ks, double err, int stagnates) {
+    ks->last_err = ks->resid_err;
+    ks->resid_err = err;
+    ks->stagnates = stagnates;
+    double red = err / fmax(1e-20, ks->init_err);
+    bool conv = red <= ks->rel_tol || err <= ks->abs_tol;
+    bool failed = !conv && (stagnates || err > ks->max_resid);
+    if (conv) ks->status = KS_CONVERGED;
+    else if (failed) ks->status = KS_FAILED;
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_ks_crit1(KrylovState *ks, const double *partMax, int *anyNonzero) {
+    if (ks->status != KS_RUNNING) return;
+    double m = reduce_partials_max(partMax);
+    if (threadIdx.x == 0) {
+        criterion(ks, m, *anyNonzero == 0);
+        *anyNonzero = 0;
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_ks_omega(KrylovState *ks, const double *partTT, const double *partTS) {
+    if (ks->status != KS_RUNNING) return;
+    double tt = reduce_partials_sum(partTT);
+    __syncthreads();
+    double ts = reduce_partials_sum(partTS);
+    if (threadIdx.x == 0) {
+        if (fabs(tt) <= kBreakdownEps) {
+            ks->status = KS_BREAKDOWN;
+            return;
+        }
+        double omega = ts / tt;
+        ks->omega = omega;
+        if (fabs(omega) <= kBreakdownEps) ks->status = KS_BREAKDOWN;
+    }
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+k_xr_update(int n, const KrylovState *ks, const double *z, const double *t, double *x, double *r,
+            const double *r0hat, double *partRho, int *anyNonzero) {
+    if (ks->status != KS_RUNNING) return;
+    const double omega = ks->omega;
+    double d = 0.0;
+    bool nz = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double zi = z[i];
+        x[i] += omega * zi;
+        double ri = r[i] + -omega * t[i];
+        r[i] = ri;
+        d += r0hat[i] * ri;
+        nz |= (zi != 0.0);
+    }
+    double td = block_sum(d);
+    if (threadIdx.x == 0) partRho[blockIdx.x] = td;
+    if (__any(nz) && (threadIdx.x & 63) == 0) atomicOr(anyNonzero, 1);
+}
+
+__global__ void k_ks_crit2(KrylovState *ks, int *anyNonzero) {
+    if (ks->status != KS_RUNNING) return;
+    // the residual checked here is still s (the object r is updated afterwards)
+    criterion(ks, ks->resid_err, *anyNonzero == 0);
+    *anyNonzero = 0;
+    if (ks->status == KS_RUNNING) ks->iterations += 1;
+}
+
+// Newton helpers
+__global__ void __launch_bounds__(kRedThreads) k_max_abs(int n, const double *x, double *part) {
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+    double t = block_max(m);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ void __launch_bounds__(kRedThreads) k_reduce_max(const double *part, double *out) {
+    double m = reduce_partials_max(part);
+    if (threadIdx.x == 0) *out = m;
+}
+__global__ void __launch_bounds__(kRedThreads) k_reduce_sum(const double *part, double *out) {
+    double m = reduce_partials_sum(part);
+    if (threadIdx.x == 0) *out = m;
+}
+__global__ void __launch_bounds__(kRedThreads)
+k_newton_update(int n, const double *cur, const double *dx, double *x, double *part) {
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double d = dx[i];
+        double nv = cur[i];
+        nv -= d;
+        x[i] = nv;
+        s += fabs(d);
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+void launch_ilu_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
+                         hipStream_t s) {
+    int threads = level_threads(g), blocks = (threads + 255) / 256;
+    for (int l = 0; l < g.nlevels; ++l)
+        hipLaunchKernelGGL(k_ilu_factor_level<double>, dim3(blocks), dim3(256), 0, s, g, l, A, L,
+                           Dinv, static_cast<double *>(nullptr), err);
+}
+void launch_ilu_factor32(const Grid &g, const double *A, float *L, float *Dinv, float *U,
+                         int *err, hipStream_t s) {
+    int threads = level_threads(g), blocks = (threads + 255) / 256;
+    for (int l = 0; l < g.nlevels; ++l)
+        hipLaunchKernelGGL(k_ilu_factor_level<float>, dim3(blocks), dim3(256), 0, s, g, l, A, L,
+                           Dinv, U, err);
+}
+void launch_ilu_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
+                        const double *d, double *v, const int *status, hipStream_t s) {
+    int threads = level_threads(g), blocks = (threads + 255) / 256;
+    for (int l = 0; l < g.nlevels; ++l)
+        hipLaunchKernelGGL(k_ilu_fwd_level<double>, dim3(blocks), dim3(256), 0, s, g, l, L, d, v,
+                           status);
+    for (int l = g.nlevels - 1; l >= 0; --l)
+        hipLaunchKernelGGL((k_ilu_bwd_level<double, double>), dim3(blocks), dim3(256), 0, s, g, l,
+                           A, g.diag + 1, Dinv, v, v, status);
+}
+void launch_ilu_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
+                        const double *d, float *w, double *v, const int *status,
+                        hipStream_t s) {
+    int threads = level_threads(g), blocks = (threads + 255) / 256;
+    for (int l = 0; l < g.nlevels; ++l)
+        hipLaunchKernelGGL(k_ilu_fwd_level<float>, dim3(blocks), dim3(256), 0, s, g, l, L, d, w,
+                           status);
+    for (int l = g.nlevels - 1; l >= 0; --l)
+        hipLaunchKernelGGL((k_ilu_bwd_level<float, float>), dim3(blocks), dim3(256), 0, s, g, l, U,
+                           0, Dinv, w, v, status);
+}
+
+void launch_spmv(const Grid &g, const double *A, const double *x, double *y, const double *a,
+                 double *partA, const double *b, double *partB, const int *status,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_spmv, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, A, x, y, a, partA, b,
+                       partB, status);
+}
+
+void launch_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v,
+                      double *partMax, double *partDot, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_init, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, b, r, x, p, v,
+                       partMax, partDot);
+}
+void launch_ks_init(KrylovState *ks, const double *partMax, const double *partDot,
+                    double rel_tol, double abs_tol, double max_resid, hipStream_t s) {
+    hipLaunchKernelGGL(k_ks_init, dim3(1), dim3(kRedThreads), 0, s, ks, partMax, partDot, rel_tol,
+                       abs_tol, max_resid);
+}
+void launch_ks_beta(KrylovState *ks, const double *partRho, hipStream_t s) {
+    hipLaunchKernelGGL(k_ks_beta, dim3(1), dim3(kRedThreads), 0, s, ks, partRho);
+}
+void launch_p_update(int n, const KrylovState *ks, const double *r, const double *v, double *p,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_p_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, r, v, p);
+}
+void launch_ks_alpha(KrylovState *ks, const double *partDen, hipStream_t s) {
+    hipLaunchKernelGGL(k_ks_alpha, dim3(1), dim3(kRedThreads), 0, s, ks, partDen);
+}
+void launch_hs_update(int n, const KrylovState *ks, const double *y, const double *v, double *x,
+                      double *r, double *partMax, int *anyNonzero, hipStream_t s) {
+    hipLaunchKernelGGL(k_hs_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, y, v, x, r,
+                       partMax, anyNonzero);
+}
+void launch_ks_crit1(KrylovState *ks, const double *partMax, int *anyNonzero, hipStream_t s) {
+    hipLaunchKernelGGL(k_ks_crit1, dim3(1), dim3(kRedThreads), 0, s, ks, partMax, anyNonzero);
+}
+void launch_ks_omega(KrylovState *ks, const double *partTT, const double *partTS, hipStream_t s) {
+    hipLaunchKernelGGL(k_ks_omega, dim3(1), dim3(kRedThreads), 0, s, ks, partTT, partTS);
+}
+void launch_xr_update(int n, const KrylovState *ks, const double *z, const double *t, double *x,
+                      double *r, const double *r0hat, double *partRho, int *anyNonzero,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_xr_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, z, t, x, r,
+                       r0hat, partRho, anyNonzero);
+}
+void launch_ks_crit2(KrylovState *ks, int *anyNonzero, hipStream_t s) {
+    hipLaunchKernelGGL(k_ks_crit2, dim3(1), dim3(1), 0, s, ks, anyNonzero);
+}
+void launch_max_abs(int n, const double *x, double *part, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_abs, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, x, part);
+}
+void launch_reduce_max(const double *part, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kRedThreads), 0, s, part, out);
+}
+void launch_reduce_sum(const double *part, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s, part, out);
+}
+void launch_newton_update(int n, const double *cur, const double *dx, double *x, double *part,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, cur, dx, x,
+                       part);
+}
+
+}  // namespace ewg

@@ -1,0 +1,61 @@
+"""ewg -- Python view of the MI355X-native ECFV two-phase Newton step (libewg.so).
+
+The compute path is the HIP library built from csrc/; this module only loads it and
+passes host arrays through its C ABI. Importing load() on a machine without the built
+library raises: there is no CPU fallback in the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from .capi import Context, Params, SolverError, Stats, bind, default_params  # noqa: F401
+from . import problems  # noqa: F401
+
+_LIB = None
+
+
+def library_path() -> str:
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    return os.path.join(here, "lib", "libewg.so")
+
+
+def load():
+    """Load libewg.so (the HIP build); raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"libewg.so not built at {path}; run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        bind(lib, "ewg")
+        vp = C.c_void_p
+        lib.ewg_time_kernel.restype = C.c_int
+        lib.ewg_time_kernel.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        lib.ewg_reset_stats.restype = C.c_int
+        lib.ewg_reset_stats.argtypes = [vp]
+        lib.ewg_device_info.restype = C.c_int
+        lib.ewg_device_info.argtypes = [C.c_char_p, C.c_int]
+        _LIB = lib
+    return _LIB
+
+
+class GpuContext(Context):
+    """libewg context for a problem (see ewg.problems)."""
+
+    def __init__(self, problem, params: Params | None = None):
+        super().__init__(load(), "ewg", problem, params)
+
+    def time_kernel(self, which: int, reps: int) -> float:
+        ms = C.c_double(0.0)
+        self._check(self.lib.ewg_time_kernel(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def reset_stats(self):
+        self._check(self.lib.ewg_reset_stats(self.h))
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(512)
+    load().ewg_device_info(buf, 512)
+    return buf.value.decode()

@@ -1,0 +1,202 @@
+/*
+ * ewg_capi.h -- C ABI of the MI355X-native ECFV two-phase Newton step.
+ *
+ * The library (libewg.so) replaces two compile-time plugins of eWoms'
+ * ewoms-numerics for the immiscible two-phase model on a structured grid:
+ *
+ *   - the linearizer  FvBaseLinearizer<TypeTag>
+ *       (ewoms/numerics/discretizations/common/fvbaselinearizer.hh:62-571)
+ *     together with the AD local linearizer it drives
+ *       (ewoms/numerics/discretizations/common/fvbaseadlocallinearizer.hh:91-308);
+ *   - the linear-solver backend ParallelBiCGStabSolverBackend<TypeTag>
+ *       (ewoms/numerics/linear/parallelbicgstabbackend.hh:81-163,
+ *        ewoms/numerics/linear/parallelbasebackend.hh:156-440)
+ *     with BiCGStab (ewoms/numerics/linear/bicgstabsolver.hh:116-326),
+ *     CombinedCriterion (ewoms/numerics/linear/combinedcriterion.hh:51-232)
+ *     and block ILU(0) (ewoms/numerics/linear/istlpreconditionerwrappers.hh:155-193).
+ *
+ * Conventions
+ *   - Plain C types only; no pointer passed in is retained after the call.
+ *   - Status: 0 = OK, > 0 = numerical issue (the host adapter maps it to
+ *     Ewoms::NumericalIssue), < 0 = API misuse. ewg_last_error() explains.
+ *   - Solution and residual vectors are AoS [p_w, S_w] per cell, cells in
+ *     lexicographic order (x fastest), like Dune::BlockVector<FieldVector<double,2>>.
+ *   - Jacobian blocks are 2x2 row-major [eq][pv], eq k = mass balance of phase k,
+ *     pv 0 = p_w, pv 1 = S_w (Dune::FieldMatrix<double,2,2>).
+ *   - One context per host thread; the context owns every device buffer.
+ */
+#ifndef EWG_CAPI_H
+#define EWG_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EWG_ABI_VERSION 1
+
+enum {
+    EWG_OK = 0,
+    EWG_NUMERICAL_ISSUE = 1,     /* breakdown, non-finite values, singular block */
+    EWG_NOT_CONVERGED = 2,       /* linear solver or Newton did not converge */
+    EWG_INVALID_ARGUMENT = -1,
+    EWG_DEVICE_ERROR = -2,
+    EWG_INVALID_STATE = -3
+};
+
+enum { EWG_LAW_VAN_GENUCHTEN = 0, EWG_LAW_BROOKS_COREY = 1 };
+
+enum { EWG_BC_NO_FLOW = 0, EWG_BC_FREE_FLOW = 1, EWG_BC_MASS_RATE = 2 };
+
+/* Structured Cartesian grid. dim = 2 uses (nx, ny) with nz = 1 and extrusion 1.
+ * Replaces the StructuredGridVanguard INTERVAL grid
+ * (ewoms/numerics/io/structuredgridvanguard.hh:136-175) with YaspGrid ordering. */
+typedef struct ewg_grid_desc {
+    int32_t dim;
+    int32_t nx, ny, nz;
+    double lx, ly, lz;
+} ewg_grid_desc;
+
+/* Two incompressible liquid phases: index 0 = wetting (water), 1 = non-wetting.
+ * Mirrors TwoPhaseImmiscibleFluidSystem<LiquidPhase, LiquidPhase>
+ * (tests/problems/lensproblem.hh:87-104) and the problem gravity
+ * (ewoms/numerics/common/multiphasebaseproblem.hh:393-398). */
+typedef struct ewg_fluid_desc {
+    double rho[2];
+    double mu[2];
+    double gravity[3];
+} ewg_fluid_desc;
+
+/* Material law EffToAbsLaw<Regularized{VanGenuchten,BrooksCorey}>.
+ * region_params[r*4 + 0..3]:
+ *   van Genuchten: alpha [1/Pa], n, Swr, Snr
+ *   Brooks-Corey:  pe [Pa] (used when no per-cell pe is given), lambda, Swr, Snr */
+typedef struct ewg_material_desc {
+    int32_t law;
+    int32_t nregions;
+    const double *region_params;
+} ewg_material_desc;
+
+/* Run-time parameters with the reference defaults
+ * (newtonmethod.hh:150-155, fvbasediscretization.hh:128,207,210,249,252,
+ *  parallelbasebackend.hh:449-514, parallelbicgstabbackend.hh:49). */
+typedef struct ewg_params {
+    double newton_tolerance;           /* NewtonTolerance = 1e-8 */
+    double newton_max_error;           /* NewtonMaxError = 1e100 */
+    int32_t newton_target_iterations;  /* NewtonTargetIterations = 10 */
+    int32_t newton_max_iterations;     /* NewtonMaxIterations = 18 */
+    double linear_tolerance;           /* LinearSolverTolerance = 1e-3 (relative) */
+    double linear_abs_tolerance;       /* LinearSolverAbsTolerance = -1 -> newton_tolerance/100 */
+    double linear_max_error;           /* LinearSolverMaxError = 1e7 */
+    int32_t linear_max_iterations;     /* LinearSolverMaxIterations = 1000 */
+    int32_t enable_storage_cache;      /* EnableStorageCache (lens problem: true) */
+    int32_t recycle_first_iteration_storage; /* FvBaseProblem::recycleFirstIterationStorage = true */
+    int32_t max_time_step_divisions;   /* MaxTimeStepDivisions = 10 */
+    double min_time_step_size;         /* MinTimeStepSize = 0 */
+    double max_time_step_size;         /* MaxTimeStepSize = inf */
+    int32_t precond_fp32;              /* 0: FP64 ILU0 (library default); 1: FP32 ILU0 factors
+                                          and preconditioner inside FP64 BiCGStab */
+    int32_t reserved[7];
+} ewg_params;
+
+/* Counters and timers of the last calls; the split follows
+ * FvBaseProblem::finalize (ewoms/numerics/discretizations/common/fvbaseproblem.hh:452-500). */
+typedef struct ewg_stats {
+    int64_t newton_iterations;      /* linearizations (NewtonMethod::numIterations summed) */
+    int64_t linear_iterations;      /* BiCGStab iterations summed */
+    int64_t linear_solves;
+    int64_t time_steps;             /* accepted time steps */
+    int64_t failed_steps;           /* Newton failures that triggered a dt division */
+    int32_t last_newton_iterations;
+    int32_t last_linear_iterations;
+    double last_newton_error;
+    double time_linearize_s;
+    double time_solve_s;
+    double time_update_s;
+} ewg_stats;
+
+typedef struct ewg_ctx ewg_ctx;
+
+void ewg_default_params(ewg_params *params);
+
+/* FvBaseLinearizer::init + ParallelBaseBackend ctor (fvbaselinearizer.hh:127-131,
+ * parallelbasebackend.hh:170-200). Allocates all device memory for the grid. */
+int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
+               const ewg_material_desc *material, const ewg_params *params,
+               ewg_ctx **ctx_out);
+void ewg_destroy(ewg_ctx *ctx);
+const char *ewg_last_error(const ewg_ctx *ctx);
+int ewg_set_params(ewg_ctx *ctx, const ewg_params *params);
+
+int64_t ewg_num_cells(const ewg_ctx *ctx);
+int64_t ewg_num_blocks(const ewg_ctx *ctx);          /* nnzb of the BCRS Jacobian */
+int64_t ewg_num_boundary_faces(const ewg_ctx *ctx);
+
+/* Static per-cell fields, the problem callbacks intrinsicPermeability / porosity /
+ * materialLawParams (tests/problems/lensproblem.hh:357-398). K is isotropic [m^2];
+ * region indexes material->region_params; pe (Brooks-Corey only) may be NULL. */
+int ewg_set_static(ewg_ctx *ctx, const double *K, const double *phi,
+                   const int32_t *region, const double *pe);
+
+/* Boundary conditions per boundary face, the problem callback boundary()
+ * (tests/problems/lensproblem.hh:464-533). Faces are grouped by side
+ * (x-, x+, y-, y+, z-, z+); within a side they are ordered lexicographically over the
+ * two remaining axes. values[2f+0..1] = (p_w, S_w) for EWG_BC_FREE_FLOW and
+ * (rate_w, rate_n) [kg/(m^2 s), negative = into the domain] for EWG_BC_MASS_RATE. */
+int ewg_set_boundary(ewg_ctx *ctx, const int32_t *type, const double *values);
+
+/* FvBaseDiscretization::solution(timeIdx) (fvbasediscretization.hh:1110-1117):
+ * timeIdx 0 = current iterate, 1 = previous time level. */
+int ewg_set_solution(ewg_ctx *ctx, int time_idx, const double *x_aos);
+int ewg_get_solution(ewg_ctx *ctx, int time_idx, double *x_aos);
+/* FvBaseDiscretization::advanceTimeLevel (fvbasediscretization.hh:1396-1407). */
+int ewg_advance_time_level(ewg_ctx *ctx);
+
+/* IstlSparseMatrixAdapter::reserve pattern (fvbaselinearizer.hh:303-340):
+ * rowptr[N+1], colidx[nnzb], rows in cell order, columns ascending. */
+int ewg_get_pattern(const ewg_ctx *ctx, int32_t *rowptr, int32_t *colidx);
+
+/* FvBaseLinearizer::linearizeDomain (fvbaselinearizer.hh:166-197): residual and
+ * Jacobian at solution(0) for time-step size dt. newton_iter selects the storage-cache
+ * semantics of fvbaselocalresidual.hh:538-575 (iteration 0 refreshes the cached
+ * storage of the previous time level). The results stay on the device; resid
+ * (2N doubles) and jac (nnzb*4 doubles, BCRS order) are optional host copies that
+ * reproduce FvBaseLinearizer::residual() / jacobian(). */
+int ewg_linearize(ewg_ctx *ctx, double dt, int newton_iter, double *resid, double *jac);
+
+/* ParallelBaseBackend::setMatrix + solve (parallelbasebackend.hh:290-338): solves
+ * J dx = R for the last linearization with ILU0-preconditioned BiCGStab from x0 = 0.
+ * Returns EWG_NOT_CONVERGED when the criterion is not met, EWG_NUMERICAL_ISSUE on
+ * breakdown. dx (2N doubles) is an optional host copy; iterations may be NULL. */
+int ewg_solve(ewg_ctx *ctx, double *dx, int32_t *iterations);
+
+/* NewtonMethod::apply (nonlinear/newtonmethod.hh:308-517) for one time step of size
+ * dt on the device, starting from solution(0). Returns 0 when converged,
+ * EWG_NOT_CONVERGED / EWG_NUMERICAL_ISSUE on failure (solution(0) then holds the
+ * last iterate; the caller restores it as FvBaseDiscretization::updateFailed does). */
+int ewg_newton(ewg_ctx *ctx, double dt, int32_t *iterations, double *final_error);
+
+/* FvBaseProblem::timeIntegration + Simulator::run time loop
+ * (fvbaseproblem.hh:507-600, utils/simulator.hh:660-831) from time t0 to t_end with
+ * initial step dt0; dt is capped by params.max_time_step_size. steps_out may be NULL. */
+int ewg_run(ewg_ctx *ctx, double t0, double t_end, double dt0, int32_t *steps_out);
+
+int ewg_get_stats(const ewg_ctx *ctx, ewg_stats *stats);
+int ewg_reset_stats(ewg_ctx *ctx);
+
+/* Measurement hook (no counterpart in the reference): launches one of the hot
+ * kernels `reps` times on the context's stream at the current state and returns the
+ * mean duration per launch measured with HIP events on that stream. */
+enum { EWG_KERNEL_LINEARIZE = 0, EWG_KERNEL_SPMV = 1, EWG_KERNEL_ILU_APPLY = 2,
+       EWG_KERNEL_ILU_FACTOR = 3, EWG_KERNEL_BICGSTAB_ITER = 4 };
+int ewg_time_kernel(ewg_ctx *ctx, int which, int reps, double *mean_ms);
+
+/* Device description for run records (no counterpart in the reference). */
+int ewg_device_info(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EWG_CAPI_H */

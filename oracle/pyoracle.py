@@ -1,0 +1,117 @@
+"""ctypes loader for the CPU oracle libewo.so -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.join(os.path.dirname(_HERE), "ewoms-numerics_amd")
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+from ewg.capi import Context, bind  # noqa: E402
+
+_LIB = None
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+
+
+def library_path():
+    return os.path.join(_HERE, "libewo.so")
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(library_path()):
+            build()
+        lib = C.CDLL(library_path())
+        bind(lib, "ewo")
+        vp = C.c_void_p
+        for name, args in [("spmv", [vp, _D, _D, _D]), ("ilu0_factor", [vp, _D, _D]),
+                           ("ilu0_apply", [vp, _D, _D, _D]),
+                           ("bicgstab", [vp, _D, _D, _D, _I]), ("storage", [vp, _D, _D])]:
+            fn = getattr(lib, "ewo_" + name)
+            fn.restype = C.c_int
+            fn.argtypes = args
+        lib.ewo_step_log.restype = C.c_int
+        lib.ewo_step_log.argtypes = [vp, _I, _D, C.c_int32]
+        lib.ewo_material.restype = C.c_int
+        lib.ewo_material.argtypes = [vp, C.c_int, C.c_double, C.c_double, _D]
+        for name in ("log", "exp"):
+            fn = getattr(lib, "ewo_" + name)
+            fn.restype = C.c_double
+            fn.argtypes = [C.c_double]
+        lib.ewo_pow.restype = C.c_double
+        lib.ewo_pow.argtypes = [C.c_double, C.c_double]
+        _LIB = lib
+    return _LIB
+
+
+def _p(a, t=_D):
+    return a.ctypes.data_as(t)
+
+
+class OracleContext(Context):
+    def __init__(self, problem, params=None):
+        super().__init__(load(), "ewo", problem, params)
+
+    def spmv(self, A, x):
+        y = np.empty(2 * self.N)
+        self._check(self.lib.ewo_spmv(self.h, _p(A), _p(np.ascontiguousarray(x)), _p(y)))
+        return y
+
+    def ilu0_factor(self, A):
+        LU = np.empty_like(A)
+        self._check(self.lib.ewo_ilu0_factor(self.h, _p(A), _p(LU)))
+        return LU
+
+    def ilu0_apply(self, LU, d):
+        v = np.empty(2 * self.N)
+        self._check(self.lib.ewo_ilu0_apply(self.h, _p(LU), _p(np.ascontiguousarray(d)), _p(v)))
+        return v
+
+    def bicgstab(self, A, b):
+        x = np.empty(2 * self.N)
+        it = C.c_int32(0)
+        rc = self.lib.ewo_bicgstab(self.h, _p(A), _p(np.ascontiguousarray(b)), _p(x), C.byref(it))
+        return rc, x, it.value
+
+    def storage(self, x):
+        M = np.empty(2 * self.N)
+        self._check(self.lib.ewo_storage(self.h, _p(np.ascontiguousarray(x).ravel()), _p(M)))
+        return M
+
+    def step_log(self, cap=100000):
+        its = np.zeros(cap, np.int32)
+        dts = np.zeros(cap)
+        n = self.lib.ewo_step_log(self.h, _p(its, _I), _p(dts), cap)
+        return its[:n], dts[:n]
+
+    def material(self, region, Sw, pe=0.0):
+        out = np.zeros(9)
+        self._check(self.lib.ewo_material(self.h, region, pe, Sw, _p(out)))
+        return out
+
+
+def pow_(x, y):
+    return load().ewo_pow(x, y)
+
+
+def log_(x):
+    return load().ewo_log(x)
+
+
+def exp_(x):
+    return load().ewo_exp(x)

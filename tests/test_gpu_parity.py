@@ -1,0 +1,137 @@
+"""GPU parity: libewg (HIP, through its C ABI) against the CPU oracle on the same inputs.
+
+Bars: residual and Jacobian within 1e-12 relative (entry-wise, with a 1e-13 * max floor
+for cancellations), ILU(0) factors and sweeps within 1e-12, identical Newton iteration
+counts, saturations within 1e-8.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(P, params=None):
+    import ewg
+    from oracle.pyoracle import OracleContext
+    return ewg.GpuContext(P, params), OracleContext(P, params)
+
+
+def _lin_cases():
+    from ewg import problems
+    return [
+        ("lens48x32", problems.lens_problem()),
+        ("lens_ref_alpha", problems.lens_problem(nx=24, ny=16, outer_alpha=3.7e-3, mu_n=5.7e-4)),
+        ("het3d_12x10x8", problems.heterogeneous_3d(12, 10, 8, seed=11, patch=4)),
+        ("het3d_32x24x16", problems.heterogeneous_3d(32, 24, 16, seed=12, patch=8)),
+    ]
+
+
+@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("newton_iter", [0, 1])
+def test_linearize_parity(case, newton_iter):
+    from ewg import problems
+    name, P = _lin_cases()[case]
+    print(f"case {name} newton_iter {newton_iter}", flush=True)
+    gpu, ora = _pair(P)
+    x = problems.perturbed_state(P, seed=5 + case)
+    x1 = problems.perturbed_state(P, seed=50 + case, dS=0.1)
+    for c in (gpu, ora):
+        c.set_solution(x1, 1)
+        c.set_solution(x1, 0)
+        c.linearize(P.dt, 0, want=False)   # fills the storage cache from x1
+        c.set_solution(x, 0)
+    rg, jg = gpu.linearize(P.dt, newton_iter)
+    ro, jo = ora.linearize(P.dt, newton_iter)
+    assert rel_err(rg, ro) < 1e-12
+    assert rel_err(jg, jo) < 1e-12
+    rp_g, ci_g = gpu.pattern()
+    rp_o, ci_o = ora.pattern()
+    assert np.array_equal(rp_g, rp_o) and np.array_equal(ci_g, ci_o)
+
+
+def test_linear_solve_parity():
+    from ewg import problems
+    P = problems.lens_problem()
+    gpu, ora = _pair(P)
+    x = problems.perturbed_state(P, seed=9, dS=0.05, dp=50.0)
+    for c in (gpu, ora):
+        c.set_solution(P.x0, 1)
+        c.set_solution(P.x0, 0)
+        c.linearize(P.dt, 0, want=False)
+        c.set_solution(x, 0)
+        c.linearize(P.dt, 1, want=False)
+    rcg, dxg, itg = gpu.solve()
+    rco, dxo, ito = ora.solve()
+    print("iterations gpu/oracle", itg, ito, flush=True)
+    assert rcg == 0 and rco == 0
+    assert abs(itg - ito) <= 1
+    # both solves meet relTol 1e-10: the updates agree far below the Newton tolerance
+    assert np.max(np.abs(dxg - dxo)) <= 1e-6 * np.max(np.abs(dxo))
+
+
+def test_newton_lens_first_steps():
+    from ewg import problems
+    P = problems.lens_problem()
+    gpu, ora = _pair(P)
+    its_g, its_o = [], []
+    for step in range(4):
+        rcg, ig, eg = gpu.newton(P.dt)
+        rco, io, eo = ora.newton(P.dt)
+        print(f"step {step}: gpu rc {rcg} it {ig} err {eg:.3e} | oracle rc {rco} it {io} err {eo:.3e}",
+              flush=True)
+        assert rcg == 0 and rco == 0
+        its_g.append(ig)
+        its_o.append(io)
+        gpu.advance_time_level()
+        ora.advance_time_level()
+    assert its_g == its_o
+    dS = np.abs(gpu.solution(0)[1::2] - ora.solution(0)[1::2]).max()
+    assert dS < 1e-8
+
+
+def test_run_lens_3000s():
+    """The reference's own test length (CMakeLists.txt:73-74: --end-time=3000)."""
+    from ewg import problems
+    P = problems.lens_problem()
+    gpu, ora = _pair(P)
+    ng = gpu.run(0.0, 3000.0, P.dt)
+    no = ora.run(0.0, 3000.0, P.dt)
+    sg, so = gpu.stats(), ora.stats()
+    print("gpu", sg, "\noracle", so, flush=True)
+    assert ng == no
+    assert sg["newton_iterations"] == so["newton_iterations"]
+    xg, xo = gpu.solution(0), ora.solution(0)
+    assert np.abs(xg[1::2] - xo[1::2]).max() < 1e-8
+    assert rel_err(xg[0::2], xo[0::2]) < 1e-9
+
+
+def test_run_het3d_small():
+    from ewg import problems
+    P = problems.heterogeneous_3d(24, 20, 12, seed=3, patch=6, steps=5)
+    gpu, ora = _pair(P)
+    ng = gpu.run(0.0, P.t_end, P.dt)
+    no = ora.run(0.0, P.t_end, P.dt)
+    sg, so = gpu.stats(), ora.stats()
+    print("gpu", sg, "\noracle", so, flush=True)
+    assert ng == no
+    assert sg["newton_iterations"] == so["newton_iterations"]
+    xg, xo = gpu.solution(0), ora.solution(0)
+    assert np.abs(xg[1::2] - xo[1::2]).max() < 1e-8
+
+
+def test_fp32_preconditioner_close_to_fp64():
+    import ewg
+    from ewg import problems
+    from oracle.pyoracle import OracleContext
+    P = problems.heterogeneous_3d(24, 20, 12, seed=4, patch=6, steps=3)
+    p32 = ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3,
+                             max_time_step_size=P.dt, precond_fp32=1)
+    g32 = ewg.GpuContext(P, p32)
+    o64 = OracleContext(P)
+    g32.run(0.0, P.t_end, P.dt)
+    o64.run(0.0, P.t_end, P.dt)
+    dS = np.abs(g32.solution(0)[1::2] - o64.solution(0)[1::2]).max()
+    print("max |dS| fp32-precond vs fp64 oracle", dS, g32.stats(), flush=True)
+    assert dS < 1e-6
