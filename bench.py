@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-timesteps/s of the ECFV immiscible two-phase Newton step on one MI355X.
+
+Workload: the 3D heterogeneous DNAPL case, 128 x 128 x 64 cells (1,048,576 cells, 2x2
+BCRS blocks on a 7-point stencil), dt = 500 s, Newton tolerance 1e-6, BiCGStab + ILU(0)
+with the reference's default linear tolerances, FP64 throughout ("fp64"); the same run
+with FP32 ILU(0) factors inside the FP64 BiCGStab ("fp32_precond") is reported beside it.
+
+A step is one implicit-Euler time step driven to Newton convergence (linearization,
+ILU(0) factorization, BiCGStab, update, norms). W warm-up steps run first, the state is
+reset to the initial condition, and then exactly K steps from t = 0 are timed with the
+state resident in HBM (default K = 40, the whole configured run). value = cells*K/seconds.
+
+The CPU baseline is the C++ oracle (single thread, the reference's default
+ThreadsPerProcess = 1) timed on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "ewoms-numerics_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(P, nnzb):
+    """Bytes each kernel must move at least once per launch (FP64 values, implicit
+    structured column indices)."""
+    N = P.N
+    blk = 32
+    lower = (nnzb - N) // 2
+    per_cell_in = 16 + 8 + 8 + 16 + (8 if P.pe is not None else 0) + \
+        (4 if len(P.region_params) > 1 else 0)
+    return {
+        # x, K, phi, storage cache, [pe], [region] in; residual + all Jacobian blocks out
+        "linearize": N * (per_cell_in + 16) + nnzb * blk,
+        # all blocks + x in, y out
+        "spmv": nnzb * blk + 2 * 16 * N,
+        # forward: L + d in, w out; backward: U + Dinv + w in, w out
+        "ilu_apply": 2 * lower * blk + N * blk + 4 * 16 * N,
+        # A (lower, diagonal, upper) in; L + Dinv out
+        "ilu_factor": nnzb * blk + lower * blk + N * blk,
+    }
+
+
+def timed_run(ctx, P, warmup, steps):
+    """W untimed warm-up steps, then the state is reset to the initial condition and the
+    first K steps of the configured run are timed (the first step, where the pressure
+    field is established, is the most expensive one and stays in the timed region)."""
+    if warmup:
+        ctx.run(0.0, warmup * P.dt, P.dt)
+        ctx.set_solution(P.x0, 0)
+        ctx.set_solution(P.x0, 1)
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    n = ctx.run(0.0, steps * P.dt, P.dt)
+    t1 = time.perf_counter()
+    return n, t1 - t0, ctx.stats()
+
+
+def cpu_baseline(P, sample_steps):
+    from oracle.pyoracle import OracleContext
+    o = OracleContext(P)
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        rc, its, err = o.newton(P.dt)
+        if rc != 0:
+            raise RuntimeError(f"oracle Newton failed: rc {rc}")
+        o.advance_time_level()
+    el = time.perf_counter() - t0
+    s = o.stats()
+    return {"value": P.N * sample_steps / el, "unit": "cell-timesteps/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"first {sample_steps} time step(s) of the same 128x128x64 case "
+                       f"({s['newton_iterations']} linearizations, "
+                       f"{s['linear_iterations']} BiCGStab iterations) in {el:.1f} s"),
+            "cpu": platform.processor() or platform.machine()}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=128)
+    ap.add_argument("--ny", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--forms", default="fp64,fp32_precond")
+    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ewg
+    from ewg import problems
+    from ewg.capi import (EWG_KERNEL_ILU_APPLY, EWG_KERNEL_ILU_FACTOR, EWG_KERNEL_LINEARIZE,
+                          EWG_KERNEL_SPMV)
+
+    P = problems.heterogeneous_3d(args.nx, args.ny, args.nz)
+    print(f"# device {ewg.device_info()}; case {P.name} {args.nx}x{args.ny}x{args.nz} "
+          f"N={P.N}", file=sys.stderr, flush=True)
+    forms = {}
+    kernels = {}
+    for form in args.forms.split(","):
+        params = ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3,
+                                    max_time_step_size=P.dt,
+                                    precond_fp32=1 if form == "fp32_precond" else 0)
+        ctx = ewg.GpuContext(P, params)
+        n, el, st = timed_run(ctx, P, args.warmup, args.steps)
+        forms[form] = {"value": P.N * n / el, "steps": n, "seconds": el,
+                       "newton_iterations": st["newton_iterations"],
+                       "linear_iterations": st["linear_iterations"],
+                       "linear_solves": st["linear_solves"],
+                       "failed_steps": st["failed_steps"],
+                       "split_s": {"linearize": st["time_linearize_s"],
+                                   "solve": st["time_solve_s"],
+                                   "update": st["time_update_s"]}}
+        print(f"# {form}: {forms[form]}", file=sys.stderr, flush=True)
+        if form == "fp64":
+            nnzb = ctx.nnzb
+            ab = algorithmic_bytes(P, nnzb)
+            for name, kid in (("linearize", EWG_KERNEL_LINEARIZE), ("spmv", EWG_KERNEL_SPMV),
+                              ("ilu_apply", EWG_KERNEL_ILU_APPLY),
+                              ("ilu_factor", EWG_KERNEL_ILU_FACTOR)):
+                ms = ctx.time_kernel(kid, args.kernel_reps)
+                gbs = ab[name] / (ms * 1e-3) / 1e9
+                kernels[name] = {"mean_ms": ms, "bytes": ab[name], "achieved_GBs": gbs,
+                                 "frac": gbs / HBM_PEAK_GBS}
+            print(f"# kernels {kernels}", file=sys.stderr, flush=True)
+        ctx.close()
+
+    head = forms.get("fp64") or next(iter(forms.values()))
+    # dominant kernel of the solve: per BiCGStab iteration 2 ILU applies + 2 SpMV
+    dom_name = max(("linearize", "spmv", "ilu_apply"),
+                   key=lambda k: kernels[k]["mean_ms"] * (2 if k != "linearize" else 0.05)) \
+        if kernels else None
+    roofline = None
+    if dom_name:
+        k = kernels[dom_name]
+        roofline = {"kernel": dom_name, "bound": "hbm", "achieved": round(k["achieved_GBs"], 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(k["frac"], 4),
+                    "traffic": None}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(P, args.cpu_sample_steps)
+    line = {
+        "metric": "cell-timesteps/s to Newton tol 1e-6 (learned+FP32-precond vs plain FP64); HBM GB/s",
+        "value": head["value"],
+        "unit": "cell-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * head["seconds"] / max(1, head["steps"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded log-normal permeability, spectral synthesis)",
+        "config": {"workload": f"3D heterogeneous two-phase DNAPL injection {args.nx}x{args.ny}x{args.nz}"
+                               f" cells, dt {P.dt:g} s, Newton tol 1e-6, BiCGStab+ILU0",
+                   "form": "fp64", "cells": P.N, "seed": P.meta["seed"]},
+        "forms": forms,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

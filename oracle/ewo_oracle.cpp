@@ -342,6 +342,7 @@ struct ewo_ctx {
     ewg_stats stats{};
     std::vector<int32_t> step_its;
     std::vector<double> step_dts;
+    std::vector<double> newton_errors;  // error of every linearization of the last Newton
     std::string err;
     bool static_set = false;
 };
@@ -925,6 +926,7 @@ bool proceed(const ewo_ctx &c) {  // newtonmethod.hh:866-884
 // NewtonMethod::apply (newtonmethod.hh:308-517); returns EWG_OK or a failure code
 int newton(ewo_ctx &c, double dt) {
     c.num_iterations = 0;
+    c.newton_errors.clear();
     std::vector<double> cur;
     std::vector<double> dx;
     c.resid.assign(2 * c.N, 0.0);
@@ -938,6 +940,7 @@ int newton(ewo_ctx &c, double dt) {
             c.stats.newton_iterations++;
             c.last_error = c.error;  // preSolve_
             c.error = max_norm(c.resid);
+            c.newton_errors.push_back(c.error);
             if (c.error > c.prm.newton_max_error) throw NumericalIssue("Newton: error too large");
             if (!proceed(c)) {
                 ++c.num_iterations;
@@ -1246,6 +1249,13 @@ int ewo_step_log(const ewo_ctx *c, int32_t *newton_its, double *dts, int32_t cap
         if (newton_its) newton_its[i] = c->step_its[i];
         if (dts) dts[i] = c->step_dts[i];
     }
+    return n;
+}
+
+int ewo_newton_history(const ewo_ctx *c, double *errs, int32_t cap) {
+    if (!c) return 0;
+    int32_t n = std::min<int32_t>(cap, static_cast<int32_t>(c->newton_errors.size()));
+    for (int32_t i = 0; i < n; ++i) errs[i] = c->newton_errors[i];
     return n;
 }
 

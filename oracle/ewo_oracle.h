@@ -55,6 +55,8 @@ int ewo_run(ewo_ctx *ctx, double t0, double t_end, double dt0, int32_t *steps_ou
 /* per accepted step: newton iterations, dt; returns count written (<= cap) */
 int ewo_step_log(const ewo_ctx *ctx, int32_t *newton_its, double *dts, int32_t cap);
 int ewo_get_stats(const ewo_ctx *ctx, ewg_stats *stats);
+/* max-norm residual of every linearization of the last Newton solve */
+int ewo_newton_history(const ewo_ctx *ctx, double *errs, int32_t cap);
 
 /* pieces pinned by closed-form tests */
 double ewo_log(double x);

@@ -47,6 +47,8 @@ def load():
             fn.argtypes = args
         lib.ewo_step_log.restype = C.c_int
         lib.ewo_step_log.argtypes = [vp, _I, _D, C.c_int32]
+        lib.ewo_newton_history.restype = C.c_int
+        lib.ewo_newton_history.argtypes = [vp, _D, C.c_int32]
         lib.ewo_material.restype = C.c_int
         lib.ewo_material.argtypes = [vp, C.c_int, C.c_double, C.c_double, _D]
         for name in ("log", "exp"):
@@ -98,6 +100,11 @@ class OracleContext(Context):
         dts = np.zeros(cap)
         n = self.lib.ewo_step_log(self.h, _p(its, _I), _p(dts), cap)
         return its[:n], dts[:n]
+
+    def newton_history(self, cap=1000):
+        e = np.zeros(cap)
+        n = self.lib.ewo_newton_history(self.h, _p(e), cap)
+        return e[:n]
 
     def material(self, region, Sw, pe=0.0):
         out = np.zeros(9)
