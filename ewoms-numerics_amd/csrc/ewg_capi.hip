@@ -15,6 +15,7 @@
 
 #include "../../include/ewg_capi.h"
 #include "ewg_internal.h"
+#include "ewg_wavefront.h"
 
 using namespace ewg;
 
@@ -45,6 +46,12 @@ struct ewg_ctx {
     double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr;
     double *dscalar = nullptr;
     int *anyNonzero = nullptr, *ilu_err = nullptr;
+    int *partAny = nullptr;
+    // tile-wavefront synchronisation (3D grids)
+    unsigned *wf_ticket = nullptr;
+    unsigned long long *wf_ex = nullptr, *wf_ey = nullptr;
+    int2 *wf_order = nullptr;
+    WfSync wf{};
     KrylovState *ks = nullptr;
     // pinned host mirrors
     KrylovState *ks_host = nullptr;
@@ -87,7 +94,8 @@ void free_all(ewg_ctx *c) {
     void *ptrs[] = {c->x0, c->x1, c->xcur, c->sto, c->R, c->J, c->K, c->phi, c->pe, c->region,
                     c->regs, c->bc_type, c->bc_val, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
-                    c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks};
+                    c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks, c->partAny,
+                    c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ks_host) (void)hipHostFree(c->ks_host);
@@ -140,7 +148,16 @@ int device_max_abs(ewg_ctx *c, const double *v, double *out) {
     return EWG_OK;
 }
 
+bool use_wavefront(const ewg_ctx *c) { return c->g.dim == 3 && c->prm.ilu_schedule == 0; }
+
 void precond_factor(ewg_ctx *c) {
+    if (use_wavefront(c)) {
+        if (c->prm.precond_fp32)
+            wf_factor32(c->g, c->J, c->L32, c->D32, c->U32, c->ilu_err, c->wf, c->stream);
+        else
+            wf_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->wf, c->stream);
+        return;
+    }
     if (c->prm.precond_fp32)
         launch_ilu_factor32(c->g, c->J, c->L32, c->D32, c->U32, c->ilu_err, c->stream);
     else
@@ -148,6 +165,13 @@ void precond_factor(ewg_ctx *c) {
 }
 
 void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status) {
+    if (use_wavefront(c)) {
+        if (c->prm.precond_fp32)
+            wf_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->wf, c->stream);
+        else
+            wf_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->wf, c->stream);
+        return;
+    }
     if (c->prm.precond_fp32)
         launch_ilu_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->stream);
     else
@@ -180,19 +204,27 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
         precond_apply(c, c->kp, c->ky, st);
         launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
         launch_ks_alpha(c->ks, c->part0, s);
-        launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->anyNonzero, s);
-        launch_ks_crit1(c->ks, c->part1, c->anyNonzero, s);
+        launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s);
+        launch_ks_crit1(c->ks, c->part1, c->partAny, s);
         precond_apply(c, c->kr, c->kz, st);
         launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
         launch_ks_omega(c->ks, c->part0, c->part1, s);
-        launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->anyNonzero, s);
-        launch_ks_crit2(c->ks, c->anyNonzero, s);
+        launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
+        launch_ks_crit2(c->ks, c->partAny, s);
         HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
                                   s));
         HIP_TRY(c, hipStreamSynchronize(s));
         if (c->ks_host->status != KS_RUNNING) break;
     }
     *iters = c->ks_host->iterations;
+    HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (*c->err_host) {
+        // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
+        if (c->g.dim == 3) wf_init_edges(c->wf, s);
+        HIP_TRY(c, hipStreamSynchronize(s));
+        return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
+    }
     switch (c->ks_host->status) {
         case KS_CONVERGED: return EWG_OK;
         case KS_BREAKDOWN: return set_err(c, EWG_NUMERICAL_ISSUE, "Breakdown of the BiCGStab solver");
@@ -411,6 +443,34 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->anyNonzero, 1);
     ALLOC(c->ilu_err, 1);
     ALLOC(c->ks, 1);
+    ALLOC(c->partAny, kRedBlocks);
+    if (g.dim == 3) {
+        WfSync &w = c->wf;
+        w.ntx = (g.nx + kWfTX - 1) / kWfTX;
+        w.nty = (g.ny + kWfTY - 1) / kWfTY;
+        w.ntiles = w.ntx * w.nty;
+        ALLOC(c->wf_ticket, 1);
+        ALLOC(c->wf_order, w.ntiles);
+        w.edge_words = (size_t)w.ntiles * kWfTY * g.nz * kWfMaxWords;
+        ALLOC(c->wf_ex, w.edge_words);
+        ALLOC(c->wf_ey, w.edge_words);
+        std::vector<int2> order;
+        for (int d = 0; d <= w.ntx + w.nty - 2; ++d)
+            for (int tx = 0; tx < w.ntx; ++tx) {
+                int ty = d - tx;
+                if (ty >= 0 && ty < w.nty) order.push_back(make_int2(tx, ty));
+            }
+        if ((e = hipMemcpy(c->wf_order, order.data(), order.size() * sizeof(int2),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return fail("wf_order", e);
+        w.ticket = c->wf_ticket;
+        w.order = c->wf_order;
+        w.ex = c->wf_ex;
+        w.ey = c->wf_ey;
+        w.err = c->ilu_err;
+        wf_init_edges(w, c->stream);
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail("wf_init", e);
+    }
 #undef ALLOC
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->ks_host), sizeof(KrylovState))) != hipSuccess)
         return fail("ks_host", e);

@@ -99,12 +99,12 @@ void launch_p_update(int n, const KrylovState *ks, const double *r, const double
 void launch_ks_alpha(KrylovState *ks, const double *partDen, hipStream_t s);
 void launch_hs_update(int n, const KrylovState *ks, const double *y, const double *v, double *x,
                       double *r, double *partMax, int *anyNonzero, hipStream_t s);
-void launch_ks_crit1(KrylovState *ks, const double *partMax, int *anyNonzero, hipStream_t s);
+void launch_ks_crit1(KrylovState *ks, const double *partMax, const int *anyNonzero, hipStream_t s);
 void launch_ks_omega(KrylovState *ks, const double *partTT, const double *partTS, hipStream_t s);
 void launch_xr_update(int n, const KrylovState *ks, const double *z, const double *t, double *x,
                       double *r, const double *r0hat, double *partRho, int *anyNonzero,
                       hipStream_t s);
-void launch_ks_crit2(KrylovState *ks, int *anyNonzero, hipStream_t s);
+void launch_ks_crit2(KrylovState *ks, const int *anyNonzero, hipStream_t s);
 
 // Newton helpers
 void launch_max_abs(int n, const double *x, double *part, hipStream_t s);

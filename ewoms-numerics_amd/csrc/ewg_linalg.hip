@@ -396,8 +396,11 @@ k_hs_update(int n, const KrylovState *ks, const double *y, const double *v, doub
         nz |= (yi != 0.0);
     }
     double tm = block_max(m);
-    if (threadIdx.x == 0) partMax[blockIdx.x] = tm;
-    if (__any(nz) && (threadIdx.x & 63) == 0) atomicOr(anyNonzero, 1);
+    int bnz = __syncthreads_or(nz);
+    if (threadIdx.x == 0) {
+        partMax[blockIdx.x] = tm;
+        anyNonzero[blockIdx.x] = bnz;
+    }
 }
 
 __device__ __forceinline__ void criterion(KrylovState *ks, double err, int stagnates) {
@@ -411,14 +414,18 @@ __device__ __forceinline__ void criterion(KrylovState *ks, double err, int stagn
     else if (failed) ks->status = KS_FAILED;
 }
 
+__device__ __forceinline__ int reduce_partials_any(const int *part) {
+    int v = 0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) v |= part[i];
+    return __syncthreads_or(v);
+}
+
 __global__ void __launch_bounds__(kRedThreads)
-k_ks_crit1(KrylovState *ks, const double *partMax, int *anyNonzero) {
+k_ks_crit1(KrylovState *ks, const double *partMax, const int *anyNonzero) {
     if (ks->status != KS_RUNNING) return;
     double m = reduce_partials_max(partMax);
-    if (threadIdx.x == 0) {
-        criterion(ks, m, *anyNonzero == 0);
-        *anyNonzero = 0;
-    }
+    int any = reduce_partials_any(anyNonzero);
+    if (threadIdx.x == 0) criterion(ks, m, any == 0);
 }
 
 __global__ void __launch_bounds__(kRedThreads)
@@ -454,16 +461,21 @@ k_xr_update(int n, const KrylovState *ks, const double *z, const double *t, doub
         nz |= (zi != 0.0);
     }
     double td = block_sum(d);
-    if (threadIdx.x == 0) partRho[blockIdx.x] = td;
-    if (__any(nz) && (threadIdx.x & 63) == 0) atomicOr(anyNonzero, 1);
+    int bnz = __syncthreads_or(nz);
+    if (threadIdx.x == 0) {
+        partRho[blockIdx.x] = td;
+        anyNonzero[blockIdx.x] = bnz;
+    }
 }
 
-__global__ void k_ks_crit2(KrylovState *ks, int *anyNonzero) {
+__global__ void __launch_bounds__(kRedThreads) k_ks_crit2(KrylovState *ks, const int *anyNonzero) {
     if (ks->status != KS_RUNNING) return;
-    // the residual checked here is still s (the object r is updated afterwards)
-    criterion(ks, ks->resid_err, *anyNonzero == 0);
-    *anyNonzero = 0;
-    if (ks->status == KS_RUNNING) ks->iterations += 1;
+    int any = reduce_partials_any(anyNonzero);
+    if (threadIdx.x == 0) {
+        // the residual checked here is still s (the object r is updated afterwards)
+        criterion(ks, ks->resid_err, any == 0);
+        if (ks->status == KS_RUNNING) ks->iterations += 1;
+    }
 }
 
 // Newton helpers
@@ -567,7 +579,7 @@ void launch_hs_update(int n, const KrylovState *ks, const double *y, const doubl
     hipLaunchKernelGGL(k_hs_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, y, v, x, r,
                        partMax, anyNonzero);
 }
-void launch_ks_crit1(KrylovState *ks, const double *partMax, int *anyNonzero, hipStream_t s) {
+void launch_ks_crit1(KrylovState *ks, const double *partMax, const int *anyNonzero, hipStream_t s) {
     hipLaunchKernelGGL(k_ks_crit1, dim3(1), dim3(kRedThreads), 0, s, ks, partMax, anyNonzero);
 }
 void launch_ks_omega(KrylovState *ks, const double *partTT, const double *partTS, hipStream_t s) {
@@ -579,8 +591,8 @@ void launch_xr_update(int n, const KrylovState *ks, const double *z, const doubl
     hipLaunchKernelGGL(k_xr_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, z, t, x, r,
                        r0hat, partRho, anyNonzero);
 }
-void launch_ks_crit2(KrylovState *ks, int *anyNonzero, hipStream_t s) {
-    hipLaunchKernelGGL(k_ks_crit2, dim3(1), dim3(1), 0, s, ks, anyNonzero);
+void launch_ks_crit2(KrylovState *ks, const int *anyNonzero, hipStream_t s) {
+    hipLaunchKernelGGL(k_ks_crit2, dim3(1), dim3(kRedThreads), 0, s, ks, anyNonzero);
 }
 void launch_max_abs(int n, const double *x, double *part, hipStream_t s) {
     hipLaunchKernelGGL(k_max_abs, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, x, part);

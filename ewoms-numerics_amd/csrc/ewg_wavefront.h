@@ -1,0 +1,43 @@
+// ewg_wavefront.h -- persistent tile-wavefront launches for the 3D ILU(0) recurrences.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ewg_internal.h"
+
+namespace ewg {
+
+constexpr int kWfTX = 8;   // tile extent in x (lanes)
+constexpr int kWfTY = 8;   // tile extent in y
+constexpr int kWfPD = 8;   // prefetch depth (local steps) of each lane's operands
+constexpr int kWfPDFactor = 4;  // factorization operands are twice as wide
+
+constexpr int kWfMaxWords = 4;  // 64-bit words per exchanged value (FP64 2x2 block)
+
+// synchronisation state: the ticket counter is reset by a one-thread kernel in front of
+// every launch; the edge buffers hold the sentinel between launches
+struct WfSync {
+    unsigned *ticket;
+    const int2 *order;         // [ntiles] tiles (tx, ty) in anti-diagonal order
+    unsigned long long *ex;    // [ntiles][TY][nz][kWfMaxWords] last-column values
+    unsigned long long *ey;    // [ntiles][TX][nz][kWfMaxWords] last-row values
+    size_t edge_words;         // words in each of ex, ey
+    int ntx, nty, ntiles;
+    int *err;                  // set when a wait exceeded its spin limit
+};
+
+// fill both edge buffers with the sentinel (context creation, and after an error)
+void wf_init_edges(const WfSync &sync, hipStream_t s);
+
+void wf_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
+                 const WfSync &sync, hipStream_t s);
+void wf_factor32(const Grid &g, const double *A, float *L, float *Dinv, float *U, int *err,
+                 const WfSync &sync, hipStream_t s);
+void wf_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
+                const double *d, double *v, const int *status, const WfSync &sync,
+                hipStream_t s);
+void wf_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
+                const double *d, float *w, double *v, const int *status, const WfSync &sync,
+                hipStream_t s);
+
+}  // namespace ewg

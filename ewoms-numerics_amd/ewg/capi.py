@@ -60,7 +60,8 @@ class Params(C.Structure):
                 ("min_time_step_size", C.c_double),
                 ("max_time_step_size", C.c_double),
                 ("precond_fp32", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("ilu_schedule", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class Stats(C.Structure):
@@ -98,6 +99,7 @@ def default_params(**overrides) -> Params:
     p.min_time_step_size = 0.0
     p.max_time_step_size = float("inf")
     p.precond_fp32 = 0
+    p.ilu_schedule = 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

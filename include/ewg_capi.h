@@ -97,7 +97,9 @@ typedef struct ewg_params {
     double max_time_step_size;         /* MaxTimeStepSize = inf */
     int32_t precond_fp32;              /* 0: FP64 ILU0 (library default); 1: FP32 ILU0 factors
                                           and preconditioner inside FP64 BiCGStab */
-    int32_t reserved[7];
+    int32_t ilu_schedule;              /* 0: tile wavefront on 3D grids, hyperplane launches
+                                          otherwise; 1: hyperplane launches (same results) */
+    int32_t reserved[6];
 } ewg_params;
 
 /* Counters and timers of the last calls; the split follows
