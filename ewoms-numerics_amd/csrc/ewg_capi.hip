@@ -52,6 +52,11 @@ struct ewg_ctx {
     unsigned long long *wf_ex = nullptr, *wf_ey = nullptr;
     int2 *wf_order = nullptr;
     WfSync wf{};
+    // the same for 16x16 tiles (schedule 3)
+    unsigned *wf16_ticket = nullptr;
+    unsigned long long *wf16_ex = nullptr, *wf16_ey = nullptr;
+    int2 *wf16_order = nullptr;
+    WfSync wf16{};
     KrylovState *ks = nullptr;
     // pinned host mirrors
     KrylovState *ks_host = nullptr;
@@ -95,7 +100,8 @@ void free_all(ewg_ctx *c) {
                     c->regs, c->bc_type, c->bc_val, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
                     c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks, c->partAny,
-                    c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order};
+                    c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wf16_ticket, c->wf16_ex,
+                    c->wf16_ey, c->wf16_order};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ks_host) (void)hipHostFree(c->ks_host);
@@ -148,7 +154,9 @@ int device_max_abs(ewg_ctx *c, const double *v, double *out) {
     return EWG_OK;
 }
 
-bool use_wavefront(const ewg_ctx *c) { return c->g.dim == 3 && c->prm.ilu_schedule == 0; }
+bool use_wavefront(const ewg_ctx *c) {
+    return c->g.dim == 3 && c->prm.ilu_schedule != 1 && c->g.N <= kWfMaxCells;
+}
 
 void precond_factor(ewg_ctx *c) {
     if (use_wavefront(c)) {
@@ -166,10 +174,20 @@ void precond_factor(ewg_ctx *c) {
 
 void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status) {
     if (use_wavefront(c)) {
+        const int sched = c->prm.ilu_schedule;
+        if (sched == 2) {
+            if (c->prm.precond_fp32)
+                wf_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->wf, c->stream);
+            else
+                wf_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->wf, c->stream);
+            return;
+        }
+        const WfSync &w = sched == 3 ? c->wf16 : c->wf;
+        const int tile = sched == 3 ? 16 : 8;
         if (c->prm.precond_fp32)
-            wf_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->wf, c->stream);
+            sweep_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, w, tile, c->stream);
         else
-            wf_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->wf, c->stream);
+            sweep_apply64(c->g, c->L64, c->D64, c->J, d, v, status, w, tile, c->stream);
         return;
     }
     if (c->prm.precond_fp32)
@@ -221,7 +239,10 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     HIP_TRY(c, hipStreamSynchronize(s));
     if (*c->err_host) {
         // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
-        if (c->g.dim == 3) wf_init_edges(c->wf, s);
+        if (c->wf.ex) {
+            wf_init_edges(c->wf, s);
+            wf_init_edges(c->wf16, s);
+        }
         HIP_TRY(c, hipStreamSynchronize(s));
         return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
     }
@@ -360,6 +381,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     if (material->nregions < 1 || !material->region_params) return EWG_INVALID_ARGUMENT;
     if (material->law != EWG_LAW_VAN_GENUCHTEN && material->law != EWG_LAW_BROOKS_COREY)
         return EWG_INVALID_ARGUMENT;
+    if (params->ilu_schedule < 0 || params->ilu_schedule > 3) return EWG_INVALID_ARGUMENT;
     auto *c = new ewg_ctx();
     c->grid = *grid;
     c->fluid = *fluid;
@@ -444,32 +466,39 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->ilu_err, 1);
     ALLOC(c->ks, 1);
     ALLOC(c->partAny, kRedBlocks);
-    if (g.dim == 3) {
-        WfSync &w = c->wf;
-        w.ntx = (g.nx + kWfTX - 1) / kWfTX;
-        w.nty = (g.ny + kWfTY - 1) / kWfTY;
-        w.ntiles = w.ntx * w.nty;
-        ALLOC(c->wf_ticket, 1);
-        ALLOC(c->wf_order, w.ntiles);
-        w.edge_words = (size_t)w.ntiles * kWfTY * g.nz * kWfMaxWords;
-        ALLOC(c->wf_ex, w.edge_words);
-        ALLOC(c->wf_ey, w.edge_words);
-        std::vector<int2> order;
-        for (int d = 0; d <= w.ntx + w.nty - 2; ++d)
-            for (int tx = 0; tx < w.ntx; ++tx) {
-                int ty = d - tx;
-                if (ty >= 0 && ty < w.nty) order.push_back(make_int2(tx, ty));
-            }
-        if ((e = hipMemcpy(c->wf_order, order.data(), order.size() * sizeof(int2),
-                           hipMemcpyHostToDevice)) != hipSuccess)
-            return fail("wf_order", e);
-        w.ticket = c->wf_ticket;
-        w.order = c->wf_order;
-        w.ex = c->wf_ex;
-        w.ey = c->wf_ey;
-        w.err = c->ilu_err;
-        wf_init_edges(w, c->stream);
-        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail("wf_init", e);
+    if (g.dim == 3 && N <= kWfMaxCells) {
+        auto setup = [&](WfSync &w, int T, unsigned *&ticket, int2 *&dorder,
+                         unsigned long long *&ex, unsigned long long *&ey) -> int {
+            w.ntx = (g.nx + T - 1) / T;
+            w.nty = (g.ny + T - 1) / T;
+            w.ntiles = w.ntx * w.nty;
+            ALLOC(ticket, 1);
+            ALLOC(dorder, w.ntiles);
+            w.edge_words = (size_t)w.ntiles * T * g.nz * kWfMaxWords;
+            ALLOC(ex, w.edge_words);
+            ALLOC(ey, w.edge_words);
+            std::vector<int2> order;
+            for (int d = 0; d <= w.ntx + w.nty - 2; ++d)
+                for (int tx = 0; tx < w.ntx; ++tx) {
+                    int ty = d - tx;
+                    if (ty >= 0 && ty < w.nty) order.push_back(make_int2(tx, ty));
+                }
+            if ((e = hipMemcpy(dorder, order.data(), order.size() * sizeof(int2),
+                               hipMemcpyHostToDevice)) != hipSuccess)
+                return fail("wf_order", e);
+            w.ticket = ticket;
+            w.order = dorder;
+            w.ex = ex;
+            w.ey = ey;
+            w.err = c->ilu_err;
+            wf_init_edges(w, c->stream);
+            if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail("wf_init", e);
+            return EWG_OK;
+        };
+        int rc = setup(c->wf, kWfTX, c->wf_ticket, c->wf_order, c->wf_ex, c->wf_ey);
+        if (rc != EWG_OK) return rc;
+        rc = setup(c->wf16, 16, c->wf16_ticket, c->wf16_order, c->wf16_ex, c->wf16_ey);
+        if (rc != EWG_OK) return rc;
     }
 #undef ALLOC
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->ks_host), sizeof(KrylovState))) != hipSuccess)
@@ -535,6 +564,8 @@ const char *ewg_last_error(const ewg_ctx *c) { return c ? c->err.c_str() : "null
 
 int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
     if (!c || !p) return EWG_INVALID_ARGUMENT;
+    if (p->ilu_schedule < 0 || p->ilu_schedule > 3)
+        return set_err(c, EWG_INVALID_ARGUMENT, "ilu_schedule must be 0..3");
     if (p->precond_fp32) {
         int rc = ensure_fp32(c);
         if (rc) return rc;

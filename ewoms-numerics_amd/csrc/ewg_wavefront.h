@@ -11,8 +11,13 @@ constexpr int kWfTX = 8;   // tile extent in x (lanes)
 constexpr int kWfTY = 8;   // tile extent in y
 constexpr int kWfPD = 8;   // prefetch depth (local steps) of each lane's operands
 constexpr int kWfPDFactor = 4;  // factorization operands are twice as wide
+constexpr int kSweepPD = 6;     // prefetch depth of the triangular sweeps' operands
+constexpr int kSweepPE = 1;     // prefetch depth of their cross-tile words
 
 constexpr int kWfMaxWords = 4;  // 64-bit words per exchanged value (FP64 2x2 block)
+// largest grid the tile-wavefront kernels take: their 32-bit buffer offsets (32 bytes per
+// cell and array, plus a skew margin and the out-of-range base 2^31) must not wrap
+constexpr long long kWfMaxCells = 1LL << 25;
 
 // synchronisation state: the ticket counter is reset by a one-thread kernel in front of
 // every launch; the edge buffers hold the sentinel between launches
@@ -39,5 +44,14 @@ void wf_apply64(const Grid &g, const double *L, const double *Dinv, const double
 void wf_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
                 const double *d, float *w, double *v, const int *status, const WfSync &sync,
                 hipStream_t s);
+
+// the same sweeps by the instruction-lean kernel of ewg_sweep.hip; tile = 8 (8x8, one
+// wavefront per tile) or 16 (16x16, four wavefronts)
+void sweep_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
+                   const double *d, double *v, const int *status, const WfSync &sync,
+                   int tile, hipStream_t s);
+void sweep_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
+                   const double *d, float *w, double *v, const int *status, const WfSync &sync,
+                   int tile, hipStream_t s);
 
 }  // namespace ewg

@@ -97,8 +97,12 @@ typedef struct ewg_params {
     double max_time_step_size;         /* MaxTimeStepSize = inf */
     int32_t precond_fp32;              /* 0: FP64 ILU0 (library default); 1: FP32 ILU0 factors
                                           and preconditioner inside FP64 BiCGStab */
-    int32_t ilu_schedule;              /* 0: tile wavefront on 3D grids, hyperplane launches
-                                          otherwise; 1: hyperplane launches (same results) */
+    int32_t ilu_schedule;              /* ILU(0) schedule on 3D grids (2D grids always use
+                                          hyperplane launches); every choice gives the same
+                                          results: 0 tile wavefront, 8x8 tiles (default);
+                                          1 one launch per hyperplane x+y+z = const;
+                                          2 tile wavefront with the generic sweep kernel;
+                                          3 tile wavefront, 16x16 tiles for the sweeps */
     int32_t reserved[6];
 } ewg_params;
 
