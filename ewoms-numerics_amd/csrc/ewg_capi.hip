@@ -52,11 +52,9 @@ struct ewg_ctx {
     unsigned long long *wf_ex = nullptr, *wf_ey = nullptr;
     int2 *wf_order = nullptr;
     WfSync wf{};
-    // the same for 16x16 tiles (schedule 3)
-    unsigned *wf16_ticket = nullptr;
-    unsigned long long *wf16_ex = nullptr, *wf16_ey = nullptr;
-    int2 *wf16_order = nullptr;
-    WfSync wf16{};
+    // ILU(0) factors in the tiles' wavefront layout (schedule 0)
+    WlPrecond<double> wl64{};
+    WlPrecond<float> wl32{};
     KrylovState *ks = nullptr;
     // pinned host mirrors
     KrylovState *ks_host = nullptr;
@@ -100,8 +98,8 @@ void free_all(ewg_ctx *c) {
                     c->regs, c->bc_type, c->bc_val, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
                     c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks, c->partAny,
-                    c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wf16_ticket, c->wf16_ex,
-                    c->wf16_ey, c->wf16_order};
+                    c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wl64.L, c->wl64.U,
+                    c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ks_host) (void)hipHostFree(c->ks_host);
@@ -158,12 +156,25 @@ bool use_wavefront(const ewg_ctx *c) {
     return c->g.dim == 3 && c->prm.ilu_schedule != 1 && c->g.N <= kWfMaxCells;
 }
 
+// schedule 0: the sweeps read the factors in the wavefront layout (its buffers exist
+// when the layout fits 32-bit byte offsets; otherwise the single-wavefront tile sweep)
+bool use_wavefront_layout(const ewg_ctx *c) {
+    return use_wavefront(c) && c->prm.ilu_schedule == 0 && c->wl64.L &&
+           (!c->prm.precond_fp32 || c->wl32.L);
+}
+
 void precond_factor(ewg_ctx *c) {
     if (use_wavefront(c)) {
         if (c->prm.precond_fp32)
             wf_factor32(c->g, c->J, c->L32, c->D32, c->U32, c->ilu_err, c->wf, c->stream);
         else
             wf_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->wf, c->stream);
+        if (use_wavefront_layout(c)) {
+            if (c->prm.precond_fp32)
+                wl_pack32(c->g, c->wf, c->L32, c->D32, c->U32, c->wl32, c->stream);
+            else
+                wl_pack64(c->g, c->wf, c->L64, c->D64, c->J, c->wl64, c->stream);
+        }
         return;
     }
     if (c->prm.precond_fp32)
@@ -173,6 +184,13 @@ void precond_factor(ewg_ctx *c) {
 }
 
 void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status) {
+    if (use_wavefront_layout(c)) {
+        if (c->prm.precond_fp32)
+            wl_apply32(c->g, c->wf, c->wl32, d, v, status, c->stream);
+        else
+            wl_apply64(c->g, c->wf, c->wl64, d, v, status, c->stream);
+        return;
+    }
     if (use_wavefront(c)) {
         const int sched = c->prm.ilu_schedule;
         if (sched == 2) {
@@ -182,12 +200,12 @@ void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status) {
                 wf_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->wf, c->stream);
             return;
         }
-        const WfSync &w = sched == 3 ? c->wf16 : c->wf;
-        const int tile = sched == 3 ? 16 : 8;
+        const int variant = 8;  // single-wavefront tiles, natural layout
         if (c->prm.precond_fp32)
-            sweep_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, w, tile, c->stream);
+            sweep_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->wf, variant,
+                          c->stream);
         else
-            sweep_apply64(c->g, c->L64, c->D64, c->J, d, v, status, w, tile, c->stream);
+            sweep_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->wf, variant, c->stream);
         return;
     }
     if (c->prm.precond_fp32)
@@ -239,10 +257,7 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     HIP_TRY(c, hipStreamSynchronize(s));
     if (*c->err_host) {
         // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
-        if (c->wf.ex) {
-            wf_init_edges(c->wf, s);
-            wf_init_edges(c->wf16, s);
-        }
+        if (c->wf.ex) wf_init_edges(c->wf, s);
         HIP_TRY(c, hipStreamSynchronize(s));
         return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
     }
@@ -269,6 +284,13 @@ int ensure_fp32(ewg_ctx *c) {
     HIP_TRY(c, dalloc(&c->D32, N * 4));
     HIP_TRY(c, dalloc(&c->U32, (size_t)c->g.dim * N * 4));
     HIP_TRY(c, dalloc(&c->w32, 2 * N));
+    if (c->wl64.L) {
+        const size_t E = wl_entries(c->g, c->wf);
+        HIP_TRY(c, dalloc(&c->wl32.L, 3 * E * 4));
+        HIP_TRY(c, dalloc(&c->wl32.U, 3 * E * 4));
+        HIP_TRY(c, dalloc(&c->wl32.D, E * 4));
+        HIP_TRY(c, dalloc(&c->wl32.w, E * 2));
+    }
     return EWG_OK;
 }
 
@@ -497,8 +519,14 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
         };
         int rc = setup(c->wf, kWfTX, c->wf_ticket, c->wf_order, c->wf_ex, c->wf_ey);
         if (rc != EWG_OK) return rc;
-        rc = setup(c->wf16, 16, c->wf16_ticket, c->wf16_order, c->wf16_ex, c->wf16_ey);
-        if (rc != EWG_OK) return rc;
+        // the wavefront layout must fit 32-bit byte offsets (32 bytes per FP64 block)
+        const size_t E = wl_entries(g, c->wf);
+        if (E * 32 <= 0x7FFFFF00ull) {
+            ALLOC(c->wl64.L, 3 * E * 4);
+            ALLOC(c->wl64.U, 3 * E * 4);
+            ALLOC(c->wl64.D, E * 4);
+            ALLOC(c->wl64.w, E * 2);
+        }
     }
 #undef ALLOC
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->ks_host), sizeof(KrylovState))) != hipSuccess)

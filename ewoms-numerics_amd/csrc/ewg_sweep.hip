@@ -147,6 +147,24 @@ struct SweepArgs {
     WfSync sync;
 };
 
+// arguments of the warp-specialized sweep, whose factors live in the wavefront layout:
+// entry ((tile * nsteps + step) * 64 + lane) holds the cell that lane of that tile
+// reaches at that local step (forward order for L and the forward result, backward
+// order for U and Dinv; positions without a cell hold zeros)
+template <class T>
+struct WsArgs {
+    int nx, ny, nz, nxny, N;
+    int nsteps;           // local steps per tile: 14 + nz
+    unsigned wl_bytes_blk, wl_bytes_vec;  // byte sizes of one layout array of blocks / pairs
+    const double *d;      // forward: right-hand side, natural order
+    const T *blk[3];      // forward: L (-z,-y,-x); backward: U (+x,+y,+z)
+    const T *Dinv;        // backward
+    T *w;                 // forward result (forward layout): written forward, read backward
+    double *out;          // backward: preconditioned vector, natural order
+    const int *status;
+    WfSync sync;
+};
+
 template <class T, bool FWD>
 struct In {
     P2<T> rhs;
@@ -370,54 +388,521 @@ __global__ void __launch_bounds__(TX *TY) k_sweep(SweepArgs<T> a) {
     }
 }
 
+// ----------------------------------------------------------------------------------
+// Warp-specialized sweep: one 8x8 tile per workgroup of three wavefronts.
+//   wave 0 computes the tile's columns as k_sweep does, but reads the values of the
+//          upstream tiles from an LDS ring instead of global memory, so that its vector
+//          memory counter only ever holds operand prefetches issued PD steps earlier and
+//          its own stores (a late cross-tile load would make every wait on it also wait
+//          for all older loads and stores);
+//   wave 1 polls the upstream tiles' edge words (16 edge lanes x 4 z-values in flight),
+//          copies each value into its ring slot once the compute wave has moved past the
+//          value the slot held, publishes how many values of each edge lane are in place,
+//          and recycles the global words to the sentinel.
+//   wave 2 publishes the tile's own last column and row: the compute wave leaves those
+//          values in an LDS outbox and counts its steps, and wave 2 stores them to the
+//          edge buffers (coherent stores to words other tiles are polling are slow to
+//          complete, and would hold up every later wait of the compute wave).
+// The compute wave checks each step against counts it read one step earlier (so the
+// check does not expose the LDS latency) and announces its progress once per PD steps.
+// ----------------------------------------------------------------------------------
+constexpr int kRing = 16;  // ring slots per edge lane (z modulo kRing)
+constexpr int kOut = 32;   // outbox slots per downstream edge lane
+constexpr unsigned kLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
+
+__device__ __forceinline__ int lds_ld(const int *p) {
+    return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_st(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <class T, bool FWD, int PD>
+__global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
+    constexpr int TX = 8, TY = 8, NT = 64;
+    static_assert(PD < kRing, "the edge wave must be able to run ahead of the compute wave");
+    __shared__ P2<T> xch[2][NT];
+    __shared__ P2<T> ering[2][8][kRing];  // [x edge | y edge][edge lane][z % kRing]
+    __shared__ P2<T> outbox[2][8][kOut];  // values for the downstream tiles, by z % kOut
+    __shared__ int filled[16];            // values of each edge lane in place (z < filled)
+    __shared__ int progress;              // compute steps completed (per PD steps)
+    __shared__ int stepdone;              // compute steps completed (every step)
+    __shared__ int pubstep;               // outputs of steps below it are published
+    __shared__ int dummy_i[NT];
+    __shared__ P2<T> dummy_v[NT];
+    __shared__ unsigned s_tk;
+    if (a.status && *a.status != KS_RUNNING) return;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tk = atomicAdd(a.sync.ticket, 1u);
+    __syncthreads();
+    const unsigned tk = s_tk;
+    if (tk >= (unsigned)a.sync.ntiles) return;
+    const int2 tile = a.sync.order[tk];
+    const int tx = tile.x, ty = tile.y;
+    const int nz = a.nz;
+    const int tileId = ty * a.sync.ntx + tx;
+    // edge lanes without an upstream tile read zeros for good (the matching blocks are
+    // zero as well) and count as complete
+    for (int i = tid; i < 2 * 8 * kRing; i += 192) (&ering[0][0][0])[i] = P2<T>{T(0), T(0)};
+    for (int i = tid; i < 2 * NT; i += 192) (&xch[0][0])[i] = P2<T>{T(0), T(0)};
+    if (tid < 16) {
+        const int dir = tid >> 3, e = tid & 7;
+        const bool valid = dir == 0 ? (tx > 0 && ty * TY + e < a.ny) : (ty > 0 && tx * TX + e < a.nx);
+        filled[tid] = valid ? 0 : nz;
+    }
+    if (tid == 0) {
+        progress = 0;
+        stepdone = 0;
+        pubstep = 0;
+    }
+    __syncthreads();
+    constexpr unsigned zw = kWfMaxWords * 8;  // bytes per z of one edge lane
+    const __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
+    const __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
+
+    if (tid >= 2 * NT) {
+        // ----------------------------------------------------------- publisher wave
+        // lanes 0..7 forward the tile's last column (x edge rows), 8..15 its last row
+        const int l = tid - 2 * NT, e = l & 15, dir = e >> 3, idx = e & 7;
+        const bool valid = l < 16 && (dir == 0 ? (tx + 1 < a.sync.ntx && ty * TY + idx < a.ny)
+                                               : (ty + 1 < a.sync.nty && tx * TX + idx < a.nx));
+        const unsigned gout = !valid ? 0u : (unsigned)((((size_t)tileId * 8 + idx) * nz) * zw);
+        const P2<T> *box = &outbox[dir][idx][0];
+        int zp = valid ? 0 : nz;  // next z to publish; value z comes from step z + 7 + idx
+        unsigned idle = 0;
+        int shown = 0;
+        while (true) {
+            // all outputs of steps below this lane's next step are out
+            int ps = zp < nz ? zp + 7 + idx : 0x3fffffff;
+            ps = min(ps, __shfl_xor(ps, 1));
+            ps = min(ps, __shfl_xor(ps, 2));
+            ps = min(ps, __shfl_xor(ps, 4));
+            ps = min(ps, __shfl_xor(ps, 8));
+            if (l == 0 && ps > shown) {
+                lds_st(&pubstep, ps);
+                shown = ps;
+            }
+            if (__builtin_amdgcn_ballot_w64(zp < nz) == 0ull) break;
+            const int done = lds_ld(&stepdone);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const bool ready = zp < nz && zp + 7 + idx < done;
+            const P2<T> v = box[zp & (kOut - 1)];
+            ste<T>(rex, ready && dir == 0 ? gout + (unsigned)zp * zw : kFar, val_ew(v));
+            ste<T>(rey, ready && dir == 1 ? gout + (unsigned)zp * zw : kFar, val_ew(v));
+            zp += ready ? 1 : 0;
+            if (__builtin_amdgcn_ballot_w64(ready) != 0ull) {
+                idle = 0;
+            } else {
+                if (++idle >= kSpinLimit) {
+                    atomicExch(a.sync.err, 1);
+                    break;
+                }
+                if ((idle & 63) == 0 &&
+                    __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        return;
+    }
+    if (tid >= NT) {
+        // ---------------------------------------------------------------- edge wave
+        const int l = tid - NT, e = l & 15, j = l >> 4;
+        const int dir = e >> 3, idx = e & 7;
+        const bool valid = dir == 0 ? (tx > 0 && ty * TY + idx < a.ny) : (ty > 0 && tx * TX + idx < a.nx);
+        const unsigned gbase =
+            !valid ? 0u
+            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nz) * zw)
+                       : (unsigned)((((size_t)(tileId - a.sync.ntx) * TX + idx) * nz) * zw);
+        P2<T> *ring = &ering[dir][idx][0];
+        EW<T> sent;
+        sent.w[0] = kSentinel;
+        if constexpr (sizeof(T) == 8) sent.w[1] = kSentinel;
+        // sub-lane j delivers the values z = j (mod 4) of its edge lane in order; the
+        // count of values in place is the smallest z any of the four still lacks
+        int z = valid ? j : nz;
+        unsigned idle = 0;
+        int published = valid ? 0 : nz;
+        while (__builtin_amdgcn_ballot_w64(z < nz) != 0ull) {
+            const bool mine = z < nz;
+            // the slot held z - kRing, consumed once the compute wave has completed the
+            // step that read it (step z - kRing + idx)
+            const bool room = z - kRing + idx < lds_ld(&progress);
+            const unsigned go = mine ? gbase + (unsigned)z * zw : kFar;
+            const EW<T> vx = lde<T>(rex, dir == 0 ? go : kFar);
+            const EW<T> vy = lde<T>(rey, dir == 1 ? go : kFar);
+            const EW<T> v = dir == 0 ? vx : vy;
+            const bool deliver = mine && room && !is_sentinel(v);
+            if (deliver) {
+                ring[z & (kRing - 1)] = ew_val(v);
+                ste<T>(rex, dir == 0 ? go : kFar, sent);
+                ste<T>(rey, dir == 1 ? go : kFar, sent);
+                z += 4;
+            }
+            // smallest missing z over the four sub-lanes (lanes e, e+16, e+32, e+48)
+            int zmin = z < nz ? z : nz;
+            zmin = min(zmin, __shfl_xor(zmin, 16));
+            zmin = min(zmin, __shfl_xor(zmin, 32));
+            const bool advance = j == 0 && zmin > published;
+            if (__builtin_amdgcn_ballot_w64(advance) != 0ull) {
+                // the ring values land before the count (each wave's LDS operations
+                // execute in order; the fences keep the compiler from reordering them)
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(kLgkm0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (advance) lds_st(&filled[e], zmin);
+                published = zmin > published ? zmin : published;
+            }
+            if (__builtin_amdgcn_ballot_w64(deliver) != 0ull) {
+                idle = 0;
+            } else {
+                if (++idle >= kSpinLimit) {
+                    atomicExch(a.sync.err, 1);
+                    break;
+                }
+                if ((idle & 63) == 0 &&
+                    __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        return;
+    }
+
+    // -------------------------------------------------------------------- compute wave
+    const int lx = tid % TX, ly = tid / TX;
+    const int xs = tx * TX + lx, ys = ty * TY + ly;
+    const bool inplane = xs < a.nx && ys < a.ny;
+    const bool prodX = inplane && lx == TX - 1 && tx + 1 < a.sync.ntx;
+    const bool prodY = inplane && ly == TY - 1 && ty + 1 < a.sync.nty;
+
+    // natural-order cell of (xs, ys, z) = c0 + z * dz in sweep coordinates
+    int c0, dz;
+    if (FWD) {
+        c0 = xs + a.nx * ys;
+        dz = a.nxny;
+    } else {
+        c0 = (a.nx - 1 - xs) + a.nx * (a.ny - 1 - ys) + a.nxny * (nz - 1);
+        dz = -a.nxny;
+    }
+    const int cs0 = c0 - (lx + ly) * dz;  // cell of local step 0
+    const int nsteps = a.nsteps;
+    constexpr unsigned bB = 4 * sizeof(T), vB = 2 * sizeof(T);
+    // wavefront-layout arrays: this lane's entry of local step 0, then 64 entries on
+    const unsigned wl0 = (unsigned)((tileId * nsteps) * 64 + tid);
+    unsigned pblk = wl0 * bB;
+    constexpr unsigned sblk = 64 * bB;
+    // natural-order vectors (forward right-hand side, backward result)
+    unsigned pnat = inplane ? (unsigned)(cs0 * 16) : kFar;
+    const unsigned snat = inplane ? (unsigned)(dz * 16) : 0u;
+    unsigned onat = pnat;
+    // forward result: written in the forward layout by the forward sweep; the backward
+    // sweep reads entry (forward tile, z + lxf + lyf, forward lane) of its cell, which
+    // moves back by one row of 64 per step
+    unsigned ow = wl0 * vB;  // forward: store offset
+    unsigned pw = kFar;      // backward: load offset
+    if (!FWD && inplane) {
+        const int x = a.nx - 1 - xs, y = a.ny - 1 - ys;  // natural coordinates
+        const int z0 = nz - 1 + lx + ly;                  // natural z at local step 0
+        const int lxf = x & 7, lyf = y & 7;
+        const int tf = (y >> 3) * a.sync.ntx + (x >> 3);
+        pw = (unsigned)(((tf * nsteps + z0 + lxf + lyf) * 64 + lxf + 8 * lyf) * (int)vB);
+    }
+    constexpr unsigned sw = 64 * vB;  // backward: the forward entry moves back by sw
+    const __amdgpu_buffer_rsrc_t rb0 = rsrc(a.blk[0], a.wl_bytes_blk),
+                                 rb1 = rsrc(a.blk[1], a.wl_bytes_blk),
+                                 rb2 = rsrc(a.blk[2], a.wl_bytes_blk);
+    const __amdgpu_buffer_rsrc_t rdi = rsrc(a.Dinv, FWD ? 0 : a.wl_bytes_blk);
+    const __amdgpu_buffer_rsrc_t rw = rsrc(a.w, a.wl_bytes_vec);
+    const size_t N = (size_t)a.N;
+    const __amdgpu_buffer_rsrc_t rd = rsrc(a.d, FWD ? N * 16 : 0);
+    const __amdgpu_buffer_rsrc_t rout = rsrc(a.out, FWD ? 0 : N * 16);
+
+    In<T, FWD> ring[PD];
+    auto load = [&](In<T, FWD> &in) {
+        if (FWD) {
+            P2<double> dv = ldp<double>(rd, pnat);
+            in.rhs = P2<T>{static_cast<T>(dv.x), static_cast<T>(dv.y)};
+            pnat += snat;
+        } else {
+            in.rhs = ldp<T>(rw, pw);
+            pw -= sw;
+        }
+        ldb<T>(rb0, pblk, in.b[0]);
+        ldb<T>(rb1, pblk, in.b[1]);
+        ldb<T>(rb2, pblk, in.b[2]);
+        if (!FWD) ldb<T>(rdi, pblk, in.di);
+        pblk += sblk;
+    };
+    // the scheduling barriers keep the prefetches in step order (see k_sweep)
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
+        load(ring[k]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    P2<T> prev = P2<T>{T(0), T(0)};
+    const bool hasL = lx > 0, hasD = ly > 0;
+    // outputs for the downstream tiles go to the outbox (the publisher wave stores them);
+    // other lanes write their own dummy slots, and one lane announces each finished step
+    P2<T> *const oX = prodX ? &outbox[0][ly][0] : &dummy_v[tid];
+    P2<T> *const oY = prodY ? &outbox[1][lx][0] : &dummy_v[tid];
+    const int oMaskX = prodX ? kOut - 1 : 0, oMaskY = prodY ? kOut - 1 : 0;
+    int *const sdone = tid == 0 ? &stepdone : &dummy_i[tid];
+    // neighbour sources: the in-tile lane of the previous step, or the ring slot of this
+    // lane's z (first column / row of the tile)
+    const P2<T> *const eX = &ering[0][ly][0];
+    const P2<T> *const eY = &ering[1][lx][0];
+    const int *const fX = &filled[ly];
+    const int *const fY = &filled[8 + lx];
+    const bool edgeX = lx == 0, edgeY = ly == 0;
+
+    // published counts, read one step ahead of the check that uses them
+    int cntX = lds_ld(fX), cntY = lds_ld(fY);
+    for (int tau0 = 0; tau0 < nsteps; tau0 += PD) {
+        // steps before tau0 are complete: their ring slots may be refilled (the compiler
+        // fence keeps the earlier steps' ring reads ahead of the announcement; LDS
+        // executes them in that order)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (tid == 0) lds_st(&progress, tau0);
+        // the outbox slots this block writes must have been published (z - kOut ago);
+        // the publisher only waits for this wave, so this rarely spins
+        {
+            bool full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
+            if (__builtin_amdgcn_ballot_w64(full) != 0ull) {
+                unsigned spins = 0;
+                do {
+                    __builtin_amdgcn_s_sleep(1);
+                    full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
+                } while (__builtin_amdgcn_ballot_w64(full) != 0ull && ++spins < kSpinLimit);
+                if (full) atomicExch(a.sync.err, 1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+#pragma unroll
+        for (int k = 0; k < PD; ++k) {
+            const int tau = tau0 + k;
+            const int z = tau - lx - ly;
+            const bool zin = (unsigned)z < (unsigned)nz;
+            const int rbuf = (tau + 1) & 1, wbuf = tau & 1;
+            const int slot = z & (kRing - 1);
+            // this step's ring values must be in place: value z of the lane's edge
+            bool wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
+            if (__builtin_amdgcn_ballot_w64(wait) != 0ull) {
+                unsigned spins = 0;
+                do {
+                    __builtin_amdgcn_s_sleep(1);
+                    cntX = lds_ld(fX);
+                    cntY = lds_ld(fY);
+                    wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
+                    ++spins;
+                    if ((spins & 63) == 0 &&
+                        __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                        break;
+                } while (__builtin_amdgcn_ballot_w64(wait) != 0ull && spins < kSpinLimit);
+                if (wait) atomicExch(a.sync.err, 1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const P2<T> nx_ = hasL ? xch[rbuf][tid - 1] : eX[slot];
+            const P2<T> ny_ = hasD ? xch[rbuf][tid - TX] : eY[slot];
+            P2<T> nz_;
+            nz_.x = z > 0 ? prev.x : T(0);
+            nz_.y = z > 0 ? prev.y : T(0);
+            const In<T, FWD> &c = ring[k];
+            T r0 = c.rhs.x, r1 = c.rhs.y;
+            P2<T> res;
+            if (FWD) {
+                mmv(c.b[0], nz_, r0, r1);
+                mmv(c.b[1], ny_, r0, r1);
+                mmv(c.b[2], nx_, r0, r1);
+                res = P2<T>{r0, r1};
+            } else {
+                mmv(c.b[0], nx_, r0, r1);
+                mmv(c.b[1], ny_, r0, r1);
+                mmv(c.b[2], nz_, r0, r1);
+                T s0 = T(0), s1 = T(0);
+                s0 += c.di[0] * r0; s0 += c.di[1] * r1;
+                s1 += c.di[2] * r0; s1 += c.di[3] * r1;
+                res = P2<T>{s0, s1};
+            }
+            // hand the downstream tiles' values to the publisher, then the result vector
+            oX[z & oMaskX] = res;
+            oY[z & oMaskY] = res;
+            if (FWD) {
+                // the last block's steps past nsteps would land in the next tile's rows
+                stp<T>(rw, tau < nsteps ? ow : kFar, res);
+                ow += sw;
+            } else {
+                stp<double>(rout, onat,
+                            P2<double>{static_cast<double>(res.x), static_cast<double>(res.y)});
+                onat += snat;
+            }
+            prev = res;
+            xch[wbuf][tid] = res;
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            lds_st(sdone, tau + 1);
+            __builtin_amdgcn_wave_barrier();
+            // counts for the next step's check
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            cntX = lds_ld(fX);
+            cntY = lds_ld(fY);
+            load(ring[k]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
 __global__ void k_sweep_reset_ticket(unsigned *ticket) { *ticket = 0u; }
+
+// Copy the factors into the wavefront layout: one thread per layout entry. FWD: L (3
+// lower slots, forward order); otherwise U (3 upper slots; UT is double for the
+// Jacobian's blocks) and Dinv (backward order). Entries without a cell get zeros.
+template <class T, class UT, bool FWD>
+__global__ void __launch_bounds__(256)
+k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t N, const T *L,
+          const UT *U0, const UT *U1, const UT *U2, const T *Dinv, T *o0, T *o1, T *o2, T *od) {
+    const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= entries) return;
+    const int lane = p & 63, q = p >> 6;
+    const int step = q % nsteps, tileId = q / nsteps;
+    const int lx = lane & 7, ly = lane >> 3;
+    int x = (tileId % ntx) * 8 + lx, y = (tileId / ntx) * 8 + ly, z = step - lx - ly;
+    const bool ok = x < nx && y < ny && z >= 0 && z < nz;
+    if (!FWD) {
+        x = nx - 1 - x;
+        y = ny - 1 - y;
+        z = nz - 1 - z;
+    }
+    const size_t cell = ok ? (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z) : 0;
+    auto put = [&](T *o, const T *srcT, const UT *srcU) {
+        T v[4] = {T(0), T(0), T(0), T(0)};
+        if (ok) {
+            for (int e = 0; e < 4; ++e)
+                v[e] = srcT ? srcT[cell * 4 + e] : static_cast<T>(srcU[cell * 4 + e]);
+        }
+        for (int e = 0; e < 4; ++e) o[(size_t)p * 4 + e] = v[e];
+    };
+    if (FWD) {
+        put(o0, L, nullptr);
+        put(o1, L + N * 4, nullptr);
+        put(o2, L + 2 * N * 4, nullptr);
+    } else {
+        put(o0, nullptr, U0);
+        put(o1, nullptr, U1);
+        put(o2, nullptr, U2);
+        put(od, Dinv, nullptr);
+    }
+}
 
 }  // namespace
 
 template <class T, bool FWD>
-static void launch_sweep(const SweepArgs<T> &a, int tiles_x, hipStream_t s) {
+static void launch_sweep(const SweepArgs<T> &a, int variant, hipStream_t s) {
     hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, a.sync.ticket);
-    if (tiles_x == 16)
-        hipLaunchKernelGGL((k_sweep<T, FWD, 16, 16, kSweepPD, kSweepPE>), dim3(a.sync.ntiles), dim3(256), 0, s, a);
+    if (variant == 0) return;  // the wavefront-layout path has its own launcher
     else
         hipLaunchKernelGGL((k_sweep<T, FWD, 8, 8, kSweepPD, kSweepPE>), dim3(a.sync.ntiles), dim3(64), 0, s, a);
 }
 
 void sweep_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                    const double *d, double *v, const int *status, const WfSync &sync,
-                   int tile, hipStream_t s) {
+                   int variant, hipStream_t s) {
     SweepArgs<double> f{};
     f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N;
     f.d = d;
     f.blk[0] = L; f.blk[1] = L + (size_t)g.N * 4; f.blk[2] = L + (size_t)2 * g.N * 4;
     f.blk_bytes = 32;
     f.Dinv = nullptr; f.w = v; f.out = nullptr; f.status = status; f.sync = sync;
-    launch_sweep<double, true>(f, tile, s);
+    launch_sweep<double, true>(f, variant, s);
     SweepArgs<double> b = f;
     b.d = nullptr;
     b.blk[0] = A + (size_t)(g.diag + 1) * g.N * 4;
     b.blk[1] = A + (size_t)(g.diag + 2) * g.N * 4;
     b.blk[2] = A + (size_t)(g.diag + 3) * g.N * 4;
     b.Dinv = Dinv;
-    launch_sweep<double, false>(b, tile, s);
+    launch_sweep<double, false>(b, variant, s);
 }
 
 void sweep_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
                    const double *d, float *w, double *v, const int *status, const WfSync &sync,
-                   int tile, hipStream_t s) {
+                   int variant, hipStream_t s) {
     SweepArgs<float> f{};
     f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N;
     f.d = d;
     f.blk[0] = L; f.blk[1] = L + (size_t)g.N * 4; f.blk[2] = L + (size_t)2 * g.N * 4;
     f.blk_bytes = 16;
     f.Dinv = nullptr; f.w = w; f.out = nullptr; f.status = status; f.sync = sync;
-    launch_sweep<float, true>(f, tile, s);
+    launch_sweep<float, true>(f, variant, s);
     SweepArgs<float> b = f;
     b.d = nullptr;
     b.blk[0] = U; b.blk[1] = U + (size_t)g.N * 4; b.blk[2] = U + (size_t)2 * g.N * 4;
     b.Dinv = Dinv;
     b.out = v;
-    launch_sweep<float, false>(b, tile, s);
+    launch_sweep<float, false>(b, variant, s);
+}
+
+size_t wl_entries(const Grid &g, const WfSync &sync) {
+    return (size_t)sync.ntiles * (size_t)(g.nz + 14) * 64;
+}
+
+template <class T, class UT>
+static void wl_pack(const Grid &g, const WfSync &sync, const T *L, const T *Dinv, const UT *U0,
+                    const UT *U1, const UT *U2, const WlPrecond<T> &wl, hipStream_t s) {
+    const unsigned entries = (unsigned)wl_entries(g, sync);
+    const int nsteps = g.nz + 14;
+    const dim3 grid((entries + 255) / 256);
+    hipLaunchKernelGGL((k_wl_pack<T, UT, true>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz, sync.ntx,
+                       nsteps, entries, (size_t)g.N, L, U0, U1, U2, Dinv, wl.L,
+                       wl.L + (size_t)entries * 4, wl.L + (size_t)2 * entries * 4, nullptr);
+    hipLaunchKernelGGL((k_wl_pack<T, UT, false>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz,
+                       sync.ntx, nsteps, entries, (size_t)g.N, L, U0, U1, U2, Dinv, wl.U,
+                       wl.U + (size_t)entries * 4, wl.U + (size_t)2 * entries * 4, wl.D);
+}
+
+void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
+               const double *A, const WlPrecond<double> &wl, hipStream_t s) {
+    const size_t N = g.N;
+    wl_pack<double, double>(g, sync, L, Dinv, A + (size_t)(g.diag + 1) * N * 4,
+                            A + (size_t)(g.diag + 2) * N * 4, A + (size_t)(g.diag + 3) * N * 4,
+                            wl, s);
+}
+
+void wl_pack32(const Grid &g, const WfSync &sync, const float *L, const float *Dinv,
+               const float *U, const WlPrecond<float> &wl, hipStream_t s) {
+    const size_t N = g.N;
+    wl_pack<float, float>(g, sync, L, Dinv, U, U + N * 4, U + 2 * N * 4, wl, s);
+}
+
+template <class T>
+static void wl_apply(const Grid &g, const WfSync &sync, const WlPrecond<T> &wl, const double *d,
+                     double *v, const int *status, hipStream_t s) {
+    const size_t entries = wl_entries(g, sync);
+    WsArgs<T> f{};
+    f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N;
+    f.nsteps = g.nz + 14;
+    f.wl_bytes_blk = (unsigned)(entries * 4 * sizeof(T));
+    f.wl_bytes_vec = (unsigned)(entries * 2 * sizeof(T));
+    f.d = d;
+    f.blk[0] = wl.L; f.blk[1] = wl.L + entries * 4; f.blk[2] = wl.L + 2 * entries * 4;
+    f.Dinv = nullptr; f.w = wl.w; f.out = nullptr; f.status = status; f.sync = sync;
+    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
+    hipLaunchKernelGGL((k_sweep_ws<T, true, kSweepPD>), dim3(sync.ntiles), dim3(192), 0, s, f);
+    WsArgs<T> b = f;
+    b.d = nullptr;
+    b.blk[0] = wl.U; b.blk[1] = wl.U + entries * 4; b.blk[2] = wl.U + 2 * entries * 4;
+    b.Dinv = wl.D;
+    b.out = v;
+    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
+    hipLaunchKernelGGL((k_sweep_ws<T, false, kSweepPD>), dim3(sync.ntiles), dim3(192), 0, s, b);
+}
+
+void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
+                double *v, const int *status, hipStream_t s) {
+    wl_apply<double>(g, sync, wl, d, v, status, s);
+}
+void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
+                double *v, const int *status, hipStream_t s) {
+    wl_apply<float>(g, sync, wl, d, v, status, s);
 }
 
 }  // namespace ewg

@@ -45,13 +45,36 @@ void wf_apply32(const Grid &g, const float *L, const float *Dinv, const float *U
                 const double *d, float *w, double *v, const int *status, const WfSync &sync,
                 hipStream_t s);
 
-// the same sweeps by the instruction-lean kernel of ewg_sweep.hip; tile = 8 (8x8, one
-// wavefront per tile) or 16 (16x16, four wavefronts)
+// the same sweeps by the single-wavefront tile kernel of ewg_sweep.hip on the 8x8 tiles
+// of sync (variant 8)
 void sweep_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                    const double *d, double *v, const int *status, const WfSync &sync,
-                   int tile, hipStream_t s);
+                   int variant, hipStream_t s);
 void sweep_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
                    const double *d, float *w, double *v, const int *status, const WfSync &sync,
-                   int tile, hipStream_t s);
+                   int variant, hipStream_t s);
+
+// The ILU(0) factors in the wavefront layout of the 8x8 tiles of sync: entry
+// ((tile * (nz + 14) + step) * 64 + lane) holds the 2x2 block of the cell that lane of
+// that tile reaches at that local step (L in forward order, U and Dinv in backward
+// order), so that each step of a tile reads one contiguous row of 64 blocks.
+template <class T>
+struct WlPrecond {
+    T *L;  // [3][entries][4]
+    T *U;  // [3][entries][4]
+    T *D;  // [entries][4]
+    T *w;  // [entries][2] forward-sweep result
+};
+size_t wl_entries(const Grid &g, const WfSync &sync);
+// copy freshly factored L, Dinv and the upper blocks (FP64: the Jacobian's; FP32: U)
+void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
+               const double *A, const WlPrecond<double> &wl, hipStream_t s);
+void wl_pack32(const Grid &g, const WfSync &sync, const float *L, const float *Dinv,
+               const float *U, const WlPrecond<float> &wl, hipStream_t s);
+// v = (LU)^-1 d with one compute, one edge and one publisher wavefront per tile
+void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
+                double *v, const int *status, hipStream_t s);
+void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
+                double *v, const int *status, hipStream_t s);
 
 }  // namespace ewg

@@ -99,10 +99,11 @@ typedef struct ewg_params {
                                           and preconditioner inside FP64 BiCGStab */
     int32_t ilu_schedule;              /* ILU(0) schedule on 3D grids (2D grids always use
                                           hyperplane launches); every choice gives the same
-                                          results: 0 tile wavefront, 8x8 tiles (default);
-                                          1 one launch per hyperplane x+y+z = const;
-                                          2 tile wavefront with the generic sweep kernel;
-                                          3 tile wavefront, 16x16 tiles for the sweeps */
+                                          results: 0 tile wavefront, 8x8 tiles with an edge
+                                          wavefront each (default); 1 one launch per
+                                          hyperplane x+y+z = const; 2 tile wavefront, generic
+                                          sweep kernel; 3 tile wavefront, one wavefront per
+                                          tile */
     int32_t reserved[6];
 } ewg_params;
 
