@@ -657,8 +657,21 @@ __global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
     const int *const fY = &filled[8 + lx];
     const bool edgeX = lx == 0, edgeY = ly == 0;
 
-    // published counts, read one step ahead of the check that uses them
+    // The counts and the neighbour values of a step are read at the end of the step
+    // before, so that their LDS latency overlaps that step's tail and the check; the
+    // counts come first (a ring value read after a count that covers it is in place),
+    // and the rare wait re-reads the values.
+    auto nbr = [&](int tau, P2<T> &vx, P2<T> &vy) {
+        const int z = tau - lx - ly;
+        const int rbuf = (tau + 1) & 1;
+        const int slot = z & (kRing - 1);
+        vx = hasL ? xch[rbuf][tid - 1] : eX[slot];
+        vy = hasD ? xch[rbuf][tid - TX] : eY[slot];
+    };
     int cntX = lds_ld(fX), cntY = lds_ld(fY);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    P2<T> nx_, ny_;
+    nbr(0, nx_, ny_);
     for (int tau0 = 0; tau0 < nsteps; tau0 += PD) {
         // steps before tau0 are complete: their ring slots may be refilled (the compiler
         // fence keeps the earlier steps' ring reads ahead of the announcement; LDS
@@ -684,8 +697,7 @@ __global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
             const int tau = tau0 + k;
             const int z = tau - lx - ly;
             const bool zin = (unsigned)z < (unsigned)nz;
-            const int rbuf = (tau + 1) & 1, wbuf = tau & 1;
-            const int slot = z & (kRing - 1);
+            const int wbuf = tau & 1;
             // this step's ring values must be in place: value z of the lane's edge
             bool wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
             if (__builtin_amdgcn_ballot_w64(wait) != 0ull) {
@@ -701,10 +713,9 @@ __global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
                         break;
                 } while (__builtin_amdgcn_ballot_w64(wait) != 0ull && spins < kSpinLimit);
                 if (wait) atomicExch(a.sync.err, 1);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                nbr(tau, nx_, ny_);
             }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            const P2<T> nx_ = hasL ? xch[rbuf][tid - 1] : eX[slot];
-            const P2<T> ny_ = hasD ? xch[rbuf][tid - TX] : eY[slot];
             P2<T> nz_;
             nz_.x = z > 0 ? prev.x : T(0);
             nz_.y = z > 0 ? prev.y : T(0);
@@ -742,10 +753,12 @@ __global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             lds_st(sdone, tau + 1);
             __builtin_amdgcn_wave_barrier();
-            // counts for the next step's check
+            // counts and neighbour values for the next step
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             cntX = lds_ld(fX);
             cntY = lds_ld(fY);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            nbr(tau + 1, nx_, ny_);
             load(ring[k]);
             __builtin_amdgcn_sched_barrier(0);
         }
