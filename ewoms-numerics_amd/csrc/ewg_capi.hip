@@ -37,6 +37,8 @@ struct ewg_ctx {
     MatReg *regs = nullptr;
     int *bc_type = nullptr;
     double *bc_val = nullptr;
+    // intensive quantities of the linearization's first pass
+    double *iqv = nullptr, *iqd = nullptr;
     // preconditioner
     double *L64 = nullptr, *D64 = nullptr;
     float *L32 = nullptr, *D32 = nullptr, *U32 = nullptr, *w32 = nullptr;
@@ -94,7 +96,8 @@ hipError_t dalloc(T **p, size_t count) {
 }
 
 void free_all(ewg_ctx *c) {
-    void *ptrs[] = {c->x0, c->x1, c->xcur, c->sto, c->R, c->J, c->K, c->phi, c->pe, c->region,
+    void *ptrs[] = {c->x0, c->x1, c->xcur, c->sto, c->R, c->J, c->iqv, c->iqd, c->K, c->phi,
+                    c->pe, c->region,
                     c->regs, c->bc_type, c->bc_val, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
                     c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks, c->partAny,
@@ -133,6 +136,8 @@ LinArgs lin_args(ewg_ctx *c, double dt, int newton_iter) {
     a.recycle = c->prm.recycle_first_iteration_storage;
     a.R = c->R;
     a.J = c->J;
+    a.iqv = c->iqv;
+    a.iqd = c->iqd;
     return a;
 }
 
@@ -234,23 +239,33 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     if (*c->err_host) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
     if (c->ks_host->status == KS_CONVERGED) return EWG_OK;
     const int *st = &c->ks->status;
-    for (int it = 0; it < c->prm.linear_max_iterations; ++it) {
-        launch_ks_beta(c->ks, c->part2, s);
-        launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s);
-        precond_apply(c, c->kp, c->ky, st);
-        launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
-        launch_ks_alpha(c->ks, c->part0, s);
-        launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s);
-        launch_ks_crit1(c->ks, c->part1, c->partAny, s);
-        precond_apply(c, c->kr, c->kz, st);
-        launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
-        launch_ks_omega(c->ks, c->part0, c->part1, s);
-        launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
-        launch_ks_crit2(c->ks, c->partAny, s);
+    // Iterations are queued speculatively and the state is read back after 1, 2, 4, then
+    // every 8 of them: every kernel of an iteration returns at once when the device-side
+    // status has left KS_RUNNING, so the iterations queued past convergence are no-ops
+    // and the count kept on the device is the one of a solver that stops exactly.
+    int batch = 1;
+    for (int it = 0; it < c->prm.linear_max_iterations;) {
+        const int nb = std::min(batch, c->prm.linear_max_iterations - it);
+        for (int q = 0; q < nb; ++q) {
+            launch_ks_beta(c->ks, c->part2, s);
+            launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s);
+            precond_apply(c, c->kp, c->ky, st);
+            launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
+            launch_ks_alpha(c->ks, c->part0, s);
+            launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s);
+            launch_ks_crit1(c->ks, c->part1, c->partAny, s);
+            precond_apply(c, c->kr, c->kz, st);
+            launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
+            launch_ks_omega(c->ks, c->part0, c->part1, s);
+            launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
+            launch_ks_crit2(c->ks, c->partAny, s);
+        }
+        it += nb;
         HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
                                   s));
         HIP_TRY(c, hipStreamSynchronize(s));
         if (c->ks_host->status != KS_RUNNING) break;
+        batch = std::min(batch * 2, 8);
     }
     *iters = c->ks_host->iterations;
     HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -466,6 +481,8 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->sto, n2);
     ALLOC(c->R, n2);
     ALLOC(c->J, nb);
+    ALLOC(c->iqv, 4 * (size_t)N);
+    ALLOC(c->iqd, 6 * (size_t)N);
     ALLOC(c->K, N);
     ALLOC(c->phi, N);
     ALLOC(c->region, N);

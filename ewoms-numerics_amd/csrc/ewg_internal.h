@@ -49,6 +49,8 @@ struct LinArgs {
     int recycle;
     double *R;            // [2N]
     double *J;            // [nslots][N][4]
+    double *iqv;          // [N][4] intensive quantities of each cell (values)
+    double *iqd;          // [N][6] their derivatives w.r.t. the cell's primary variables
 };
 
 // BiCGStab scalars, kept on the device so that one host read per iteration suffices

@@ -11,6 +11,11 @@
 //                                   (fvbaselinearizer.hh:495-508),
 // so every block is written by exactly one thread (no atomics, bitwise reproducible),
 // and with the slot-major layout each of the 7 block stores is a unit-stride stream.
+//
+// The intensive quantities are evaluated once per cell by a first pass (k_iq, with the
+// cell's two derivative slots) and stored; the flux pass reads the neighbours' values
+// (the value part of the dual evaluation is the plain evaluation, operation for
+// operation) instead of evaluating the material laws of all six neighbours again.
 #include "ewg_internal.h"
 
 namespace ewg {
@@ -59,9 +64,11 @@ __device__ __forceinline__ void face_centre(const Grid &g, const int ijk[3], int
     pos[a] = ((dir & 1) ? (ijk[a] + 1) : ijk[a]) * g.h[a];
 }
 
+template <int DIM>
 __device__ __forceinline__ double dot_g(const LinArgs &a, const double v[3]) {
     double s = 0.0;
-    for (int d = 0; d < a.g.dim; ++d) s += a.grav[d] * v[d];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) s += a.grav[d] * v[d];
     return s;
 }
 
@@ -95,11 +102,12 @@ __device__ __forceinline__ void storage(const LinArgs &a, double phi, T Sw, T M[
 
 // Interior face flux from the focus side: fvbasegradientcalculator.hh:195-240,
 // darcyfluxmodule.hh:185-324, :461-527, immisciblelocalresidual.hh:123-149.
+template <int DIM>
 __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in,
                                               const IQ<double> &ex, double Kin, double Kex,
                                               const int ijk[3], int dir, Ev F[2]) {
     const Grid &g = a.g;
-    const int dim = g.dim;
+    constexpr int dim = DIM;
     int q[3] = {ijk[0], ijk[1], ijk[2]};
     q[dir >> 1] += (dir & 1) ? 1 : -1;
     double posIn[3], posEx[3], posFace[3];
@@ -127,7 +135,7 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
     }
     double absDistTotSq = 0.0;
     for (int d = 0; d < dim; ++d) absDistTotSq += dvTot[d] * dvTot[d];
-    double gIn = dot_g(a, dvIn), gEx = dot_g(a, dvEx);
+    double gIn = dot_g<DIM>(a, dvIn), gEx = dot_g<DIM>(a, dvEx);
     for (int k = 0; k < 2; ++k) {
         Ev pStatIn = -ev(a.rho[k]) * gIn;
         Ev pStatEx = ev(-a.rho[k] * gEx);
@@ -155,11 +163,12 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
 
 // Dirichlet free-flow boundary: darcyfluxmodule.hh:332-453, :488-509,
 // fvbasegradientcalculator.hh:278-312, immiscibleboundaryratevector.hh:91-153.
+template <int DIM>
 __device__ __forceinline__ void freeflow_flux(const LinArgs &a, const IQ<Ev> &in, int cell,
                                               double Kc, const int ijk[3], int dir,
                                               double pwB, double SwB, Ev B[2]) {
     const Grid &g = a.g;
-    const int dim = g.dim;
+    constexpr int dim = DIM;
     const MatReg &r = region_of(a, cell);
     double pe = a.pe ? a.pe[cell] : 0.0;
     double pcB = pcnw(r, pe, SwB);
@@ -185,7 +194,7 @@ __device__ __forceinline__ void freeflow_flux(const LinArgs &a, const IQ<Ev> &in
     for (int d = 0; d < 3; ++d) dvIn[d] = posIn[d] - posFace[d];
     double absDistSq = 0.0;
     for (int d = 0; d < dim; ++d) absDistSq += dvIn[d] * dvIn[d];
-    double gTimesDist = dot_g(a, dvIn);
+    double gTimesDist = dot_g<DIM>(a, dvIn);
     for (int k = 0; k < 2; ++k) {
         Ev pStatIn = (-gTimesDist) * ev(a.rho[k]);
         Ev qq = pStatIn / absDistSq;
@@ -227,6 +236,37 @@ __device__ __forceinline__ void store_block(double *J, int slot, int N, int cell
     b[1] = make_double2(r[1].d0, r[1].d1);
 }
 
+// first pass: iqv[cell] = (p_w, p_n, lambda_w, lambda_n) values, iqd[cell] = derivatives
+// of (p_n, lambda_w, lambda_n) w.r.t. the cell's (p_w, S_w)
+__global__ void __launch_bounds__(256) k_iq(LinArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.g.N) return;
+    const double2 xi = reinterpret_cast<const double2 *>(a.x)[i];
+    const IQ<Ev> q = intensive<Ev>(a, i, Ev{xi.x, 1.0, 0.0}, Ev{xi.y, 0.0, 1.0});
+    double4 v;
+    v.x = q.p[0].v;
+    v.y = q.p[1].v;
+    v.z = q.lam[0].v;
+    v.w = q.lam[1].v;
+    reinterpret_cast<double4 *>(a.iqv)[i] = v;
+    double2 *d = reinterpret_cast<double2 *>(a.iqd + (size_t)i * 6);
+    d[0] = make_double2(q.p[1].d0, q.p[1].d1);
+    d[1] = make_double2(q.lam[0].d0, q.lam[0].d1);
+    d[2] = make_double2(q.lam[1].d0, q.lam[1].d1);
+}
+
+__device__ __forceinline__ IQ<double> iq_values(const LinArgs &a, int cell) {
+    const double4 v = reinterpret_cast<const double4 *>(a.iqv)[cell];
+    IQ<double> q;
+    q.p[0] = v.x;
+    q.p[1] = v.y;
+    q.Sw = 0.0;  // not used by the flux
+    q.lam[0] = v.z;
+    q.lam[1] = v.w;
+    return q;
+}
+
+template <int DIM>
 __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
     const Grid &g = a.g;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -235,17 +275,27 @@ __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
     cell_coords(g, i, ijk);
     const double2 xi = reinterpret_cast<const double2 *>(a.x)[i];
     const double Ki = a.K[i];
-    IQ<Ev> in = intensive<Ev>(a, i, Ev{xi.x, 1.0, 0.0}, Ev{xi.y, 0.0, 1.0});
+    IQ<Ev> in;
+    {
+        const double4 v = reinterpret_cast<const double4 *>(a.iqv)[i];
+        const double2 *d = reinterpret_cast<const double2 *>(a.iqd + (size_t)i * 6);
+        const double2 d0 = d[0], d1 = d[1], d2 = d[2];
+        in.p[0] = Ev{xi.x, 1.0, 0.0};
+        in.Sw = Ev{xi.y, 0.0, 1.0};
+        in.p[1] = Ev{v.y, d0.x, d0.y};
+        in.lam[0] = Ev{v.z, d1.x, d1.y};
+        in.lam[1] = Ev{v.w, d2.x, d2.y};
+    }
     Ev res0[2] = {ev(0.0), ev(0.0)};
-    const int nfaces = 2 * g.dim;
+    constexpr int nfaces = 2 * DIM;
     // fluxes over the interior faces, intersection order (fvbaselocalresidual.hh:295-367)
+#pragma unroll
     for (int dir = 0; dir < nfaces; ++dir) {
         if (!has_neighbour(g, ijk, dir)) continue;
         int nb = i + ((dir & 1) ? 1 : -1) * stride_of(g, dir >> 1);
-        const double2 xj = reinterpret_cast<const double2 *>(a.x)[nb];
-        IQ<double> ex = intensive<double>(a, nb, xj.x, xj.y);
+        const IQ<double> ex = iq_values(a, nb);
         Ev F[2];
-        interior_flux(a, in, ex, Ki, a.K[nb], ijk, dir, F);
+        interior_flux<DIM>(a, in, ex, Ki, a.K[nb], ijk, dir, F);
         double alpha = 1.0;
         alpha *= g.area[dir >> 1];
         Ev rj[2];
@@ -296,13 +346,14 @@ __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
         }
     }
     // boundary segments (fvbaselocalresidual.hh:425-487)
+#pragma unroll
     for (int dir = 0; dir < nfaces; ++dir) {
         if (has_neighbour(g, ijk, dir)) continue;
         long long bf = boundary_face(a, ijk, dir);
         int t = a.bc_type[bf];
         Ev Bv[2] = {ev(0.0), ev(0.0)};
         if (t == 1) {
-            freeflow_flux(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf], a.bc_val[2 * bf + 1], Bv);
+            freeflow_flux<DIM>(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf], a.bc_val[2 * bf + 1], Bv);
         } else if (t == 2) {
             Bv[0] = ev(a.bc_val[2 * bf]);
             Bv[1] = ev(a.bc_val[2 * bf + 1]);
@@ -329,7 +380,11 @@ __global__ void __launch_bounds__(256) k_storage(LinArgs a, const double *x, dou
 
 void launch_linearize(const LinArgs &a, hipStream_t s) {
     int blocks = (a.g.N + 255) / 256;
-    hipLaunchKernelGGL(k_linearize, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_iq, dim3(blocks), dim3(256), 0, s, a);
+    if (a.g.dim == 3)
+        hipLaunchKernelGGL(k_linearize<3>, dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_linearize<2>, dim3(blocks), dim3(256), 0, s, a);
 }
 
 void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s) {
