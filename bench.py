@@ -31,11 +31,14 @@ for _p in (ROOT, os.path.join(ROOT, "ewoms-numerics_amd")):
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(P, nnzb):
-    """Bytes each kernel must move at least once per launch (FP64 values, implicit
-    structured column indices)."""
+def algorithmic_bytes(P, nnzb, fp32_precond=False):
+    """Bytes each kernel must move at least once per launch (FP64 values unless stated,
+    implicit structured column indices; intermediate buffers of an implementation do
+    not count)."""
     N = P.N
     blk = 32
+    fblk = 16 if fp32_precond else 32  # preconditioner factor blocks
+    fval = 4 if fp32_precond else 8    # the forward sweep's intermediate vector
     lower = (nnzb - N) // 2
     per_cell_in = 16 + 8 + 8 + 16 + (8 if P.pe is not None else 0) + \
         (4 if len(P.region_params) > 1 else 0)
@@ -44,11 +47,26 @@ def algorithmic_bytes(P, nnzb):
         "linearize": N * (per_cell_in + 16) + nnzb * blk,
         # all blocks + x in, y out
         "spmv": nnzb * blk + 2 * 16 * N,
-        # forward: L + d in, w out; backward: U + Dinv + w in, w out
-        "ilu_apply": 2 * lower * blk + N * blk + 4 * 16 * N,
-        # A (lower, diagonal, upper) in; L + Dinv out
-        "ilu_factor": nnzb * blk + lower * blk + N * blk,
+        # forward: L + d in, w out; backward: U + Dinv + w in, v out
+        "ilu_apply": 2 * lower * fblk + N * fblk + (16 + 2 * fval + 2 * fval + 16) * N,
+        # A (lower, diagonal, upper) in; L + Dinv out (FP32: and the U copy)
+        "ilu_factor": nnzb * blk + lower * fblk + N * fblk + (lower * fblk if fp32_precond else 0),
     }
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (profiles/pmc_<kernel>_het3d_128x128x64.md, line 'traffic_bytes_per_launch: <n>'),
+    or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_het3d_128x128x64.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        return None
+    return None
 
 
 def timed_run(ctx, P, warmup, steps):
@@ -96,7 +114,7 @@ def main():
     ap.add_argument("--forms", default="fp64,fp32_precond")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--copy-reps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -105,18 +123,20 @@ def main():
 
     import ewg
     from ewg import problems
-    from ewg.capi import (EWG_KERNEL_ILU_APPLY, EWG_KERNEL_ILU_FACTOR, EWG_KERNEL_LINEARIZE,
-                          EWG_KERNEL_SPMV)
+    from ewg.capi import (EWG_KERNEL_COPY, EWG_KERNEL_ILU_APPLY, EWG_KERNEL_ILU_FACTOR,
+                          EWG_KERNEL_LINEARIZE, EWG_KERNEL_SPMV)
 
     P = problems.heterogeneous_3d(args.nx, args.ny, args.nz)
     print(f"# device {ewg.device_info()}; case {P.name} {args.nx}x{args.ny}x{args.nz} "
           f"N={P.N}", file=sys.stderr, flush=True)
     forms = {}
     kernels = {}
+    copy_gbs = None
     for form in args.forms.split(","):
+        fp32 = form == "fp32_precond"
         params = ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3,
-                                    max_time_step_size=P.dt,
-                                    precond_fp32=1 if form == "fp32_precond" else 0)
+                                    max_time_step_size=P.dt, precond_fp32=1 if fp32 else 0,
+                                    kernel_events=1)
         ctx = ewg.GpuContext(P, params)
         n, el, st = timed_run(ctx, P, args.warmup, args.steps)
         forms[form] = {"value": P.N * n / el, "steps": n, "seconds": el,
@@ -127,31 +147,48 @@ def main():
                        "split_s": {"linearize": st["time_linearize_s"],
                                    "solve": st["time_solve_s"],
                                    "update": st["time_update_s"]}}
-        print(f"# {form}: {forms[form]}", file=sys.stderr, flush=True)
+        # launches inside the timed run, bracketed by device events on the library's
+        # stream (the first apply and SpMV of each batch of BiCGStab iterations, every
+        # factorization and linearization); mean duration and algorithmic bandwidth
+        ab = algorithmic_bytes(P, ctx.nnzb, fp32)
+        ks = {}
+        for name, kid in (("linearize", EWG_KERNEL_LINEARIZE), ("spmv", EWG_KERNEL_SPMV),
+                          ("ilu_apply", EWG_KERNEL_ILU_APPLY),
+                          ("ilu_factor", EWG_KERNEL_ILU_FACTOR)):
+            cnt, tot = ctx.kernel_samples(kid)
+            if cnt == 0:
+                continue
+            ms = tot / cnt
+            gbs = ab[name] / (ms * 1e-3) / 1e9
+            ks[name] = {"mean_ms": ms, "samples": cnt, "bytes": ab[name], "achieved_GBs": gbs,
+                        "frac": gbs / HBM_PEAK_GBS}
+        forms[form]["kernels"] = ks
+        print(f"# {form} kernels {ks}", file=sys.stderr, flush=True)
         if form == "fp64":
-            nnzb = ctx.nnzb
-            ab = algorithmic_bytes(P, nnzb)
-            for name, kid in (("linearize", EWG_KERNEL_LINEARIZE), ("spmv", EWG_KERNEL_SPMV),
-                              ("ilu_apply", EWG_KERNEL_ILU_APPLY),
-                              ("ilu_factor", EWG_KERNEL_ILU_FACTOR)):
-                ms = ctx.time_kernel(kid, args.kernel_reps)
-                gbs = ab[name] / (ms * 1e-3) / 1e9
-                kernels[name] = {"mean_ms": ms, "bytes": ab[name], "achieved_GBs": gbs,
-                                 "frac": gbs / HBM_PEAK_GBS}
-            print(f"# kernels {kernels}", file=sys.stderr, flush=True)
+            kernels = ks
+            ms = ctx.time_kernel(EWG_KERNEL_COPY, args.copy_reps)
+            copy_bytes = 2 * 8 * 4 * ctx.nnzb
+            copy_gbs = copy_bytes / (ms * 1e-3) / 1e9
+            print(f"# copy {copy_bytes} bytes {ms:.4f} ms -> {copy_gbs:.0f} GB/s",
+                  file=sys.stderr, flush=True)
         ctx.close()
 
     head = forms.get("fp64") or next(iter(forms.values()))
-    # dominant kernel of the solve: per BiCGStab iteration 2 ILU applies + 2 SpMV
-    dom_name = max(("linearize", "spmv", "ilu_apply"),
-                   key=lambda k: kernels[k]["mean_ms"] * (2 if k != "linearize" else 0.05)) \
-        if kernels else None
+    # dominant kernel: the largest share of the timed run's device time (per BiCGStab
+    # iteration 2 ILU applies and 2 SpMV, per Newton iteration 1 linearization)
+    dom_name = None
+    if kernels:
+        share = {"ilu_apply": 2 * head["linear_iterations"],
+                 "spmv": 2 * head["linear_iterations"],
+                 "linearize": head["newton_iterations"],
+                 "ilu_factor": head["linear_solves"]}
+        dom_name = max(kernels, key=lambda k: kernels[k]["mean_ms"] * share[k])
     roofline = None
     if dom_name:
         k = kernels[dom_name]
         roofline = {"kernel": dom_name, "bound": "hbm", "achieved": round(k["achieved_GBs"], 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(k["frac"], 4),
-                    "traffic": None}
+                    "traffic": pmc_traffic(dom_name)}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(P, args.cpu_sample_steps)
@@ -173,6 +210,7 @@ def main():
                    "form": "fp64", "cells": P.N, "seed": P.meta["seed"]},
         "forms": forms,
         "kernels": kernels,
+        "copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -57,6 +57,14 @@ struct ewg_ctx {
     // ILU(0) factors in the tiles' wavefront layout (schedule 0)
     WlPrecond<double> wl64{};
     WlPrecond<float> wl32{};
+    // event-timed samples of the hot launches (params.kernel_events), by EWG_KERNEL_* id
+    struct Sample {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool armed = false, pending = false;
+        int64_t count = 0;
+        double total_ms = 0.0;
+    } samples[4];
+    double *copy_dst = nullptr;  // destination of the bandwidth-reference copy
     KrylovState *ks = nullptr;
     // pinned host mirrors
     KrylovState *ks_host = nullptr;
@@ -105,6 +113,11 @@ void free_all(ewg_ctx *c) {
                     c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->copy_dst) (void)hipFree(c->copy_dst);
+    for (auto &sm : c->samples) {
+        if (sm.e0) (void)hipEventDestroy(sm.e0);
+        if (sm.e1) (void)hipEventDestroy(sm.e1);
+    }
     if (c->ks_host) (void)hipHostFree(c->ks_host);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
     if (c->err_host) (void)hipHostFree(c->err_host);
@@ -155,6 +168,35 @@ int device_max_abs(ewg_ctx *c, const double *v, double *out) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     *out = c->scal_host[0];
     return EWG_OK;
+}
+
+// Sampled launches: sample_begin/sample_end bracket one launch with events on the
+// context's stream (one sample per kind in flight); sample_collect, called after a
+// stream synchronisation, adds the finished ones to the totals.
+void sample_begin(ewg_ctx *c, int which) {
+    if (!c->prm.kernel_events) return;
+    auto &sm = c->samples[which];
+    if (sm.pending) return;
+    if (!sm.e0 && (hipEventCreate(&sm.e0) != hipSuccess || hipEventCreate(&sm.e1) != hipSuccess))
+        return;
+    if (hipEventRecord(sm.e0, c->stream) == hipSuccess) sm.armed = true;
+}
+void sample_end(ewg_ctx *c, int which) {
+    auto &sm = c->samples[which];
+    if (!sm.armed) return;
+    sm.armed = false;
+    if (hipEventRecord(sm.e1, c->stream) == hipSuccess) sm.pending = true;
+}
+void sample_collect(ewg_ctx *c) {
+    for (auto &sm : c->samples) {
+        if (!sm.pending) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, sm.e0, sm.e1) == hipSuccess) {
+            sm.count += 1;
+            sm.total_ms += ms;
+        }
+        sm.pending = false;
+    }
 }
 
 bool use_wavefront(const ewg_ctx *c) {
@@ -228,13 +270,16 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     if (absTol < 0.0) absTol = c->prm.newton_tolerance / 100.0;
     HIP_TRY(c, hipMemsetAsync(c->ilu_err, 0, sizeof(int), s));
     HIP_TRY(c, hipMemsetAsync(c->anyNonzero, 0, sizeof(int), s));
+    sample_begin(c, EWG_KERNEL_ILU_FACTOR);
     precond_factor(c);
+    sample_end(c, EWG_KERNEL_ILU_FACTOR);
     launch_bicg_init(n, b, c->kr, x, c->kp, c->kv, c->part0, c->part1, s);
     launch_ks_init(c->ks, c->part0, c->part1, c->prm.linear_tolerance, absTol,
                    c->prm.linear_max_error, s);
     HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
+    sample_collect(c);
     *iters = 0;
     if (*c->err_host) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
     if (c->ks_host->status == KS_CONVERGED) return EWG_OK;
@@ -247,10 +292,18 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     for (int it = 0; it < c->prm.linear_max_iterations;) {
         const int nb = std::min(batch, c->prm.linear_max_iterations - it);
         for (int q = 0; q < nb; ++q) {
+            // the first iteration of a batch runs for certain (the state read back before
+            // it was KS_RUNNING): its first apply and SpMV are the sampled launches
             launch_ks_beta(c->ks, c->part2, s);
             launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s);
+            if (q == 0) sample_begin(c, EWG_KERNEL_ILU_APPLY);
             precond_apply(c, c->kp, c->ky, st);
+            if (q == 0) {
+                sample_end(c, EWG_KERNEL_ILU_APPLY);
+                sample_begin(c, EWG_KERNEL_SPMV);
+            }
             launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
+            if (q == 0) sample_end(c, EWG_KERNEL_SPMV);
             launch_ks_alpha(c->ks, c->part0, s);
             launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s);
             launch_ks_crit1(c->ks, c->part1, c->partAny, s);
@@ -264,6 +317,7 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
         HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
                                   s));
         HIP_TRY(c, hipStreamSynchronize(s));
+        sample_collect(c);
         if (c->ks_host->status != KS_RUNNING) break;
         batch = std::min(batch * 2, 8);
     }
@@ -285,7 +339,9 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
 
 int linearize_device(ewg_ctx *c, double dt, int newton_iter) {
     LinArgs a = lin_args(c, dt, newton_iter);
+    sample_begin(c, EWG_KERNEL_LINEARIZE);
     launch_linearize(a, c->stream);
+    sample_end(c, EWG_KERNEL_LINEARIZE);
     HIP_TRY(c, hipGetLastError());
     c->have_system = true;
     return EWG_OK;
@@ -786,6 +842,21 @@ int ewg_get_stats(const ewg_ctx *c, ewg_stats *s) {
 int ewg_reset_stats(ewg_ctx *c) {
     if (!c) return EWG_INVALID_ARGUMENT;
     c->stats = ewg_stats{};
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &sm : c->samples) {
+        sm.armed = sm.pending = false;
+        sm.count = 0;
+        sm.total_ms = 0.0;
+    }
+    return EWG_OK;
+}
+
+int ewg_kernel_samples(ewg_ctx *c, int which, int64_t *count, double *total_ms) {
+    if (!c || !count || !total_ms || which < 0 || which > 3) return EWG_INVALID_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    sample_collect(c);
+    *count = c->samples[which].count;
+    *total_ms = c->samples[which].total_ms;
     return EWG_OK;
 }
 
@@ -793,7 +864,9 @@ int ewg_time_kernel(ewg_ctx *c, int which, int reps, double *mean_ms) {
     if (!c || reps < 1 || !mean_ms) return EWG_INVALID_ARGUMENT;
     if (!c->static_set) return set_err(c, EWG_INVALID_STATE, "static fields not set");
     hipStream_t s = c->stream;
-    if (which != EWG_KERNEL_LINEARIZE && !c->have_system) {
+    const size_t nb_vals = (size_t)c->g.nslots * c->g.N * 4;
+    if (which == EWG_KERNEL_COPY && !c->copy_dst) HIP_TRY(c, dalloc(&c->copy_dst, nb_vals));
+    if (which != EWG_KERNEL_LINEARIZE && which != EWG_KERNEL_COPY && !c->have_system) {
         int rc = linearize_device(c, 1.0, 0);
         if (rc) return rc;
     }
@@ -814,6 +887,7 @@ int ewg_time_kernel(ewg_ctx *c, int which, int reps, double *mean_ms) {
                 break;
             case EWG_KERNEL_ILU_APPLY: precond_apply(c, c->R, c->ky, nullptr); break;
             case EWG_KERNEL_ILU_FACTOR: precond_factor(c); break;
+            case EWG_KERNEL_COPY: launch_copy(nb_vals, c->J, c->copy_dst, s); break;
             default:
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);

@@ -479,6 +479,13 @@ __global__ void __launch_bounds__(kRedThreads) k_ks_crit2(KrylovState *ks, const
 }
 
 // Newton helpers
+// plain streaming copy, 32 bytes per lane and iteration: the bandwidth reference
+__global__ void __launch_bounds__(256) k_copy(size_t n4, const double4 *src, double4 *dst) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void __launch_bounds__(kRedThreads) k_max_abs(int n, const double *x, double *part) {
     double m = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -600,6 +607,12 @@ void launch_max_abs(int n, const double *x, double *part, hipStream_t s) {
 void launch_reduce_max(const double *part, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kRedThreads), 0, s, part, out);
 }
+void launch_copy(size_t n, const double *src, double *dst, hipStream_t s) {
+    const size_t n4 = n / 4;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, n4, reinterpret_cast<const double4 *>(src),
+                       reinterpret_cast<double4 *>(dst));
+}
+
 void launch_reduce_sum(const double *part, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kRedThreads), 0, s, part, out);
 }

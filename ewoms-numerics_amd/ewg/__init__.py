@@ -34,6 +34,9 @@ def load():
         lib.ewg_time_kernel.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         lib.ewg_reset_stats.restype = C.c_int
         lib.ewg_reset_stats.argtypes = [vp]
+        lib.ewg_kernel_samples.restype = C.c_int
+        lib.ewg_kernel_samples.argtypes = [vp, C.c_int, C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_double)]
         lib.ewg_device_info.restype = C.c_int
         lib.ewg_device_info.argtypes = [C.c_char_p, C.c_int]
         _LIB = lib
@@ -53,6 +56,14 @@ class GpuContext(Context):
 
     def reset_stats(self):
         self._check(self.lib.ewg_reset_stats(self.h))
+
+    def kernel_samples(self, which: int):
+        """(count, total_ms) of the event-timed launches of kernel `which` (needs
+        params.kernel_events = 1)."""
+        n = C.c_int64(0)
+        ms = C.c_double(0.0)
+        self._check(self.lib.ewg_kernel_samples(self.h, which, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
 
 def device_info() -> str:

@@ -28,7 +28,7 @@ EWG_KERNEL_LINEARIZE = 0
 EWG_KERNEL_SPMV = 1
 EWG_KERNEL_ILU_APPLY = 2
 EWG_KERNEL_ILU_FACTOR = 3
-EWG_KERNEL_BICGSTAB_ITER = 4
+EWG_KERNEL_COPY = 5
 
 
 class GridDesc(C.Structure):
@@ -61,7 +61,8 @@ class Params(C.Structure):
                 ("max_time_step_size", C.c_double),
                 ("precond_fp32", C.c_int32),
                 ("ilu_schedule", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("kernel_events", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class Stats(C.Structure):
@@ -100,6 +101,7 @@ def default_params(**overrides) -> Params:
     p.max_time_step_size = float("inf")
     p.precond_fp32 = 0
     p.ilu_schedule = 0
+    p.kernel_events = 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

@@ -104,7 +104,11 @@ typedef struct ewg_params {
                                           hyperplane x+y+z = const; 2 tile wavefront, generic
                                           sweep kernel; 3 tile wavefront, one wavefront per
                                           tile */
-    int32_t reserved[6];
+    int32_t kernel_events;             /* 1: bracket sampled launches with device events (the
+                                          first preconditioner apply and SpMV of each batch
+                                          of BiCGStab iterations, every factorization and
+                                          linearization); read with ewg_kernel_samples */
+    int32_t reserved[5];
 } ewg_params;
 
 /* Counters and timers of the last calls; the split follows
@@ -194,10 +198,17 @@ int ewg_reset_stats(ewg_ctx *ctx);
 
 /* Measurement hook (no counterpart in the reference): launches one of the hot
  * kernels `reps` times on the context's stream at the current state and returns the
- * mean duration per launch measured with HIP events on that stream. */
+ * mean duration per launch measured with HIP events on that stream. EWG_KERNEL_COPY is a
+ * plain device-to-device copy of the Jacobian's values (the bandwidth reference). */
 enum { EWG_KERNEL_LINEARIZE = 0, EWG_KERNEL_SPMV = 1, EWG_KERNEL_ILU_APPLY = 2,
-       EWG_KERNEL_ILU_FACTOR = 3, EWG_KERNEL_BICGSTAB_ITER = 4 };
+       EWG_KERNEL_ILU_FACTOR = 3, EWG_KERNEL_COPY = 5 };
 int ewg_time_kernel(ewg_ctx *ctx, int which, int reps, double *mean_ms);
+
+/* Launches timed with device events while params.kernel_events is set (no counterpart in
+ * the reference): count and summed milliseconds of the sampled launches of
+ * which = EWG_KERNEL_LINEARIZE, _SPMV, _ILU_APPLY or _ILU_FACTOR since the last
+ * ewg_reset_stats. */
+int ewg_kernel_samples(ewg_ctx *ctx, int which, int64_t *count, double *total_ms);
 
 /* Device description for run records (no counterpart in the reference). */
 int ewg_device_info(char *buf, int len);
