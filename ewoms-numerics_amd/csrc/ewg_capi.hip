@@ -760,6 +760,23 @@ int ewg_linearize(ewg_ctx *c, double dt, int newton_iter, double *resid, double 
     return EWG_OK;
 }
 
+int ewg_set_system(ewg_ctx *c, const double *resid, const double *jac) {
+    if (!c) return EWG_INVALID_ARGUMENT;
+    const size_t N = c->g.N;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (resid) HIP_TRY(c, hipMemcpy(c->R, resid, 2 * N * sizeof(double), hipMemcpyHostToDevice));
+    if (jac) {
+        // absent neighbours keep zero blocks in the slot layout
+        std::vector<double> slots((size_t)c->g.nslots * N * 4, 0.0);
+        for (size_t b = 0; b < c->bcrs_src.size(); ++b)
+            std::memcpy(slots.data() + 4 * c->bcrs_src[b], jac + 4 * b, 4 * sizeof(double));
+        HIP_TRY(c, hipMemcpy(c->J, slots.data(), slots.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+    }
+    if (resid || jac) c->have_system = true;
+    return EWG_OK;
+}
+
 int ewg_solve(ewg_ctx *c, double *dx, int32_t *iterations) {
     if (!c) return EWG_INVALID_ARGUMENT;
     if (!c->have_system) return set_err(c, EWG_INVALID_STATE, "no linearization");

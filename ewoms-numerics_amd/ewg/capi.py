@@ -145,6 +145,7 @@ def bind(lib: C.CDLL, prefix: str) -> C.CDLL:
     f("advance_time_level", C.c_int, vp)
     f("get_pattern", C.c_int, vp, _I, _I)
     f("linearize", C.c_int, vp, C.c_double, C.c_int, _D, _D)
+    f("set_system", C.c_int, vp, _D, _D)
     f("solve", C.c_int, vp, _D, _I)
     f("newton", C.c_int, vp, C.c_double, _I, _D)
     f("run", C.c_int, vp, C.c_double, C.c_double, C.c_double, _I)
@@ -239,6 +240,11 @@ class Context:
         j = np.empty(4 * self.nnzb) if want else None
         self._check(self._fn("linearize")(self.h, dt, newton_iter, _dptr(r), _dptr(j)))
         return r, j
+
+    def set_system(self, resid=None, jac=None):
+        r = None if resid is None else np.ascontiguousarray(resid, dtype=np.float64)
+        j = None if jac is None else np.ascontiguousarray(jac, dtype=np.float64)
+        self._check(self._fn("set_system")(self.h, _dptr(r), _dptr(j)))
 
     def solve(self):
         dx = np.empty(2 * self.N)

@@ -176,6 +176,12 @@ int ewg_get_pattern(const ewg_ctx *ctx, int32_t *rowptr, int32_t *colidx);
  * reproduce FvBaseLinearizer::residual() / jacobian(). */
 int ewg_linearize(ewg_ctx *ctx, double dt, int newton_iter, double *resid, double *jac);
 
+/* ParallelBaseBackend::prepare/setResidual/setMatrix (parallelbasebackend.hh:231-312)
+ * for a system built elsewhere: uploads a residual (2N doubles, AoS) and Jacobian values
+ * (nnzb*4 doubles in the BCRS order of ewg_get_pattern) as the system ewg_solve solves.
+ * Either may be NULL to keep the current one. */
+int ewg_set_system(ewg_ctx *ctx, const double *resid, const double *jac);
+
 /* ParallelBaseBackend::setMatrix + solve (parallelbasebackend.hh:290-338): solves
  * J dx = R for the last linearization with ILU0-preconditioned BiCGStab from x0 = 0.
  * Returns EWG_NOT_CONVERGED when the criterion is not met, EWG_NUMERICAL_ISSUE on

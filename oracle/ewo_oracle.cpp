@@ -1170,6 +1170,17 @@ int ewo_bicgstab(ewo_ctx *c, const double *A, const double *b, double *x, int32_
     return conv ? EWG_OK : EWG_NOT_CONVERGED;
 }
 
+// ParallelBaseBackend::prepare/setResidual/setMatrix (parallelbasebackend.hh:231-312)
+int ewo_set_system(ewo_ctx *c, const double *resid, const double *jac) {
+    if (!c) return EWG_INVALID_ARGUMENT;
+    if (resid) c->resid.assign(resid, resid + 2 * c->N);
+    if (jac) c->jac.assign(jac, jac + 4 * c->colidx.size());
+    if (c->resid.size() != (size_t)(2 * c->N) || c->jac.size() != 4 * c->colidx.size())
+        return fail(c, EWG_INVALID_STATE, "incomplete system");
+    c->have_system = true;
+    return EWG_OK;
+}
+
 int ewo_solve(ewo_ctx *c, double *dx, int32_t *iterations) {
     if (!c) return EWG_INVALID_ARGUMENT;
     if (!c->have_system) return fail(c, EWG_INVALID_STATE, "no linearization");

@@ -49,6 +49,7 @@ int ewo_ilu0_apply(const ewo_ctx *ctx, const double *LU, const double *d, double
 int ewo_bicgstab(ewo_ctx *ctx, const double *A, const double *b, double *x,
                  int32_t *iterations);
 
+int ewo_set_system(ewo_ctx *ctx, const double *resid, const double *jac);
 int ewo_solve(ewo_ctx *ctx, double *dx, int32_t *iterations);   /* last linearization */
 int ewo_newton(ewo_ctx *ctx, double dt, int32_t *iterations, double *final_error);
 int ewo_run(ewo_ctx *ctx, double t0, double t_end, double dt0, int32_t *steps_out);
