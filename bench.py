@@ -3,8 +3,11 @@
 
 Workload: the 3D heterogeneous DNAPL case, 128 x 128 x 64 cells (1,048,576 cells, 2x2
 BCRS blocks on a 7-point stencil), dt = 500 s, Newton tolerance 1e-6, BiCGStab + ILU(0)
-with the reference's default linear tolerances, FP64 throughout ("fp64"); the same run
-with FP32 ILU(0) factors inside the FP64 BiCGStab ("fp32_precond") is reported beside it.
+with the reference's default linear tolerances, FP64 throughout ("fp64"). Reported beside
+it: the same run with FP32 ILU(0) factors inside the FP64 BiCGStab ("fp32_precond"), and
+with FP32 face fluxes in the linearization as well ("mixed"); for these the pressure and
+saturation deviation from the FP64 run and the Newton and BiCGStab iteration counts are
+reported for every time step (an untimed second pass).
 
 A step is one implicit-Euler time step driven to Newton convergence (linearization,
 ILU(0) factorization, BiCGStab, update, norms). W warm-up steps run first, the state is
@@ -84,6 +87,51 @@ def timed_run(ctx, P, warmup, steps):
     return n, t1 - t0, ctx.stats()
 
 
+FORMS = {
+    "fp64": {},
+    "fp32_precond": {"precond_fp32": 1},
+    "fp32_flux": {"linearize_fp32": 1},
+    "mixed": {"precond_fp32": 1, "linearize_fp32": 1},
+}
+
+
+def form_params(form, **extra):
+    import ewg
+    return ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3, **FORMS[form],
+                              **extra)
+
+
+def per_step_deviation(P, form, steps):
+    """Steps the form and the FP64 path side by side from the initial condition: max |dS_w|,
+    max |dp_w| / max |p_w| of the FP64 state, Newton and BiCGStab iterations per step."""
+    import numpy as np
+    import ewg
+    ref = ewg.GpuContext(P, form_params("fp64", max_time_step_size=P.dt))
+    oth = ewg.GpuContext(P, form_params(form, max_time_step_size=P.dt))
+    out = {"max_dS": [], "max_rel_dp": [], "newton": [], "newton_fp64": [],
+           "krylov": [], "krylov_fp64": []}
+    for _ in range(steps):
+        its = []
+        for c in (ref, oth):
+            k0 = c.stats()["linear_iterations"]
+            rc, it, err = c.newton(P.dt)
+            if rc != 0:
+                raise RuntimeError(f"{form}: Newton failed (rc {rc})")
+            its.append((it, c.stats()["linear_iterations"] - k0))
+            c.advance_time_level()
+        xr, xo = ref.solution(0), oth.solution(0)
+        out["max_dS"].append(float(np.abs(xo[1::2] - xr[1::2]).max()))
+        out["max_rel_dp"].append(float(np.abs(xo[0::2] - xr[0::2]).max() /
+                                       np.abs(xr[0::2]).max()))
+        out["newton_fp64"].append(its[0][0])
+        out["newton"].append(its[1][0])
+        out["krylov_fp64"].append(its[0][1])
+        out["krylov"].append(its[1][1])
+    ref.close()
+    oth.close()
+    return out
+
+
 def cpu_baseline(P, sample_steps):
     from oracle.pyoracle import OracleContext
     o = OracleContext(P)
@@ -111,7 +159,9 @@ def main():
     ap.add_argument("--nx", type=int, default=128)
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=64)
-    ap.add_argument("--forms", default="fp64,fp32_precond")
+    ap.add_argument("--forms", default="fp64,fp32_precond,mixed")
+    ap.add_argument("--no-deviation", action="store_true",
+                    help="skip the untimed per-step comparison of the reduced-precision forms")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-reps", type=int, default=10)
@@ -133,10 +183,10 @@ def main():
     kernels = {}
     copy_gbs = None
     for form in args.forms.split(","):
-        fp32 = form == "fp32_precond"
-        params = ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3,
-                                    max_time_step_size=P.dt, precond_fp32=1 if fp32 else 0,
-                                    kernel_events=1)
+        if form not in FORMS:
+            raise SystemExit(f"bench.py: unknown form {form!r} (known: {', '.join(FORMS)})")
+        fp32 = bool(FORMS[form].get("precond_fp32"))
+        params = form_params(form, max_time_step_size=P.dt, kernel_events=1)
         ctx = ewg.GpuContext(P, params)
         n, el, st = timed_run(ctx, P, args.warmup, args.steps)
         forms[form] = {"value": P.N * n / el, "steps": n, "seconds": el,
@@ -172,6 +222,13 @@ def main():
             print(f"# copy {copy_bytes} bytes {ms:.4f} ms -> {copy_gbs:.0f} GB/s",
                   file=sys.stderr, flush=True)
         ctx.close()
+        if form != "fp64" and not args.no_deviation:
+            dev = per_step_deviation(P, form, args.steps)
+            forms[form]["vs_fp64"] = dev
+            print(f"# {form} vs fp64: max dS {max(dev['max_dS']):.3e}, max rel dp "
+                  f"{max(dev['max_rel_dp']):.3e}, newton {sum(dev['newton'])} vs "
+                  f"{sum(dev['newton_fp64'])}, krylov {sum(dev['krylov'])} vs "
+                  f"{sum(dev['krylov_fp64'])}", file=sys.stderr, flush=True)
 
     head = forms.get("fp64") or next(iter(forms.values()))
     # dominant kernel: the largest share of the timed run's device time (per BiCGStab

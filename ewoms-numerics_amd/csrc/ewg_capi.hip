@@ -151,6 +151,7 @@ LinArgs lin_args(ewg_ctx *c, double dt, int newton_iter) {
     a.J = c->J;
     a.iqv = c->iqv;
     a.iqd = c->iqd;
+    a.flux_fp32 = c->prm.linearize_fp32;
     return a;
 }
 
@@ -475,6 +476,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     if (material->law != EWG_LAW_VAN_GENUCHTEN && material->law != EWG_LAW_BROOKS_COREY)
         return EWG_INVALID_ARGUMENT;
     if (params->ilu_schedule < 0 || params->ilu_schedule > 3) return EWG_INVALID_ARGUMENT;
+    if (params->linearize_fp32 != 0 && params->linearize_fp32 != 1) return EWG_INVALID_ARGUMENT;
     auto *c = new ewg_ctx();
     c->grid = *grid;
     c->fluid = *fluid;
@@ -667,6 +669,8 @@ int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
     if (!c || !p) return EWG_INVALID_ARGUMENT;
     if (p->ilu_schedule < 0 || p->ilu_schedule > 3)
         return set_err(c, EWG_INVALID_ARGUMENT, "ilu_schedule must be 0..3");
+    if (p->linearize_fp32 != 0 && p->linearize_fp32 != 1)
+        return set_err(c, EWG_INVALID_ARGUMENT, "linearize_fp32 must be 0 or 1");
     if (p->precond_fp32) {
         int rc = ensure_fp32(c);
         if (rc) return rc;

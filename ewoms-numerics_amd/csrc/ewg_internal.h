@@ -51,6 +51,7 @@ struct LinArgs {
     double *J;            // [nslots][N][4]
     double *iqv;          // [N][4] intensive quantities of each cell (values)
     double *iqd;          // [N][6] their derivatives w.r.t. the cell's primary variables
+    int flux_fp32;        // 1: face fluxes and their derivatives in FP32
 };
 
 // BiCGStab scalars, kept on the device so that one host read per iteration suffices

@@ -16,6 +16,10 @@
 // cell's two derivative slots) and stored; the flux pass reads the neighbours' values
 // (the value part of the dual evaluation is the plain evaluation, operation for
 // operation) instead of evaluating the material laws of all six neighbours again.
+//
+// The FP32 flux form (LinArgs::flux_fp32) evaluates each face flux and its derivatives in
+// float duals; the pressure differences are taken in double before the narrowing, and the
+// residual, the Jacobian blocks and the storage term stay double.
 #include "ewg_internal.h"
 
 namespace ewg {
@@ -76,6 +80,26 @@ __device__ __forceinline__ const MatReg &region_of(const LinArgs &a, int cell) {
     return a.regs[a.region ? a.region[cell] : 0];
 }
 
+// Scalar type S of the flux arithmetic: double, or float for the FP32 flux form, where the
+// pressure differences are still taken in double (they cancel values of order 1e5 Pa) and
+// the face fluxes are accumulated into the double residual and Jacobian.
+template <class S>
+__device__ __forceinline__ Dual<S> cst(double x) {
+    return Dual<S>{static_cast<S>(x), S(0), S(0)};
+}
+template <class S>
+__device__ __forceinline__ Dual<S> narrow(const Ev &x);
+template <>
+__device__ __forceinline__ Ev narrow<double>(const Ev &x) {
+    return x;
+}
+template <>
+__device__ __forceinline__ Evf narrow<float>(const Ev &x) {
+    return to_f32(x);
+}
+__device__ __forceinline__ Ev widen(const Ev &x) { return x; }
+__device__ __forceinline__ Ev widen(const Evf &x) { return to_f64(x); }
+
 // ImmiscibleIntensiveQuantities::update (immiscibleintensivequantities.hh:87-145)
 template <class T>
 __device__ __forceinline__ IQ<T> intensive(const LinArgs &a, int cell, T p0, T Sw) {
@@ -102,10 +126,11 @@ __device__ __forceinline__ void storage(const LinArgs &a, double phi, T Sw, T M[
 
 // Interior face flux from the focus side: fvbasegradientcalculator.hh:195-240,
 // darcyfluxmodule.hh:185-324, :461-527, immisciblelocalresidual.hh:123-149.
-template <int DIM>
+template <int DIM, class S>
 __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in,
                                               const IQ<double> &ex, double Kin, double Kex,
                                               const int ijk[3], int dir, Ev F[2]) {
+    using D = Dual<S>;
     const Grid &g = a.g;
     constexpr int dim = DIM;
     int q[3] = {ijk[0], ijk[1], ijk[2]};
@@ -122,10 +147,11 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
         double t = posEx[d] - posIn[d];
         distSq += t * t;
     }
-    Ev grad[2][3];
+    D grad[2][3];
     for (int k = 0; k < 2; ++k) {
-        Ev deltay = ev(ex.p[k]) - in.p[k];
-        for (int d = 0; d < dim; ++d) grad[k][d] = deltay * ((posEx[d] - posIn[d]) / distSq);
+        D deltay = narrow<S>(ev(ex.p[k]) - in.p[k]);
+        for (int d = 0; d < dim; ++d)
+            grad[k][d] = deltay * static_cast<S>((posEx[d] - posIn[d]) / distSq);
     }
     double dvIn[3], dvEx[3], dvTot[3];
     for (int d = 0; d < 3; ++d) {
@@ -137,36 +163,37 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
     for (int d = 0; d < dim; ++d) absDistTotSq += dvTot[d] * dvTot[d];
     double gIn = dot_g<DIM>(a, dvIn), gEx = dot_g<DIM>(a, dvEx);
     for (int k = 0; k < 2; ++k) {
-        Ev pStatIn = -ev(a.rho[k]) * gIn;
-        Ev pStatEx = ev(-a.rho[k] * gEx);
-        Ev qq = (pStatEx - pStatIn) / absDistTotSq;
-        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + ev(dvTot[d]) * qq;
+        D pStatIn = -cst<S>(a.rho[k]) * static_cast<S>(gIn);
+        D pStatEx = cst<S>(-a.rho[k] * gEx);
+        D qq = (pStatEx - pStatIn) / static_cast<S>(absDistTotSq);
+        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + cst<S>(dvTot[d]) * qq;
     }
-    double Kf = harmonic_mean(Kin, Kex);
+    const S Kf = static_cast<S>(harmonic_mean(Kin, Kex));
     for (int k = 0; k < 2; ++k) {
-        Ev tmp = ev(0.0);
-        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * normal[d];
-        bool upIsEx = tmp.v > 0.0;
-        Ev lam = upIsEx ? ev(ex.lam[k]) : in.lam[k];
-        Ev v[3];
+        D tmp = cst<S>(0.0);
+        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * static_cast<S>(normal[d]);
+        bool upIsEx = tmp.v > S(0);
+        D lam = upIsEx ? cst<S>(ex.lam[k]) : narrow<S>(in.lam[k]);
+        D v[3];
         for (int r = 0; r < dim; ++r) {
-            v[r] = ev(0.0);
-            for (int c = 0; c < dim; ++c) v[r] = v[r] + (r == c ? Kf : 0.0) * grad[k][c];
+            v[r] = cst<S>(0.0);
+            for (int c = 0; c < dim; ++c) v[r] = v[r] + (r == c ? Kf : S(0)) * grad[k][c];
         }
-        Ev mlam = -lam;
+        D mlam = -lam;
         for (int r = 0; r < dim; ++r) v[r] = v[r] * mlam;
-        Ev vol = ev(0.0);
-        for (int d = 0; d < dim; ++d) vol = vol + v[d] * normal[d];
-        F[k] = ev(0.0) + vol * ev(a.rho[k]);
+        D vol = cst<S>(0.0);
+        for (int d = 0; d < dim; ++d) vol = vol + v[d] * static_cast<S>(normal[d]);
+        F[k] = widen(cst<S>(0.0) + vol * cst<S>(a.rho[k]));
     }
 }
 
 // Dirichlet free-flow boundary: darcyfluxmodule.hh:332-453, :488-509,
 // fvbasegradientcalculator.hh:278-312, immiscibleboundaryratevector.hh:91-153.
-template <int DIM>
+template <int DIM, class S>
 __device__ __forceinline__ void freeflow_flux(const LinArgs &a, const IQ<Ev> &in, int cell,
                                               double Kc, const int ijk[3], int dir,
                                               double pwB, double SwB, Ev B[2]) {
+    using D = Dual<S>;
     const Grid &g = a.g;
     constexpr int dim = DIM;
     const MatReg &r = region_of(a, cell);
@@ -185,10 +212,11 @@ __device__ __forceinline__ void freeflow_flux(const LinArgs &a, const IQ<Ev> &in
         double t = posFace[d] - posIn[d];
         distSq += t * t;
     }
-    Ev grad[2][3];
+    D grad[2][3];
     for (int k = 0; k < 2; ++k) {
-        Ev deltay = pB[k] - in.p[k];
-        for (int d = 0; d < dim; ++d) grad[k][d] = deltay * ((posFace[d] - posIn[d]) / distSq);
+        D deltay = narrow<S>(pB[k] - in.p[k]);
+        for (int d = 0; d < dim; ++d)
+            grad[k][d] = deltay * static_cast<S>((posFace[d] - posIn[d]) / distSq);
     }
     double dvIn[3];
     for (int d = 0; d < 3; ++d) dvIn[d] = posIn[d] - posFace[d];
@@ -196,27 +224,28 @@ __device__ __forceinline__ void freeflow_flux(const LinArgs &a, const IQ<Ev> &in
     for (int d = 0; d < dim; ++d) absDistSq += dvIn[d] * dvIn[d];
     double gTimesDist = dot_g<DIM>(a, dvIn);
     for (int k = 0; k < 2; ++k) {
-        Ev pStatIn = (-gTimesDist) * ev(a.rho[k]);
-        Ev qq = pStatIn / absDistSq;
-        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + ev(dvIn[d]) * qq;
+        D pStatIn = static_cast<S>(-gTimesDist) * cst<S>(a.rho[k]);
+        D qq = pStatIn / static_cast<S>(absDistSq);
+        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + cst<S>(dvIn[d]) * qq;
     }
+    const S Ks = static_cast<S>(Kc);
     for (int k = 0; k < 2; ++k) {
-        Ev tmp = ev(0.0);
-        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * normal[d];
-        bool upIsBoundary = tmp.v > 0.0;
-        Ev mob = upIsBoundary ? ev(krB[k] / a.mu[k]) : in.lam[k];
-        Ev v[3];
+        D tmp = cst<S>(0.0);
+        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * static_cast<S>(normal[d]);
+        bool upIsBoundary = tmp.v > S(0);
+        D mob = upIsBoundary ? cst<S>(krB[k] / a.mu[k]) : narrow<S>(in.lam[k]);
+        D v[3];
         for (int rr = 0; rr < dim; ++rr) {
-            v[rr] = ev(0.0);
-            for (int c = 0; c < dim; ++c) v[rr] = v[rr] + (rr == c ? Kc : 0.0) * grad[k][c];
+            v[rr] = cst<S>(0.0);
+            for (int c = 0; c < dim; ++c) v[rr] = v[rr] + (rr == c ? Ks : S(0)) * grad[k][c];
         }
-        Ev mmob = -mob;
+        D mmob = -mob;
         for (int rr = 0; rr < dim; ++rr) v[rr] = v[rr] * mmob;
-        Ev vol = ev(0.0);
-        for (int d = 0; d < dim; ++d) vol = vol + v[d] * normal[d];
+        D vol = cst<S>(0.0);
+        for (int d = 0; d < dim; ++d) vol = vol + v[d] * static_cast<S>(normal[d]);
         // density by pressure comparison: both candidates are the same constant here
         double dens = a.rho[k];
-        B[k] = ev(0.0) + vol * ev(dens);
+        B[k] = widen(cst<S>(0.0) + vol * cst<S>(dens));
     }
 }
 
@@ -266,7 +295,7 @@ __device__ __forceinline__ IQ<double> iq_values(const LinArgs &a, int cell) {
     return q;
 }
 
-template <int DIM>
+template <int DIM, class S>
 __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
     const Grid &g = a.g;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,7 +324,7 @@ __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
         int nb = i + ((dir & 1) ? 1 : -1) * stride_of(g, dir >> 1);
         const IQ<double> ex = iq_values(a, nb);
         Ev F[2];
-        interior_flux<DIM>(a, in, ex, Ki, a.K[nb], ijk, dir, F);
+        interior_flux<DIM, S>(a, in, ex, Ki, a.K[nb], ijk, dir, F);
         double alpha = 1.0;
         alpha *= g.area[dir >> 1];
         Ev rj[2];
@@ -353,7 +382,7 @@ __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
         int t = a.bc_type[bf];
         Ev Bv[2] = {ev(0.0), ev(0.0)};
         if (t == 1) {
-            freeflow_flux<DIM>(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf], a.bc_val[2 * bf + 1], Bv);
+            freeflow_flux<DIM, S>(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf], a.bc_val[2 * bf + 1], Bv);
         } else if (t == 2) {
             Bv[0] = ev(a.bc_val[2 * bf]);
             Bv[1] = ev(a.bc_val[2 * bf + 1]);
@@ -381,10 +410,17 @@ __global__ void __launch_bounds__(256) k_storage(LinArgs a, const double *x, dou
 void launch_linearize(const LinArgs &a, hipStream_t s) {
     int blocks = (a.g.N + 255) / 256;
     hipLaunchKernelGGL(k_iq, dim3(blocks), dim3(256), 0, s, a);
-    if (a.g.dim == 3)
-        hipLaunchKernelGGL(k_linearize<3>, dim3(blocks), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_linearize<2>, dim3(blocks), dim3(256), 0, s, a);
+    if (a.flux_fp32) {
+        if (a.g.dim == 3)
+            hipLaunchKernelGGL((k_linearize<3, float>), dim3(blocks), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_linearize<2, float>), dim3(blocks), dim3(256), 0, s, a);
+    } else {
+        if (a.g.dim == 3)
+            hipLaunchKernelGGL((k_linearize<3, double>), dim3(blocks), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_linearize<2, double>), dim3(blocks), dim3(256), 0, s, a);
+    }
 }
 
 void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s) {

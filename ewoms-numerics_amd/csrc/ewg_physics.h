@@ -102,33 +102,56 @@ EWG_HD double dpow(double x, double y) {
 // ----------------------------------------------------------------------------------
 // Forward-mode dual number: value + d/dp_w + d/dS_w of the focus cell. Rules:
 // (uv)' = u'v + v'u, (u/v)' = (v u' - u v')/v^2; mixed scalar ops leave the
-// derivatives untouched or scale them.
+// derivatives untouched or scale them. Dual<double> (Ev) is the linearization's type,
+// Dual<float> (Evf) the one of the FP32 flux form.
 // ----------------------------------------------------------------------------------
-struct Ev {
-    double v, d0, d1;
+template <class T>
+struct Dual {
+    T v, d0, d1;
 };
+using Ev = Dual<double>;
+using Evf = Dual<float>;
 EWG_HD Ev ev(double x) { return Ev{x, 0.0, 0.0}; }
-EWG_HD Ev operator+(Ev a, Ev b) { return Ev{a.v + b.v, a.d0 + b.d0, a.d1 + b.d1}; }
-EWG_HD Ev operator-(Ev a, Ev b) { return Ev{a.v - b.v, a.d0 - b.d0, a.d1 - b.d1}; }
-EWG_HD Ev operator-(Ev a) { return Ev{-a.v, -a.d0, -a.d1}; }
-EWG_HD Ev operator*(Ev a, Ev b) {
-    return Ev{a.v * b.v, a.d0 * b.v + b.d0 * a.v, a.d1 * b.v + b.d1 * a.v};
+template <class T>
+EWG_HD Dual<T> operator+(Dual<T> a, Dual<T> b) { return Dual<T>{a.v + b.v, a.d0 + b.d0, a.d1 + b.d1}; }
+template <class T>
+EWG_HD Dual<T> operator-(Dual<T> a, Dual<T> b) { return Dual<T>{a.v - b.v, a.d0 - b.d0, a.d1 - b.d1}; }
+template <class T>
+EWG_HD Dual<T> operator-(Dual<T> a) { return Dual<T>{-a.v, -a.d0, -a.d1}; }
+template <class T>
+EWG_HD Dual<T> operator*(Dual<T> a, Dual<T> b) {
+    return Dual<T>{a.v * b.v, a.d0 * b.v + b.d0 * a.v, a.d1 * b.v + b.d1 * a.v};
 }
-EWG_HD Ev operator/(Ev a, Ev b) {
-    double vv = b.v * b.v;
-    Ev r;
+template <class T>
+EWG_HD Dual<T> operator/(Dual<T> a, Dual<T> b) {
+    T vv = b.v * b.v;
+    Dual<T> r;
     r.d0 = (b.v * a.d0 - a.v * b.d0) / vv;
     r.d1 = (b.v * a.d1 - a.v * b.d1) / vv;
     r.v = a.v / b.v;
     return r;
 }
-EWG_HD Ev operator+(Ev a, double s) { return Ev{a.v + s, a.d0, a.d1}; }
-EWG_HD Ev operator+(double s, Ev a) { return Ev{s + a.v, a.d0, a.d1}; }
-EWG_HD Ev operator-(Ev a, double s) { return Ev{a.v - s, a.d0, a.d1}; }
-EWG_HD Ev operator-(double s, Ev a) { return Ev{s - a.v, -a.d0, -a.d1}; }
-EWG_HD Ev operator*(Ev a, double s) { return Ev{a.v * s, a.d0 * s, a.d1 * s}; }
-EWG_HD Ev operator*(double s, Ev a) { return Ev{a.v * s, a.d0 * s, a.d1 * s}; }
-EWG_HD Ev operator/(Ev a, double s) { return Ev{a.v / s, a.d0 / s, a.d1 / s}; }
+template <class T>
+EWG_HD Dual<T> operator+(Dual<T> a, T s) { return Dual<T>{a.v + s, a.d0, a.d1}; }
+template <class T>
+EWG_HD Dual<T> operator+(T s, Dual<T> a) { return Dual<T>{s + a.v, a.d0, a.d1}; }
+template <class T>
+EWG_HD Dual<T> operator-(Dual<T> a, T s) { return Dual<T>{a.v - s, a.d0, a.d1}; }
+template <class T>
+EWG_HD Dual<T> operator-(T s, Dual<T> a) { return Dual<T>{s - a.v, -a.d0, -a.d1}; }
+template <class T>
+EWG_HD Dual<T> operator*(Dual<T> a, T s) { return Dual<T>{a.v * s, a.d0 * s, a.d1 * s}; }
+template <class T>
+EWG_HD Dual<T> operator*(T s, Dual<T> a) { return Dual<T>{a.v * s, a.d0 * s, a.d1 * s}; }
+template <class T>
+EWG_HD Dual<T> operator/(Dual<T> a, T s) { return Dual<T>{a.v / s, a.d0 / s, a.d1 / s}; }
+// precision changes of a dual
+EWG_HD Evf to_f32(const Ev &a) {
+    return Evf{static_cast<float>(a.v), static_cast<float>(a.d0), static_cast<float>(a.d1)};
+}
+EWG_HD Ev to_f64(const Evf &a) {
+    return Ev{static_cast<double>(a.v), static_cast<double>(a.d0), static_cast<double>(a.d1)};
+}
 
 EWG_HD double vpow(double x, double y) { return dpow(x, y); }
 EWG_HD Ev vpow(Ev x, double y) {

@@ -62,7 +62,8 @@ class Params(C.Structure):
                 ("precond_fp32", C.c_int32),
                 ("ilu_schedule", C.c_int32),
                 ("kernel_events", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("linearize_fp32", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):
@@ -102,6 +103,7 @@ def default_params(**overrides) -> Params:
     p.precond_fp32 = 0
     p.ilu_schedule = 0
     p.kernel_events = 0
+    p.linearize_fp32 = 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

@@ -108,7 +108,12 @@ typedef struct ewg_params {
                                           first preconditioner apply and SpMV of each batch
                                           of BiCGStab iterations, every factorization and
                                           linearization); read with ewg_kernel_samples */
-    int32_t reserved[5];
+    int32_t linearize_fp32;            /* 0: FP64 linearization (library default); 1: face
+                                          fluxes and their derivatives in FP32, accumulated
+                                          into the FP64 residual and Jacobian (intensive
+                                          quantities, storage terms and pressure differences
+                                          stay FP64) */
+    int32_t reserved[4];
 } ewg_params;
 
 /* Counters and timers of the last calls; the split follows

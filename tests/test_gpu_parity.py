@@ -164,3 +164,66 @@ def test_wavefront_ilu_matches_hyperplane_schedule(shape, fp32):
         rc, dx, it = out[sched]
         assert rc == 0 and it == it1, (sched, rc, it, it1)
         assert np.array_equal(dx, dx1), sched
+
+
+@pytest.mark.parametrize("case", [0, 3])
+def test_fp32_flux_linearization_close_to_fp64(case):
+    """FP32 face fluxes (pressure differences in FP64, accumulation into the FP64 residual
+    and Jacobian) against the FP64 oracle: both agree to a few FP32 roundings of the flux
+    terms (bound: 1e-5 of the largest entry), and the FP64 path is untouched."""
+    import ewg
+    from ewg import problems
+    from oracle.pyoracle import OracleContext
+    name, P = _lin_cases()[case]
+    print(f"case {name}", flush=True)
+    g32 = ewg.GpuContext(P, ewg.default_params(linearize_fp32=1))
+    g64 = ewg.GpuContext(P)
+    ora = OracleContext(P)
+    x = problems.perturbed_state(P, seed=5 + case)
+    x1 = problems.perturbed_state(P, seed=50 + case, dS=0.1)
+    for c in (g32, g64, ora):
+        c.set_solution(x1, 1)
+        c.set_solution(x1, 0)
+        c.linearize(P.dt, 0, want=False)
+        c.set_solution(x, 0)
+    r32, j32 = g32.linearize(P.dt, 1)
+    r64, j64 = g64.linearize(P.dt, 1)
+    ro, jo = ora.linearize(P.dt, 1)
+    er = np.abs(r32 - ro).max() / np.abs(ro).max()
+    ej = np.abs(j32 - jo).max() / np.abs(jo).max()
+    print(f"fp32 flux: residual {er:.3e}, jacobian {ej:.3e} (of the largest entry)", flush=True)
+    assert er < 1e-5 and ej < 1e-5
+    assert er > 0.0 or ej > 0.0   # the FP32 form is really taken
+    assert rel_err(r64, ro) < 1e-12 and rel_err(j64, jo) < 1e-12
+
+
+@pytest.mark.parametrize("form", ["fp32_flux", "mixed"])
+def test_mixed_precision_steps_close_to_fp64_oracle(form):
+    """FP32 face fluxes (alone, and with the FP32 preconditioner) against the FP64 oracle,
+    both at Newton tolerance 1e-6, every time step: saturations within 1e-6, pressures
+    within 1e-6 relative, Newton iterations within one of the oracle's."""
+    import ewg
+    from ewg import problems
+    from oracle.pyoracle import OracleContext
+    P = problems.heterogeneous_3d(24, 20, 12, seed=4, patch=6, steps=4)
+    extra = {"linearize_fp32": 1}
+    if form == "mixed":
+        extra["precond_fp32"] = 1
+    g = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6, **extra))
+    o = OracleContext(P, ewg.default_params(newton_tolerance=1e-6))
+    for step in range(4):
+        kg0, ko0 = g.stats()["linear_iterations"], o.stats()["linear_iterations"]
+        rcg, ig, _ = g.newton(P.dt)
+        rco, io, _ = o.newton(P.dt)
+        g.advance_time_level()
+        o.advance_time_level()
+        xg, xo = g.solution(0), o.solution(0)
+        dS = np.abs(xg[1::2] - xo[1::2]).max()
+        dp = np.abs(xg[0::2] - xo[0::2]).max() / np.abs(xo[0::2]).max()
+        kg = g.stats()["linear_iterations"] - kg0
+        ko = o.stats()["linear_iterations"] - ko0
+        print(f"{form} step {step}: newton {ig} vs {io}, bicgstab {kg} vs {ko}, "
+              f"max|dS| {dS:.3e}, max|dp|/max|p| {dp:.3e}", flush=True)
+        assert rcg == 0 and rco == 0
+        assert abs(ig - io) <= 1
+        assert dS < 1e-6 and dp < 1e-6
