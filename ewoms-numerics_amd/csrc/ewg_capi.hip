@@ -14,6 +14,8 @@
 #include <vector>
 
 #include "../../include/ewg_capi.h"
+#include "../../include/ewg_cnn.h"
+#include "ewg_cnn_internal.h"
 #include "ewg_internal.h"
 #include "ewg_wavefront.h"
 
@@ -80,6 +82,12 @@ struct ewg_ctx {
     bool static_set = false;
     bool have_system = false;
     ewg_stats stats{};
+    // learned warm start (params.warm_start with an attached network)
+    ewg_cnn *cnn = nullptr;
+    bool no_recycle = false;   // storage of the previous level from solution(1)
+    int64_t warm_accepted = 0, warm_rejected = 0;
+    double warm_predict_s = 0.0;
+    double first_error = 0.0;  // Newton error at the first linearization of the last solve
     std::string err;
 };
 
@@ -146,7 +154,7 @@ LinArgs lin_args(ewg_ctx *c, double dt, int newton_iter) {
     a.dt = dt;
     a.newton_iter = newton_iter;
     a.enable_cache = c->prm.enable_storage_cache;
-    a.recycle = c->prm.recycle_first_iteration_storage;
+    a.recycle = c->prm.recycle_first_iteration_storage && !c->no_recycle;
     a.R = c->R;
     a.J = c->J;
     a.iqv = c->iqv;
@@ -389,6 +397,7 @@ int newton(ewg_ctx *c, double dt) {
         double t1 = now_s();
         c->stats.time_linearize_s += t1 - t0;
         c->error = e;
+        if (c->num_iterations == 0) c->first_error = e;
         if (c->error > c->prm.newton_max_error) {
             c->num_iterations = 2 * c->prm.newton_target_iterations;
             return set_err(c, EWG_NUMERICAL_ISSUE, "Newton: error above NewtonMaxError");
@@ -425,6 +434,30 @@ int newton(ewg_ctx *c, double dt) {
         return set_err(c, EWG_NOT_CONVERGED, "Newton did not converge");
     }
     return EWG_OK;
+}
+
+// One Newton solve of a time step. With a warm start, the solve starts from
+// solution(1) + the network's prediction; if it fails to converge from there, it is redone
+// from the plain guess solution(1).
+int newton_step(ewg_ctx *c, double dt) {
+    if (!(c->cnn && c->prm.warm_start)) return newton(c, dt);
+    double t0 = now_s();
+    int rc = cnn::predict_guess(c->cnn, c->x1, c->K, c->phi, dt, c->x0, c->stream);
+    if (rc != EWG_OK) return set_err(c, rc, "warm start: prediction failed");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->warm_predict_s += now_s() - t0;
+    c->no_recycle = true;
+    rc = newton(c, dt);
+    c->no_recycle = false;
+    if (rc == EWG_OK) {
+        c->warm_accepted++;
+        return rc;
+    }
+    if (rc < 0) return rc;
+    c->warm_rejected++;
+    HIP_TRY(c, hipMemcpyAsync(c->x0, c->x1, 2 * (size_t)c->g.N * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+    return newton(c, dt);
 }
 
 double suggest_dt(const ewg_ctx *c, double oldDt) {  // newtonmethod.hh:527-544
@@ -798,7 +831,7 @@ int ewg_solve(ewg_ctx *c, double *dx, int32_t *iterations) {
 int ewg_newton(ewg_ctx *c, double dt, int32_t *iterations, double *final_error) {
     if (!c || !(dt > 0.0)) return EWG_INVALID_ARGUMENT;
     if (!c->static_set) return set_err(c, EWG_INVALID_STATE, "static fields not set");
-    int rc = newton(c, dt);
+    int rc = newton_step(c, dt);
     if (iterations) *iterations = c->num_iterations;
     if (final_error) *final_error = c->error;
     c->stats.last_newton_iterations = c->num_iterations;
@@ -821,7 +854,7 @@ int ewg_run(ewg_ctx *c, double t0, double t_end, double dt0, int32_t *steps_out)
     while (!finished(t, dt)) {
         bool ok = false;
         for (int fails = 0; fails < c->prm.max_time_step_divisions; ++fails) {
-            int rc = newton(c, dt);
+            int rc = newton_step(c, dt);
             if (rc == EWG_OK) {
                 ok = true;
                 break;
@@ -869,6 +902,8 @@ int ewg_reset_stats(ewg_ctx *c) {
         sm.count = 0;
         sm.total_ms = 0.0;
     }
+    c->warm_accepted = c->warm_rejected = 0;
+    c->warm_predict_s = 0.0;
     return EWG_OK;
 }
 
@@ -937,6 +972,28 @@ int ewg_device_info(char *buf, int len) {
     }
     std::snprintf(buf, len, "%s (%s), %d CUs, %.1f GiB", p.name, p.gcnArchName,
                   p.multiProcessorCount, p.totalGlobalMem / 1073741824.0);
+    return EWG_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int ewg_set_warm_start(ewg_ctx *c, ewg_cnn *net) {
+    if (!c) return EWG_INVALID_ARGUMENT;
+    if (net && !cnn::fits_grid(net, c->grid.nx, c->grid.ny, c->grid.nz))
+        return set_err(c, EWG_INVALID_ARGUMENT, "warm start: network made for another grid");
+    c->cnn = net;
+    return EWG_OK;
+}
+
+int ewg_warm_start_stats(const ewg_ctx *c, int64_t *accepted, int64_t *rejected,
+                         double *predict_s, double *last_initial_error) {
+    if (!c) return EWG_INVALID_ARGUMENT;
+    if (last_initial_error) *last_initial_error = c->first_error;
+    if (accepted) *accepted = c->warm_accepted;
+    if (rejected) *rejected = c->warm_rejected;
+    if (predict_s) *predict_s = c->warm_predict_s;
     return EWG_OK;
 }
 

@@ -63,7 +63,8 @@ class Params(C.Structure):
                 ("ilu_schedule", C.c_int32),
                 ("kernel_events", C.c_int32),
                 ("linearize_fp32", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("warm_start", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):
@@ -104,6 +105,7 @@ def default_params(**overrides) -> Params:
     p.ilu_schedule = 0
     p.kernel_events = 0
     p.linearize_fp32 = 0
+    p.warm_start = 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

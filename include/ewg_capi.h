@@ -113,7 +113,10 @@ typedef struct ewg_params {
                                           into the FP64 residual and Jacobian (intensive
                                           quantities, storage terms and pressure differences
                                           stay FP64) */
-    int32_t reserved[4];
+    int32_t warm_start;                /* 1: start each Newton solve from the prediction of
+                                          the network attached with ewg_set_warm_start
+                                          (ewg_cnn.h); 0: from the previous time level */
+    int32_t reserved[3];
 } ewg_params;
 
 /* Counters and timers of the last calls; the split follows

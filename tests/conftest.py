@@ -13,14 +13,6 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libewg.so)")
 
 
-def gpu_available():
-    try:
-        import torch  # only used to ask whether a GPU is present
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def ewg_lib():
     import ewg

@@ -9,8 +9,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    text = open(os.path.join(ROOT, "include", "ewg_capi.h")).read()
+def _declared(header="ewg_capi.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(ewg_[a-z_0-9]+)\s*\(", text)))
 
@@ -27,8 +27,11 @@ def test_library_exports_every_declared_symbol():
     path = ewg.library_path()
     assert os.path.exists(path), "build libewg.so first (__graft_entry__.build())"
     lib = C.CDLL(path)
-    missing = [n for n in _declared() if not hasattr(lib, n)]
-    assert not missing, missing
+    for header in ("ewg_capi.h", "ewg_cnn.h"):
+        names = _declared(header)
+        assert names
+        missing = [n for n in names if not hasattr(lib, n)]
+        assert not missing, (header, missing)
 
 
 def test_default_params_match_reference_defaults():

@@ -1,0 +1,190 @@
+"""GPU: the warm-start network's kernels (BF16 operands, FP32 accumulation on the matrix
+cores) against the CPU FP64 restatement oracle/cnn_ref.py, its training, and its use as
+the Newton initial guess.
+
+Bars: against the reference that rounds to BF16 where the GPU stores BF16, outputs within
+2e-3 and gradients within 1e-2 (norm-wise per layer, FP32 summation order); against plain
+FP64 (BF16-rounded weights and inputs only) within 3e-2 and 5e-2.
+"""
+import numpy as np
+import pytest
+
+from oracle import cnn_ref as R
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (16, 6, 5)  # nx, ny, nz of one sample
+
+
+def _setup(batch, seed):
+    from ewg.cnn import DeviceArray, WarmStartNet, bf16_bits
+    nx, ny, nz = SHAPE
+    rng = np.random.default_rng(seed)
+    params = R.init_params(seed=seed, scale=2.0)
+    params = (params + rng.normal(0.0, 0.05, params.size)).astype(np.float32)
+    feat = np.zeros((batch, nz, ny, nx, 8))
+    feat[..., :5] = rng.normal(size=(batch, nz, ny, nx, 5))
+    net = WarmStartNet(nx, ny, nz, max_batch=batch, params=params)
+    fdev = DeviceArray.from_numpy(bf16_bits(feat).reshape(batch, -1))
+    return net, params, R.bf16_round(feat), fdev, rng
+
+
+def _layer_slices():
+    out, o = [], 0
+    for li, (ci, co) in enumerate(R.LAYERS):
+        out.append((f"W{li}", slice(o, o + ci * co * 27)))
+        o += ci * co * 27
+        out.append((f"b{li}", slice(o, o + co)))
+        o += co
+    return out
+
+
+def test_forward_matches_cpu_reference():
+    net, params, feat, fdev, _ = _setup(2, 11)
+    print(f"forward {SHAPE} batch 2", flush=True)
+    y = net.forward(fdev, 2).numpy().reshape(feat.shape[:-1] + (2,))
+    y16, _ = R.forward(params, feat, bf16=True)
+    y64, _ = R.forward(R.bf16_round(params), feat, bf16=False)
+    scale = np.abs(y64).max()
+    e16 = np.abs(y - y16).max() / scale
+    e64 = np.abs(y - y64).max() / scale
+    print(f"forward: vs bf16-emulating reference {e16:.2e}, vs FP64 {e64:.2e}", flush=True)
+    assert e16 < 2e-3 and e64 < 3e-2
+
+
+def test_gradients_match_cpu_reference():
+    from ewg.cnn import DeviceArray
+    net, params, feat, fdev, rng = _setup(2, 12)
+    print(f"gradients {SHAPE} batch 2", flush=True)
+    target = rng.normal(size=feat.shape[:-1] + (2,)).astype(np.float32)
+    tdev = DeviceArray.from_numpy(target.reshape(2, -1))
+    loss, g = net.gradients(fdev, tdev, 2)
+    l16, g16 = R.loss_and_grad(params, feat, target, bf16=True)
+    l64, g64 = R.loss_and_grad(R.bf16_round(params), feat, target, bf16=False)
+    assert abs(loss - l16) < 1e-3 * l16 and abs(loss - l64) < 3e-2 * l64
+    for name, sl in _layer_slices():
+        n16 = np.linalg.norm(g16[sl])
+        e16 = np.linalg.norm(g[sl] - g16[sl]) / n16
+        e64 = np.linalg.norm(g[sl] - g64[sl]) / np.linalg.norm(g64[sl])
+        print(f"{name}: |g| {n16:.3e}  vs bf16-emulating {e16:.2e}  vs FP64 {e64:.2e}",
+              flush=True)
+        assert e16 < 1e-2 and e64 < 5e-2, name
+
+
+def test_training_fits_a_teacher_network():
+    """Adam with the library's kernels drives the loss on a fixed batch down: targets are
+    the FP64 outputs of another network of the same shape."""
+    from ewg.cnn import DeviceArray
+    net, _, feat, fdev, _ = _setup(4, 13)
+    teacher = R.init_params(seed=99, scale=2.0)
+    target = R.forward(teacher, feat)[0].astype(np.float32)
+    tdev = DeviceArray.from_numpy(target.reshape(4, -1))
+    net.set_params(R.init_params(seed=5))
+    losses = [net.train_step(fdev, tdev, 4, 3e-3) for _ in range(150)]
+    print(f"teacher fit: loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+    assert losses[-1] < 0.2 * losses[0]
+
+
+def _member(steps):
+    from ewg import problems
+    return problems.heterogeneous_3d(32, 32, 16, seed=4242, sigma_lnK=1.0, corr_cells=6.0,
+                                     rate=1e-3, patch=4, steps=steps)
+
+
+def test_zero_network_reproduces_the_plain_newton_path():
+    """A network that predicts no update gives the plain guess; the storage of the old
+    level then comes from solution(1) instead of the first iteration: the same values, so
+    the run is bit-identical to the plain one."""
+    import ewg
+    from ewg import cnn
+    P = _member(4)
+    plain = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6))
+    warm = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6, warm_start=1))
+    net = cnn.WarmStartNet(P.nx, P.ny, P.nz)
+    net.set_norm({"p_ref": 1e5, "p_scale": 1e5, "lnk_ref": np.log(1e-12), "lnk_scale": 1.0,
+                  "phi_ref": 0.25, "phi_scale": 0.02, "dt_scale": 500.0, "dp_scale": 1.0,
+                  "ds_scale": 1.0})
+    cnn.attach(warm, net)
+    for step in range(4):
+        a = plain.newton(P.dt)
+        b = warm.newton(P.dt)
+        print(f"step {step}: plain {a}, warm {b}", flush=True)
+        assert a[0] == 0 and b[0] == 0 and a[1] == b[1]
+        plain.advance_time_level()
+        warm.advance_time_level()
+    assert np.array_equal(plain.solution(0), warm.solution(0))
+    st = cnn.warm_stats(warm)
+    assert st["accepted"] == 4 and st["rejected"] == 0
+
+
+def test_rejected_prediction_falls_back_to_the_plain_guess():
+    """A prediction whose residual exceeds NewtonMaxError makes the solve fail; the step
+    is redone from the plain guess and matches the plain run bit for bit. (Here: initial
+    errors 0.06 and 1e-3 from the plain guess, 3.9 with 2e5 Pa added everywhere.)"""
+    import ewg
+    from ewg import cnn
+    P = _member(3)
+    prm = dict(newton_tolerance=1e-6, newton_max_error=0.5)
+    plain = ewg.GpuContext(P, ewg.default_params(**prm))
+    warm = ewg.GpuContext(P, ewg.default_params(warm_start=1, **prm))
+    net = cnn.WarmStartNet(P.nx, P.ny, P.nz)
+    params = np.zeros(cnn.NUM_PARAMS, dtype=np.float32)
+    params[89120] = 1.0   # output bias: dp = 1 * dp_scale everywhere
+    net.set_params(params)
+    net.set_norm({"p_ref": 1e5, "p_scale": 1e5, "lnk_ref": np.log(1e-12), "lnk_scale": 1.0,
+                  "phi_ref": 0.25, "phi_scale": 0.02, "dt_scale": 500.0, "dp_scale": 2e5,
+                  "ds_scale": 1.0})
+    cnn.attach(warm, net)
+    for step in range(3):
+        a = plain.newton(P.dt)
+        b = warm.newton(P.dt)
+        print(f"step {step}: plain {a}, warm {b}", flush=True)
+        assert a[0] == 0 and b[0] == 0 and a[1] == b[1]
+        plain.advance_time_level()
+        warm.advance_time_level()
+    assert np.array_equal(plain.solution(0), warm.solution(0))
+    st = cnn.warm_stats(warm)
+    assert st["accepted"] == 0 and st["rejected"] == 3
+
+
+def test_trained_warm_start_gains_over_the_plain_guess():
+    """Train on (state, converged update) pairs harvested from plain runs of ensemble
+    members, then step a held-out member from the prediction: every step converges from
+    it, no step takes more Newton iterations than from the plain guess, and the Newton
+    error at the first linearization drops below the plain guess's (the first step, where
+    the pressure field settles from the initial condition, is not learned)."""
+    import ewg
+    from ewg import cnn
+    log = lambda m: print(m, flush=True)
+    data = cnn.harvest(300, steps_per_member=13, seed=3, crop=(32, 32, 16))
+    net = cnn.WarmStartNet(32, 32, 16, max_batch=8)
+    net.set_params(R.init_params(seed=1))
+    net.set_norm(cnn.fit_norm(data))
+    feats, targets = cnn.device_dataset(net, data)
+    hist = cnn.train(net, feats, targets, epochs=60, batch=8, lr=2e-3)
+    log(f"training: {len(data['x'])} pairs, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    assert hist[-1] < 0.5 * hist[0]
+    P = _member(10)
+    plain = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6))
+    warm = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6, warm_start=1))
+    pred = cnn.WarmStartNet(P.nx, P.ny, P.nz, params=net.params(), norm=net.norm())
+    cnn.attach(warm, pred)
+    ratios = []
+    for step in range(10):
+        a = plain.newton(P.dt)
+        e_plain = cnn.warm_stats(plain)["last_initial_error"]
+        b = warm.newton(P.dt)
+        e_warm = cnn.warm_stats(warm)["last_initial_error"]
+        plain.advance_time_level()
+        warm.advance_time_level()
+        dS = np.abs(plain.solution(0)[1::2] - warm.solution(0)[1::2]).max()
+        log(f"step {step}: newton plain {a[1]} warm {b[1]}; initial error plain {e_plain:.3e} "
+            f"warm {e_warm:.3e} (x{e_warm / e_plain:.3f}); max |dS| vs plain {dS:.2e}")
+        assert a[0] == 0 and b[0] == 0
+        assert b[1] <= a[1]
+        assert dS < 1e-5
+        ratios.append(e_warm / e_plain)
+    st = cnn.warm_stats(warm)
+    log(f"warm start: {st}")
+    assert st["rejected"] == 0
+    assert np.median(ratios[1:]) < 0.9 and ratios[-1] < 0.8
