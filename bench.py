@@ -3,16 +3,27 @@
 
 Workload: the 3D heterogeneous DNAPL case, 128 x 128 x 64 cells (1,048,576 cells, 2x2
 BCRS blocks on a 7-point stencil), dt = 500 s, Newton tolerance 1e-6, BiCGStab + ILU(0)
-with the reference's default linear tolerances, FP64 throughout ("fp64"). Reported beside
-it: the same run with FP32 ILU(0) factors inside the FP64 BiCGStab ("fp32_precond"), and
-with FP32 face fluxes in the linearization as well ("mixed"); for these the pressure and
-saturation deviation from the FP64 run and the Newton and BiCGStab iteration counts are
-reported for every time step (an untimed second pass).
+with the reference's default linear tolerances.
+
+Forms (each timed the same way):
+  fp64           FP64 throughout (the plain path);
+  fp32_precond   FP32 ILU(0) factors and sweeps inside the FP64 BiCGStab;
+  fp32_flux      FP32 face fluxes accumulated into the FP64 residual and Jacobian;
+  mixed          both of the above;
+  learned_mixed  mixed, with each Newton solve started from the warm-start network's
+                 prediction (BF16 matrix-core convolutions); the network is trained
+                 before, outside every timed region, on pairs harvested from other
+                 realizations.
+`value` is the metric's form, learned_mixed, when it runs (else fp64); the plain FP64
+value and the ratio are reported beside it. For each reduced-precision form a second,
+untimed pass steps it beside the FP64 path and reports the pressure and saturation
+deviation and the Newton and BiCGStab iterations of every time step.
 
 A step is one implicit-Euler time step driven to Newton convergence (linearization,
-ILU(0) factorization, BiCGStab, update, norms). W warm-up steps run first, the state is
-reset to the initial condition, and then exactly K steps from t = 0 are timed with the
-state resident in HBM (default K = 40, the whole configured run). value = cells*K/seconds.
+ILU(0) factorization, BiCGStab, update, norms; with the warm start also the prediction).
+W warm-up steps run first, the state is reset to the initial condition, and then exactly
+K steps from t = 0 are timed with the state resident in HBM (default K = 40, the whole
+configured run). value = cells*K/seconds.
 
 The CPU baseline is the C++ oracle (single thread, the reference's default
 ThreadsPerProcess = 1) timed on a bounded sample of the same workload on this host.
@@ -92,7 +103,20 @@ FORMS = {
     "fp32_precond": {"precond_fp32": 1},
     "fp32_flux": {"linearize_fp32": 1},
     "mixed": {"precond_fp32": 1, "linearize_fp32": 1},
+    "learned_mixed": {"precond_fp32": 1, "linearize_fp32": 1, "warm_start": 1},
 }
+FORM_DETAIL = {
+    "fp64": "FP64 throughout",
+    "fp32_precond": "FP32 ILU(0) factors and sweeps inside FP64 BiCGStab",
+    "fp32_flux": "FP32 face fluxes accumulated into the FP64 residual and Jacobian",
+    "mixed": "FP32 ILU(0) factors and sweeps + FP32 face fluxes; FP64 state, residual, "
+             "Newton norms, Krylov recurrences and update",
+    "learned_mixed": "mixed + Newton initial guess from the BF16 MFMA network (saturation "
+                     "update), judged by the unchanged Newton test with fallback; FP64 state, "
+                     "residual, Newton norms, Krylov recurrences and update",
+}
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+CNN_FLOP_PER_CELL = 2 * 88992  # multiply-adds of the five 3x3x3 convolutions, x2
 
 
 def form_params(form, **extra):
@@ -101,13 +125,16 @@ def form_params(form, **extra):
                               **extra)
 
 
-def per_step_deviation(P, form, steps):
+def per_step_deviation(P, form, steps, net=None):
     """Steps the form and the FP64 path side by side from the initial condition: max |dS_w|,
     max |dp_w| / max |p_w| of the FP64 state, Newton and BiCGStab iterations per step."""
     import numpy as np
     import ewg
     ref = ewg.GpuContext(P, form_params("fp64", max_time_step_size=P.dt))
     oth = ewg.GpuContext(P, form_params(form, max_time_step_size=P.dt))
+    if net is not None:
+        from ewg import cnn
+        cnn.attach(oth, net)
     out = {"max_dS": [], "max_rel_dp": [], "newton": [], "newton_fp64": [],
            "krylov": [], "krylov_fp64": []}
     for _ in range(steps):
@@ -130,6 +157,29 @@ def per_step_deviation(P, form, steps):
     ref.close()
     oth.close()
     return out
+
+
+def warm_start_network(P, args):
+    """Train the warm-start network (outside every timed region) and measure its forward
+    pass on the full grid: device time of the five convolutions and the matrix-core
+    utilisation at algorithmic FLOPs."""
+    import numpy as np
+    from ewg import cnn
+    net, info = cnn.train_warm_start(P.nx, P.ny, P.nz, n_pairs=args.train_pairs,
+                                     epochs=args.train_epochs, seed=args.train_seed)
+    feat = net.features(P.x0, P.K, P.phi, P.dt)
+    ms = []
+    for r in range(6):
+        net.forward(feat, 1)
+        if r:
+            ms.append(net.last_forward_ms())
+    fms = float(np.median(ms))
+    tflops = CNN_FLOP_PER_CELL * P.N / (fms * 1e-3) / 1e12
+    info["forward_ms"] = fms
+    info["forward_TFLOPs"] = round(tflops, 1)
+    info["mfma_frac"] = round(tflops / MFMA_BF16_PEAK_TFLOPS, 4)
+    print(f"# warm-start network: {info}", file=sys.stderr, flush=True)
+    return net, info
 
 
 def cpu_baseline(P, sample_steps):
@@ -159,7 +209,10 @@ def main():
     ap.add_argument("--nx", type=int, default=128)
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=64)
-    ap.add_argument("--forms", default="fp64,fp32_precond,mixed")
+    ap.add_argument("--forms", default="fp64,fp32_precond,mixed,learned_mixed")
+    ap.add_argument("--train-pairs", type=int, default=2000)
+    ap.add_argument("--train-epochs", type=int, default=30)
+    ap.add_argument("--train-seed", type=int, default=2)
     ap.add_argument("--no-deviation", action="store_true",
                     help="skip the untimed per-step comparison of the reduced-precision forms")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
@@ -182,12 +235,18 @@ def main():
     forms = {}
     kernels = {}
     copy_gbs = None
+    net, net_info = None, None
     for form in args.forms.split(","):
         if form not in FORMS:
             raise SystemExit(f"bench.py: unknown form {form!r} (known: {', '.join(FORMS)})")
         fp32 = bool(FORMS[form].get("precond_fp32"))
         params = form_params(form, max_time_step_size=P.dt, kernel_events=1)
         ctx = ewg.GpuContext(P, params)
+        if FORMS[form].get("warm_start"):
+            if net is None:
+                net, net_info = warm_start_network(P, args)
+            from ewg import cnn
+            cnn.attach(ctx, net)
         n, el, st = timed_run(ctx, P, args.warmup, args.steps)
         forms[form] = {"value": P.N * n / el, "steps": n, "seconds": el,
                        "newton_iterations": st["newton_iterations"],
@@ -213,9 +272,13 @@ def main():
             ks[name] = {"mean_ms": ms, "samples": cnt, "bytes": ab[name], "achieved_GBs": gbs,
                         "frac": gbs / HBM_PEAK_GBS}
         forms[form]["kernels"] = ks
+        if FORMS[form].get("warm_start"):
+            from ewg import cnn
+            ws = cnn.warm_stats(ctx)
+            forms[form]["warm_start"] = {"accepted": ws["accepted"], "rejected": ws["rejected"],
+                                         "predict_s": ws["predict_s"], "network": net_info}
         print(f"# {form} kernels {ks}", file=sys.stderr, flush=True)
         if form == "fp64":
-            kernels = ks
             ms = ctx.time_kernel(EWG_KERNEL_COPY, args.copy_reps)
             copy_bytes = 2 * 8 * 4 * ctx.nnzb
             copy_gbs = copy_bytes / (ms * 1e-3) / 1e9
@@ -223,14 +286,19 @@ def main():
                   file=sys.stderr, flush=True)
         ctx.close()
         if form != "fp64" and not args.no_deviation:
-            dev = per_step_deviation(P, form, args.steps)
+            dev = per_step_deviation(P, form, args.steps,
+                                     net if FORMS[form].get("warm_start") else None)
             forms[form]["vs_fp64"] = dev
             print(f"# {form} vs fp64: max dS {max(dev['max_dS']):.3e}, max rel dp "
                   f"{max(dev['max_rel_dp']):.3e}, newton {sum(dev['newton'])} vs "
                   f"{sum(dev['newton_fp64'])}, krylov {sum(dev['krylov'])} vs "
                   f"{sum(dev['krylov_fp64'])}", file=sys.stderr, flush=True)
 
-    head = forms.get("fp64") or next(iter(forms.values()))
+    # the metric's form: learned warm start + FP32 preconditioner (+ FP32 face fluxes) when
+    # it ran, compared with the plain FP64 path
+    head_form = next((f for f in ("learned_mixed", "fp64") if f in forms), next(iter(forms)))
+    head = forms[head_form]
+    kernels = head["kernels"]
     # dominant kernel: the largest share of the timed run's device time (per BiCGStab
     # iteration 2 ILU applies and 2 SpMV, per Newton iteration 1 linearization)
     dom_name = None
@@ -243,9 +311,12 @@ def main():
     roofline = None
     if dom_name:
         k = kernels[dom_name]
-        roofline = {"kernel": dom_name, "bound": "hbm", "achieved": round(k["achieved_GBs"], 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(k["frac"], 4),
-                    "traffic": pmc_traffic(dom_name)}
+        prec = "fp32" if FORMS[head_form].get("precond_fp32") and dom_name.startswith("ilu") \
+            else "fp64"
+        roofline = {"kernel": f"{dom_name} ({prec})", "bound": "hbm",
+                    "achieved": round(k["achieved_GBs"], 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(k["frac"], 4),
+                    "traffic": pmc_traffic(dom_name if prec == "fp64" else f"{dom_name}_fp32")}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(P, args.cpu_sample_steps)
@@ -264,7 +335,13 @@ def main():
         "data": "synthetic (seeded log-normal permeability, spectral synthesis)",
         "config": {"workload": f"3D heterogeneous two-phase DNAPL injection {args.nx}x{args.ny}x{args.nz}"
                                f" cells, dt {P.dt:g} s, Newton tol 1e-6, BiCGStab+ILU0",
-                   "form": "fp64", "cells": P.N, "seed": P.meta["seed"]},
+                   "form": head_form, "form_detail": FORM_DETAIL[head_form],
+                   "cells": P.N, "seed": P.meta["seed"]},
+        "plain_fp64": ({"value": forms["fp64"]["value"],
+                        "ms_per_step": 1e3 * forms["fp64"]["seconds"] / max(1, forms["fp64"]["steps"])}
+                       if "fp64" in forms else None),
+        "speedup_vs_plain_fp64": (round(head["value"] / forms["fp64"]["value"], 4)
+                                  if "fp64" in forms else None),
         "forms": forms,
         "kernels": kernels,
         "copy_GBs": round(copy_gbs, 1) if copy_gbs else None,

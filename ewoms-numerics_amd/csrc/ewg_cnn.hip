@@ -417,8 +417,11 @@ __global__ void k_guess(const double *xold, const float *y, int n, ewg_cnn_norm 
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const float2 yv = reinterpret_cast<const float2 *>(y)[c];
-    const double p = xold[2 * c] + (double)yv.x * nm.dp_scale;
-    double s = xold[2 * c + 1] - (double)yv.y * nm.ds_scale;
+    double dp = (double)yv.x * nm.dp_scale, dsn = (double)yv.y * nm.ds_scale;
+    dp = fabs(dp) >= nm.dp_min ? dp : 0.0;
+    dsn = fabs(dsn) >= nm.ds_min ? dsn : 0.0;
+    const double p = xold[2 * c] + dp;
+    double s = xold[2 * c + 1] - dsn;
     s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
     xg[2 * c] = p;
     xg[2 * c + 1] = s;
@@ -451,7 +454,7 @@ struct ewg_cnn {
     float *part = nullptr;
     double *lpart = nullptr;
     std::vector<double> lpart_host;
-    ewg_cnn_norm norm{0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 1.0, 1.0, 1.0};
+    ewg_cnn_norm norm{0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0, 0.0};
     double last_fwd_ms = 0.0;
     std::string err;
 };
@@ -785,6 +788,8 @@ int ewg_cnn_set_norm(ewg_cnn *n, const ewg_cnn_norm *norm) {
     if (!(norm->p_scale != 0.0 && norm->lnk_scale != 0.0 && norm->phi_scale != 0.0 &&
           norm->dt_scale != 0.0))
         return set_err(n, EWG_INVALID_ARGUMENT, "feature scales must be nonzero");
+    if (!(norm->dp_min >= 0.0 && norm->ds_min >= 0.0))
+        return set_err(n, EWG_INVALID_ARGUMENT, "thresholds must be non-negative");
     n->norm = *norm;
     return EWG_OK;
 }

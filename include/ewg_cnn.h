@@ -41,13 +41,15 @@ extern "C" {
 
 typedef struct ewg_cnn ewg_cnn;
 
-/* Feature and target scaling. */
+/* Feature and target scaling; the guess takes the predicted update only where it matters:
+ * dS_n where |dS_n| >= ds_min, dp_w where |dp_w| >= dp_min (0: everywhere). */
 typedef struct ewg_cnn_norm {
     double p_ref, p_scale;
     double lnk_ref, lnk_scale;
     double phi_ref, phi_scale;
     double dt_scale;
     double dp_scale, ds_scale;
+    double dp_min, ds_min;
 } ewg_cnn_norm;
 
 /* A network for samples of nx*ny*nz cells (nx a multiple of 16, ny, nz >= 1) and batches

@@ -147,30 +147,23 @@ def test_rejected_prediction_falls_back_to_the_plain_guess():
     assert st["accepted"] == 0 and st["rejected"] == 3
 
 
-def test_trained_warm_start_gains_over_the_plain_guess():
-    """Train on (state, converged update) pairs harvested from plain runs of ensemble
-    members, then step a held-out member from the prediction: every step converges from
-    it, no step takes more Newton iterations than from the plain guess, and the Newton
-    error at the first linearization drops below the plain guess's (the first step, where
-    the pressure field settles from the initial condition, is not learned)."""
+def test_trained_warm_start_on_a_held_out_member():
+    """Train on pairs harvested from plain runs of ensemble members, then step a held-out
+    member from the prediction: every step converges from it (none is rejected), none
+    takes more Newton iterations than from the plain guess, and the saturations stay
+    within the Newton tolerance's reach of the plain run."""
     import ewg
     from ewg import cnn
     log = lambda m: print(m, flush=True)
-    data = cnn.harvest(300, steps_per_member=13, seed=3, crop=(32, 32, 16))
-    net = cnn.WarmStartNet(32, 32, 16, max_batch=8)
-    net.set_params(R.init_params(seed=1))
-    net.set_norm(cnn.fit_norm(data))
-    feats, targets = cnn.device_dataset(net, data)
-    hist = cnn.train(net, feats, targets, epochs=60, batch=8, lr=2e-3)
-    log(f"training: {len(data['x'])} pairs, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
-    assert hist[-1] < 0.5 * hist[0]
-    P = _member(10)
+    P = _member(12)
+    pred, info = cnn.train_warm_start(P.nx, P.ny, P.nz, n_pairs=400, epochs=40, seed=3,
+                                      steps_per_member=20)
+    log(f"training: {info}")
+    assert info["loss_last"] < 0.1 * info["loss_first"]
     plain = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6))
     warm = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6, warm_start=1))
-    pred = cnn.WarmStartNet(P.nx, P.ny, P.nz, params=net.params(), norm=net.norm())
     cnn.attach(warm, pred)
-    ratios = []
-    for step in range(10):
+    for step in range(12):
         a = plain.newton(P.dt)
         e_plain = cnn.warm_stats(plain)["last_initial_error"]
         b = warm.newton(P.dt)
@@ -179,12 +172,52 @@ def test_trained_warm_start_gains_over_the_plain_guess():
         warm.advance_time_level()
         dS = np.abs(plain.solution(0)[1::2] - warm.solution(0)[1::2]).max()
         log(f"step {step}: newton plain {a[1]} warm {b[1]}; initial error plain {e_plain:.3e} "
-            f"warm {e_warm:.3e} (x{e_warm / e_plain:.3f}); max |dS| vs plain {dS:.2e}")
+            f"warm {e_warm:.3e}; max |dS| vs plain {dS:.2e}")
         assert a[0] == 0 and b[0] == 0
         assert b[1] <= a[1]
         assert dS < 1e-5
-        ratios.append(e_warm / e_plain)
     st = cnn.warm_stats(warm)
     log(f"warm start: {st}")
-    assert st["rejected"] == 0
-    assert np.median(ratios[1:]) < 0.9 and ratios[-1] < 0.8
+    assert st["accepted"] == 12 and st["rejected"] == 0
+
+
+def test_trained_warm_start_saves_work_on_the_1m_cell_case():
+    """The 128x128x64 heterogeneous case, 20 steps of 500 s at Newton tolerance 1e-6 with
+    the FP32 preconditioner and face fluxes, from the plain guess and from the network
+    (trained on 2000 pairs of other realizations): the prediction removes the steps whose
+    second Newton iteration just misses the tolerance, and with them most of the BiCGStab
+    work after the first step."""
+    import ewg
+    from ewg import cnn, problems
+    log = lambda m: print(m, flush=True)
+    P = problems.heterogeneous_3d(128, 128, 64)
+    pred, info = cnn.train_warm_start(P.nx, P.ny, P.nz, n_pairs=2000, epochs=30, seed=2)
+    log(f"training: {info}")
+    totals = {}
+    for form in ("plain", "warm"):
+        prm = ewg.default_params(newton_tolerance=1e-6, max_time_step_size=P.dt,
+                                 precond_fp32=1, linearize_fp32=1,
+                                 warm_start=1 if form == "warm" else 0)
+        ctx = ewg.GpuContext(P, prm)
+        if form == "warm":
+            cnn.attach(ctx, pred)
+        newton, krylov = [], []
+        for step in range(20):
+            k0 = ctx.stats()["linear_iterations"]
+            rc, it, err = ctx.newton(P.dt)
+            assert rc == 0
+            ctx.advance_time_level()
+            newton.append(it)
+            krylov.append(ctx.stats()["linear_iterations"] - k0)
+        totals[form] = (sum(newton), sum(krylov), ctx.solution(0))
+        log(f"{form}: newton {newton} krylov {krylov}")
+        if form == "warm":
+            st = cnn.warm_stats(ctx)
+            log(f"warm start: {st}")
+            assert st["accepted"] == 20 and st["rejected"] == 0
+        ctx.close()
+    (np_, kp, xp), (nw, kw, xw) = totals["plain"], totals["warm"]
+    dS = np.abs(xp[1::2] - xw[1::2]).max()
+    log(f"newton {np_} -> {nw}, bicgstab {kp} -> {kw}, max |dS| warm vs plain {dS:.2e}")
+    assert nw <= np_ and kw < 0.75 * kp
+    assert dS < 1e-5
