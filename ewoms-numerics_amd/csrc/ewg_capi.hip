@@ -219,24 +219,23 @@ bool use_wavefront_layout(const ewg_ctx *c) {
            (!c->prm.precond_fp32 || c->wl32.L);
 }
 
+// The factorization always runs in FP64; the FP32 preconditioner stores the factors and
+// the upper blocks rounded to FP32 (one rounding of the exact-arithmetic-order FP64
+// factors, the same for every schedule).
 void precond_factor(ewg_ctx *c) {
-    if (use_wavefront(c)) {
-        if (c->prm.precond_fp32)
-            wf_factor32(c->g, c->J, c->L32, c->D32, c->U32, c->ilu_err, c->wf, c->stream);
-        else
-            wf_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->wf, c->stream);
-        if (use_wavefront_layout(c)) {
-            if (c->prm.precond_fp32)
-                wl_pack32(c->g, c->wf, c->L32, c->D32, c->U32, c->wl32, c->stream);
-            else
-                wl_pack64(c->g, c->wf, c->L64, c->D64, c->J, c->wl64, c->stream);
-        }
-        return;
-    }
-    if (c->prm.precond_fp32)
-        launch_ilu_factor32(c->g, c->J, c->L32, c->D32, c->U32, c->ilu_err, c->stream);
+    const bool fp32 = c->prm.precond_fp32;
+    if (use_wavefront(c))
+        wf_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->wf, c->stream);
     else
         launch_ilu_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->stream);
+    if (use_wavefront_layout(c)) {
+        if (fp32)
+            wl_pack32(c->g, c->wf, c->L64, c->D64, c->J, c->wl32, c->stream);
+        else
+            wl_pack64(c->g, c->wf, c->L64, c->D64, c->J, c->wl64, c->stream);
+        return;
+    }
+    if (fp32) launch_round_factors(c->g, c->L64, c->D64, c->J, c->L32, c->D32, c->U32, c->stream);
 }
 
 void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status) {

@@ -77,8 +77,9 @@ void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s)
 // ILU(0)
 void launch_ilu_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
                          hipStream_t s);
-void launch_ilu_factor32(const Grid &g, const double *A, float *L, float *Dinv, float *U,
-                         int *err, hipStream_t s);
+// FP32 preconditioner: the FP64 factors and the Jacobian's upper blocks rounded to FP32
+void launch_round_factors(const Grid &g, const double *L, const double *Dinv, const double *A,
+                          float *L32, float *D32, float *U32, hipStream_t s);
 void launch_ilu_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                         const double *d, double *v, const int *status, hipStream_t s);
 void launch_ilu_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,

@@ -525,12 +525,21 @@ void launch_ilu_factor64(const Grid &g, const double *A, double *L, double *Dinv
         hipLaunchKernelGGL(k_ilu_factor_level<double>, dim3(blocks), dim3(256), 0, s, g, l, A, L,
                            Dinv, static_cast<double *>(nullptr), err);
 }
-void launch_ilu_factor32(const Grid &g, const double *A, float *L, float *Dinv, float *U,
-                         int *err, hipStream_t s) {
-    int threads = level_threads(g), blocks = (threads + 255) / 256;
-    for (int l = 0; l < g.nlevels; ++l)
-        hipLaunchKernelGGL(k_ilu_factor_level<float>, dim3(blocks), dim3(256), 0, s, g, l, A, L,
-                           Dinv, U, err);
+// FP32 copies of FP64 factors and of the Jacobian's upper blocks (rounded to nearest)
+__global__ void k_round_factors(size_t nl, size_t nd, const double *L, const double *D,
+                                const double *Aup, float *L32, float *D32, float *U32) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nl) {
+        L32[e] = static_cast<float>(L[e]);
+        U32[e] = static_cast<float>(Aup[e]);
+    }
+    if (e < nd) D32[e] = static_cast<float>(D[e]);
+}
+void launch_round_factors(const Grid &g, const double *L, const double *Dinv, const double *A,
+                          float *L32, float *D32, float *U32, hipStream_t s) {
+    const size_t nl = (size_t)g.dim * g.N * 4, nd = (size_t)g.N * 4;
+    hipLaunchKernelGGL(k_round_factors, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, nl,
+                       nd, L, Dinv, A + (size_t)(g.diag + 1) * g.N * 4, L32, D32, U32);
 }
 void launch_ilu_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                         const double *d, double *v, const int *status, hipStream_t s) {

@@ -770,10 +770,10 @@ __global__ void k_sweep_reset_ticket(unsigned *ticket) { *ticket = 0u; }
 // Copy the factors into the wavefront layout: one thread per layout entry. FWD: L (3
 // lower slots, forward order); otherwise U (3 upper slots; UT is double for the
 // Jacobian's blocks) and Dinv (backward order). Entries without a cell get zeros.
-template <class T, class UT, bool FWD>
+template <class T, class S, bool FWD>
 __global__ void __launch_bounds__(256)
-k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t N, const T *L,
-          const UT *U0, const UT *U1, const UT *U2, const T *Dinv, T *o0, T *o1, T *o2, T *od) {
+k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t N, const S *L,
+          const S *U0, const S *U1, const S *U2, const S *Dinv, T *o0, T *o1, T *o2, T *od) {
     const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= entries) return;
     const int lane = p & 63, q = p >> 6;
@@ -787,23 +787,23 @@ k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t 
         z = nz - 1 - z;
     }
     const size_t cell = ok ? (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z) : 0;
-    auto put = [&](T *o, const T *srcT, const UT *srcU) {
+    // values of type S (the factorization's), stored as T (FP32 rounds to nearest)
+    auto put = [&](T *o, const S *src) {
         T v[4] = {T(0), T(0), T(0), T(0)};
         if (ok) {
-            for (int e = 0; e < 4; ++e)
-                v[e] = srcT ? srcT[cell * 4 + e] : static_cast<T>(srcU[cell * 4 + e]);
+            for (int e = 0; e < 4; ++e) v[e] = static_cast<T>(src[cell * 4 + e]);
         }
         for (int e = 0; e < 4; ++e) o[(size_t)p * 4 + e] = v[e];
     };
     if (FWD) {
-        put(o0, L, nullptr);
-        put(o1, L + N * 4, nullptr);
-        put(o2, L + 2 * N * 4, nullptr);
+        put(o0, L);
+        put(o1, L + N * 4);
+        put(o2, L + 2 * N * 4);
     } else {
-        put(o0, nullptr, U0);
-        put(o1, nullptr, U1);
-        put(o2, nullptr, U2);
-        put(od, Dinv, nullptr);
+        put(o0, U0);
+        put(o1, U1);
+        put(o2, U2);
+        put(od, Dinv);
     }
 }
 
@@ -858,16 +858,16 @@ size_t wl_entries(const Grid &g, const WfSync &sync) {
     return (size_t)sync.ntiles * (size_t)(g.nz + 14) * 64;
 }
 
-template <class T, class UT>
-static void wl_pack(const Grid &g, const WfSync &sync, const T *L, const T *Dinv, const UT *U0,
-                    const UT *U1, const UT *U2, const WlPrecond<T> &wl, hipStream_t s) {
+template <class T, class S>
+static void wl_pack(const Grid &g, const WfSync &sync, const S *L, const S *Dinv, const S *U0,
+                    const S *U1, const S *U2, const WlPrecond<T> &wl, hipStream_t s) {
     const unsigned entries = (unsigned)wl_entries(g, sync);
     const int nsteps = g.nz + 14;
     const dim3 grid((entries + 255) / 256);
-    hipLaunchKernelGGL((k_wl_pack<T, UT, true>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz, sync.ntx,
+    hipLaunchKernelGGL((k_wl_pack<T, S, true>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz, sync.ntx,
                        nsteps, entries, (size_t)g.N, L, U0, U1, U2, Dinv, wl.L,
                        wl.L + (size_t)entries * 4, wl.L + (size_t)2 * entries * 4, nullptr);
-    hipLaunchKernelGGL((k_wl_pack<T, UT, false>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz,
+    hipLaunchKernelGGL((k_wl_pack<T, S, false>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz,
                        sync.ntx, nsteps, entries, (size_t)g.N, L, U0, U1, U2, Dinv, wl.U,
                        wl.U + (size_t)entries * 4, wl.U + (size_t)2 * entries * 4, wl.D);
 }
@@ -880,10 +880,12 @@ void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double 
                             wl, s);
 }
 
-void wl_pack32(const Grid &g, const WfSync &sync, const float *L, const float *Dinv,
-               const float *U, const WlPrecond<float> &wl, hipStream_t s) {
+void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
+               const double *A, const WlPrecond<float> &wl, hipStream_t s) {
     const size_t N = g.N;
-    wl_pack<float, float>(g, sync, L, Dinv, U, U + N * 4, U + 2 * N * 4, wl, s);
+    wl_pack<float, double>(g, sync, L, Dinv, A + (size_t)(g.diag + 1) * N * 4,
+                           A + (size_t)(g.diag + 2) * N * 4, A + (size_t)(g.diag + 3) * N * 4,
+                           wl, s);
 }
 
 template <class T>

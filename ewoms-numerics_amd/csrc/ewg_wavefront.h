@@ -36,8 +36,6 @@ void wf_init_edges(const WfSync &sync, hipStream_t s);
 
 void wf_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
                  const WfSync &sync, hipStream_t s);
-void wf_factor32(const Grid &g, const double *A, float *L, float *Dinv, float *U, int *err,
-                 const WfSync &sync, hipStream_t s);
 void wf_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                 const double *d, double *v, const int *status, const WfSync &sync,
                 hipStream_t s);
@@ -66,11 +64,12 @@ struct WlPrecond {
     T *w;  // [entries][2] forward-sweep result
 };
 size_t wl_entries(const Grid &g, const WfSync &sync);
-// copy freshly factored L, Dinv and the upper blocks (FP64: the Jacobian's; FP32: U)
+// copy freshly factored L, Dinv (FP64) and the Jacobian's upper blocks into the layout,
+// as FP64 or rounded to FP32
 void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
                const double *A, const WlPrecond<double> &wl, hipStream_t s);
-void wl_pack32(const Grid &g, const WfSync &sync, const float *L, const float *Dinv,
-               const float *U, const WlPrecond<float> &wl, hipStream_t s);
+void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
+               const double *A, const WlPrecond<float> &wl, hipStream_t s);
 // v = (LU)^-1 d with one compute, one edge and one publisher wavefront per tile
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
                 double *v, const int *status, hipStream_t s);

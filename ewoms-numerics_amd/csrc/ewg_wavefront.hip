@@ -525,11 +525,6 @@ void wf_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *e
     FactorOp<double> op{A, L, Dinv, nullptr, err, g.N, g.diag, g.nx, g.nxny};
     launch<FactorOp<double>, kWfPDFactor>(g, 0, op, sync, nullptr, s);
 }
-void wf_factor32(const Grid &g, const double *A, float *L, float *Dinv, float *U, int *err,
-                 const WfSync &sync, hipStream_t s) {
-    FactorOp<float> op{A, L, Dinv, U, err, g.N, g.diag, g.nx, g.nxny};
-    launch<FactorOp<float>, kWfPDFactor>(g, 0, op, sync, nullptr, s);
-}
 void wf_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                 const double *d, double *v, const int *status, const WfSync &sync,
                 hipStream_t s) {
