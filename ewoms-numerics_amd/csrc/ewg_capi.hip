@@ -16,80 +16,12 @@
 #include "../../include/ewg_capi.h"
 #include "../../include/ewg_cnn.h"
 #include "ewg_cnn_internal.h"
+#include "ewg_ctx.h"
 #include "ewg_internal.h"
 #include "ewg_wavefront.h"
 
 using namespace ewg;
 
-struct ewg_ctx {
-    Grid g{};
-    ewg_grid_desc grid{};
-    ewg_fluid_desc fluid{};
-    ewg_params prm{};
-    int law = 0;
-    int nregions = 0;
-    long long B = 0;
-    long long side_off[7] = {0};
-    hipStream_t stream = nullptr;
-    // device state
-    double *x0 = nullptr, *x1 = nullptr, *xcur = nullptr, *sto = nullptr;
-    double *R = nullptr, *J = nullptr;
-    double *K = nullptr, *phi = nullptr, *pe = nullptr;
-    int *region = nullptr;
-    MatReg *regs = nullptr;
-    int *bc_type = nullptr;
-    double *bc_val = nullptr;
-    // intensive quantities of the linearization's first pass
-    double *iqv = nullptr, *iqd = nullptr;
-    // preconditioner
-    double *L64 = nullptr, *D64 = nullptr;
-    float *L32 = nullptr, *D32 = nullptr, *U32 = nullptr, *w32 = nullptr;
-    // Krylov vectors
-    double *kr = nullptr, *kp = nullptr, *kv = nullptr, *ky = nullptr, *kz = nullptr,
-           *kx = nullptr;
-    double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr;
-    double *dscalar = nullptr;
-    int *anyNonzero = nullptr, *ilu_err = nullptr;
-    int *partAny = nullptr;
-    // tile-wavefront synchronisation (3D grids)
-    unsigned *wf_ticket = nullptr;
-    unsigned long long *wf_ex = nullptr, *wf_ey = nullptr;
-    int2 *wf_order = nullptr;
-    WfSync wf{};
-    // ILU(0) factors in the tiles' wavefront layout (schedule 0)
-    WlPrecond<double> wl64{};
-    WlPrecond<float> wl32{};
-    // event-timed samples of the hot launches (params.kernel_events), by EWG_KERNEL_* id
-    struct Sample {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        bool armed = false, pending = false;
-        int64_t count = 0;
-        double total_ms = 0.0;
-    } samples[4];
-    double *copy_dst = nullptr;  // destination of the bandwidth-reference copy
-    KrylovState *ks = nullptr;
-    // pinned host mirrors
-    KrylovState *ks_host = nullptr;
-    double *scal_host = nullptr;
-    int *err_host = nullptr;
-    // host copies of the BCRS pattern (rows ascending columns) and slot mapping
-    std::vector<int32_t> rowptr, colidx;
-    std::vector<int64_t> bcrs_src;   // BCRS block -> slot * N + row
-    // Newton state (newtonmethod.hh:197 starts error_ at 1e100)
-    double error = 1e100, last_error = 1e100;
-    int num_iterations = 0;
-    int last_linear_iterations = 0;
-    bool static_set = false;
-    bool have_system = false;
-    ewg_stats stats{};
-    // learned warm start (params.warm_start with an attached network)
-    ewg_cnn *cnn = nullptr;
-    bool no_recycle = false;   // storage of the previous level from solution(1)
-    int64_t warm_accepted = 0, warm_rejected = 0;
-    double warm_predict_s = 0.0;
-    double first_error = 0.0;  // Newton error at the first linearization of the last solve
-    std::string err;
-};
 
 namespace {
 
@@ -209,7 +141,8 @@ void sample_collect(ewg_ctx *c) {
 }
 
 bool use_wavefront(const ewg_ctx *c) {
-    return c->g.dim == 3 && c->prm.ilu_schedule != 1 && c->g.N <= kWfMaxCells;
+    return c->g.dim == 3 && c->prm.ilu_schedule != 1 && c->g.N <= kWfMaxCells &&
+           c->wf.ticket != nullptr;
 }
 
 // schedule 0: the sweeps read the factors in the wavefront layout (its buffers exist
@@ -471,6 +404,12 @@ double suggest_dt(const ewg_ctx *c, double oldDt) {  // newtonmethod.hh:527-544
 
 }  // namespace
 
+namespace ewg {
+LinArgs ctx_lin_args(ewg_ctx *c, double dt, int newton_iter) {
+    return lin_args(c, dt, newton_iter);
+}
+}  // namespace ewg
+
 extern "C" {
 
 void ewg_default_params(ewg_params *p) {
@@ -595,7 +534,8 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->ilu_err, 1);
     ALLOC(c->ks, 1);
     ALLOC(c->partAny, kRedBlocks);
-    if (g.dim == 3 && N <= kWfMaxCells) {
+    // the tile-wavefront buffers (3D; every schedule but the hyperplane launches)
+    if (g.dim == 3 && N <= kWfMaxCells && params->ilu_schedule != 1) {
         auto setup = [&](WfSync &w, int T, unsigned *&ticket, int2 *&dorder,
                          unsigned long long *&ex, unsigned long long *&ey) -> int {
             w.ntx = (g.nx + T - 1) / T;
@@ -701,6 +641,9 @@ int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
     if (!c || !p) return EWG_INVALID_ARGUMENT;
     if (p->ilu_schedule < 0 || p->ilu_schedule > 3)
         return set_err(c, EWG_INVALID_ARGUMENT, "ilu_schedule must be 0..3");
+    if (p->ilu_schedule != 1 && c->g.dim == 3 && c->g.N <= kWfMaxCells && !c->wf.ticket)
+        return set_err(c, EWG_INVALID_ARGUMENT,
+                       "a tile-wavefront schedule needs a context created with one");
     if (p->linearize_fp32 != 0 && p->linearize_fp32 != 1)
         return set_err(c, EWG_INVALID_ARGUMENT, "linearize_fp32 must be 0 or 1");
     if (p->precond_fp32) {

@@ -9,77 +9,13 @@
 // x + y + z = l are independent: factorization and both triangular sweeps run one
 // hyperplane per launch (exact natural-order ILU(0); the summation order inside each row
 // is the sequential one).
-#include "ewg_internal.h"
+#include "ewg_cell_ops.h"
 
 namespace ewg {
 
 namespace {
 
-template <class T>
-__device__ __forceinline__ void bmul(const T *a, const T *b, T *out) {
-    T r0 = T(0), r1 = T(0), r2 = T(0), r3 = T(0);
-    r0 += a[0] * b[0]; r0 += a[1] * b[2];
-    r1 += a[0] * b[1]; r1 += a[1] * b[3];
-    r2 += a[2] * b[0]; r2 += a[3] * b[2];
-    r3 += a[2] * b[1]; r3 += a[3] * b[3];
-    out[0] = r0; out[1] = r1; out[2] = r2; out[3] = r3;
-}
-
-template <class T>
-__device__ __forceinline__ void bmmv(const T *a, const T *x, T *y) {  // y -= a x
-    y[0] -= a[0] * x[0];
-    y[0] -= a[1] * x[1];
-    y[1] -= a[2] * x[0];
-    y[1] -= a[3] * x[1];
-}
-
-template <class T>
-__device__ __forceinline__ void bmv(const T *a, const T *x, T *y) {  // y = a x
-    T s0 = T(0), s1 = T(0);
-    s0 += a[0] * x[0]; s0 += a[1] * x[1];
-    s1 += a[2] * x[0]; s1 += a[3] * x[1];
-    y[0] = s0; y[1] = s1;
-}
-
-template <class T>
-__device__ __forceinline__ void load_blk(const T *base, size_t idx, T *o) {
-    const T *p = base + idx * 4;
-    o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = p[3];
-}
-template <class T>
-__device__ __forceinline__ void load_blk_cast(const double *base, size_t idx, T *o) {
-    const double *p = base + idx * 4;
-    o[0] = static_cast<T>(p[0]); o[1] = static_cast<T>(p[1]);
-    o[2] = static_cast<T>(p[2]); o[3] = static_cast<T>(p[3]);
-}
-template <class T>
-__device__ __forceinline__ void store_blk(T *base, size_t idx, const T *o) {
-    T *p = base + idx * 4;
-    p[0] = o[0]; p[1] = o[1]; p[2] = o[2]; p[3] = o[3];
-}
-
-// map thread -> cell of hyperplane `level`; returns -1 if the thread has no cell
-__device__ __forceinline__ int level_cell(const Grid &g, int level, int t, int ijk[3]) {
-    int y, z;
-    if (g.dim == 3) {
-        if (t >= g.ny * g.nz) return -1;
-        y = t % g.ny;
-        z = t / g.ny;
-    } else {
-        if (t >= g.ny) return -1;
-        y = t;
-        z = 0;
-    }
-    int x = level - y - z;
-    if (x < 0 || x >= g.nx) return -1;
-    ijk[0] = x; ijk[1] = y; ijk[2] = z;
-    return x + g.nx * y + g.nxny * z;
-}
-
-__device__ __forceinline__ int lower_axis(const Grid &g, int s) { return g.dim - 1 - s; }
-__device__ __forceinline__ int axis_stride(const Grid &g, int a) {
-    return a == 0 ? 1 : (a == 1 ? g.nx : g.nxny);
-}
+using namespace cell;
 
 template <class T>
 __global__ void __launch_bounds__(256)
@@ -87,38 +23,7 @@ k_ilu_factor_level(Grid g, int level, const double *A, T *L, T *Dinv, T *U, int 
     int ijk[3];
     int i = level_cell(g, level, blockIdx.x * blockDim.x + threadIdx.x, ijk);
     if (i < 0) return;
-    T D[4];
-    load_blk_cast<T>(A, (size_t)g.diag * g.N + i, D);
-    for (int s = 0; s < g.dim; ++s) {  // lower slots, ascending column
-        int ax = lower_axis(g, s);
-        T Lij[4] = {T(0), T(0), T(0), T(0)};
-        if (ijk[ax] > 0) {
-            int j = i - axis_stride(g, ax);
-            T Aij[4], Dj[4], Aji[4], Bm[4];
-            load_blk_cast<T>(A, (size_t)s * g.N + i, Aij);
-            load_blk<T>(Dinv, j, Dj);
-            bmul(Aij, Dj, Lij);
-            load_blk_cast<T>(A, (size_t)(g.diag + 1 + ax) * g.N + j, Aji);
-            bmul(Lij, Aji, Bm);
-            for (int e = 0; e < 4; ++e) D[e] -= Bm[e];
-        }
-        store_blk(L, (size_t)s * g.N + i, Lij);
-    }
-    T det = D[0] * D[3] - D[1] * D[2];
-    if (!(fabs(static_cast<double>(det)) >= 1e-30)) {
-        atomicExch(err, 1);
-        det = T(1);
-    }
-    T det_1 = T(1) / det;
-    T inv[4] = {D[3] * det_1, -D[1] * det_1, -D[2] * det_1, D[0] * det_1};
-    store_blk(Dinv, i, inv);
-    if (U) {
-        for (int s = 0; s < g.dim; ++s) {
-            T Ui[4];
-            load_blk_cast<T>(A, (size_t)(g.diag + 1 + s) * g.N + i, Ui);
-            store_blk(U, (size_t)s * g.N + i, Ui);
-        }
-    }
+    if (!ilu_factor_cell<T>(g, i, ijk, A, L, Dinv, U)) atomicExch(err, 1);
 }
 
 // forward sweep: w_i = d_i - sum_j L_ij w_j
@@ -129,18 +34,7 @@ k_ilu_fwd_level(Grid g, int level, const T *L, const double *d, T *w, const int 
     int ijk[3];
     int i = level_cell(g, level, blockIdx.x * blockDim.x + threadIdx.x, ijk);
     if (i < 0) return;
-    T rhs[2] = {static_cast<T>(d[2 * i]), static_cast<T>(d[2 * i + 1])};
-    for (int s = 0; s < g.dim; ++s) {
-        int ax = lower_axis(g, s);
-        if (ijk[ax] == 0) continue;
-        int j = i - axis_stride(g, ax);
-        T Lij[4];
-        load_blk<T>(L, (size_t)s * g.N + i, Lij);
-        T wj[2] = {w[2 * j], w[2 * j + 1]};
-        bmmv(Lij, wj, rhs);
-    }
-    w[2 * i] = rhs[0];
-    w[2 * i + 1] = rhs[1];
+    ilu_fwd_cell<T>(g, i, ijk, L, d, w);
 }
 
 // backward sweep: w_i = inv(D_i) (w_i - sum_k U_ik w_k), out = (double) w
@@ -152,27 +46,7 @@ k_ilu_bwd_level(Grid g, int level, const UT *U, int uslot0, const T *Dinv, T *w,
     int ijk[3];
     int i = level_cell(g, level, blockIdx.x * blockDim.x + threadIdx.x, ijk);
     if (i < 0) return;
-    int n[3] = {g.nx, g.ny, g.nz};
-    T rhs[2] = {w[2 * i], w[2 * i + 1]};
-    for (int s = 0; s < g.dim; ++s) {  // upper slots ascending: +x, +y, +z
-        if (ijk[s] + 1 >= n[s]) continue;
-        int k = i + axis_stride(g, s);
-        const UT *up = U + ((size_t)(uslot0 + s) * g.N + i) * 4;
-        T Uik[4] = {static_cast<T>(up[0]), static_cast<T>(up[1]), static_cast<T>(up[2]),
-                    static_cast<T>(up[3])};
-        T wk[2] = {w[2 * k], w[2 * k + 1]};
-        bmmv(Uik, wk, rhs);
-    }
-    T Di[4];
-    load_blk<T>(Dinv, i, Di);
-    T res[2];
-    bmv(Di, rhs, res);
-    w[2 * i] = res[0];
-    w[2 * i + 1] = res[1];
-    if (out != reinterpret_cast<double *>(w)) {
-        out[2 * i] = static_cast<double>(res[0]);
-        out[2 * i + 1] = static_cast<double>(res[1]);
-    }
+    ilu_bwd_cell<T, UT>(g, i, ijk, U, uslot0, Dinv, w, out);
 }
 
 inline int level_threads(const Grid &g) { return g.dim == 3 ? g.ny * g.nz : g.ny; }
@@ -232,34 +106,8 @@ k_spmv(Grid g, const double *A, const double *x, double *y, const double *a, dou
     if (status && *status != KS_RUNNING) return;
     double accA = 0.0, accB = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += gridDim.x * blockDim.x) {
-        int ijk[3];
-        ijk[0] = i % g.nx;
-        int r = i / g.nx;
-        ijk[1] = r % g.ny;
-        ijk[2] = r / g.ny;
-        int n[3] = {g.nx, g.ny, g.nz};
-        double s0 = 0.0, s1 = 0.0;
-        for (int slot = 0; slot < g.nslots; ++slot) {
-            int off;
-            if (slot == g.diag) {
-                off = 0;
-            } else if (slot < g.diag) {
-                int ax = g.dim - 1 - slot;
-                if (ijk[ax] == 0) continue;
-                off = -(ax == 0 ? 1 : (ax == 1 ? g.nx : g.nxny));
-            } else {
-                int ax = slot - g.diag - 1;
-                if (ijk[ax] + 1 >= n[ax]) continue;
-                off = (ax == 0 ? 1 : (ax == 1 ? g.nx : g.nxny));
-            }
-            const double2 *blk = reinterpret_cast<const double2 *>(A + ((size_t)slot * g.N + i) * 4);
-            double2 r0 = blk[0], r1 = blk[1];
-            double2 xv = reinterpret_cast<const double2 *>(x)[i + off];
-            s0 += r0.x * xv.x;
-            s0 += r0.y * xv.y;
-            s1 += r1.x * xv.x;
-            s1 += r1.y * xv.y;
-        }
+        const double2 yv = spmv_cell(g, i, A, x);
+        const double s0 = yv.x, s1 = yv.y;
         reinterpret_cast<double2 *>(y)[i] = make_double2(s0, s1);
         if (partA) {
             double2 av = reinterpret_cast<const double2 *>(a)[i];

@@ -1,399 +1,26 @@
-// ewg_linearize.hip -- ECFV forward-AD linearization of the immiscible two-phase model.
-//
-// One thread per cell evaluates the element context of its cell exactly as
-// FvBaseAdLocalLinearizer::linearize does (fvbaseadlocallinearizer.hh:178-204): the cell
-// is the focus DOF, its (p_w, S_w) carry the two derivative slots, neighbour quantities
-// enter as values only. The thread writes
-//   R[cell]                         value of the focus residual,
-//   J[diag][cell]                   derivative of the focus residual,
-//   J[mirror(dir)][neighbour]       derivative of the neighbour's residual w.r.t. the
-//                                   focus PVs, i.e. column `cell` of row `neighbour`
-//                                   (fvbaselinearizer.hh:495-508),
-// so every block is written by exactly one thread (no atomics, bitwise reproducible),
-// and with the slot-major layout each of the 7 block stores is a unit-stride stream.
-//
-// The intensive quantities are evaluated once per cell by a first pass (k_iq, with the
-// cell's two derivative slots) and stored; the flux pass reads the neighbours' values
-// (the value part of the dual evaluation is the plain evaluation, operation for
-// operation) instead of evaluating the material laws of all six neighbours again.
-//
-// The FP32 flux form (LinArgs::flux_fp32) evaluates each face flux and its derivatives in
-// float duals; the pressure differences are taken in double before the narrowing, and the
-// residual, the Jacobian blocks and the storage term stay double.
-#include "ewg_internal.h"
+// ewg_linearize.hip -- ECFV forward-AD linearization of the immiscible two-phase model over
+// the whole grid: one thread per cell, first the intensive quantities (k_iq), then the
+// element context of every cell as focus DOF (k_linearize); the per-cell code, with the
+// reference functions it follows, is in ewg_lin_device.h.
+#include "ewg_lin_device.h"
 
 namespace ewg {
 
 namespace {
 
-template <class T>
-struct IQ {
-    T p[2];
-    T Sw;
-    T lam[2];
-};
+using namespace lin;
 
-__device__ __forceinline__ void cell_coords(const Grid &g, int i, int ijk[3]) {
-    ijk[0] = i % g.nx;
-    int r = i / g.nx;
-    ijk[1] = r % g.ny;
-    ijk[2] = r / g.ny;
-}
-
-__device__ __forceinline__ int stride_of(const Grid &g, int axis) {
-    return axis == 0 ? 1 : (axis == 1 ? g.nx : g.nxny);
-}
-
-__device__ __forceinline__ bool has_neighbour(const Grid &g, const int ijk[3], int dir) {
-    int a = dir >> 1;
-    int n = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
-    return (dir & 1) ? (ijk[a] + 1 < n) : (ijk[a] > 0);
-}
-
-__device__ __forceinline__ int slot_of_dir(const Grid &g, int dir) {
-    int a = dir >> 1;
-    return (dir & 1) ? g.diag + 1 + a : g.diag - 1 - a;
-}
-
-__device__ __forceinline__ void centre(const Grid &g, const int ijk[3], double pos[3]) {
-    pos[0] = (ijk[0] + 0.5) * g.h[0];
-    pos[1] = (ijk[1] + 0.5) * g.h[1];
-    pos[2] = (ijk[2] + 0.5) * g.h[2];
-}
-
-__device__ __forceinline__ void face_centre(const Grid &g, const int ijk[3], int dir,
-                                            double pos[3]) {
-    centre(g, ijk, pos);
-    int a = dir >> 1;
-    pos[a] = ((dir & 1) ? (ijk[a] + 1) : ijk[a]) * g.h[a];
-}
-
-template <int DIM>
-__device__ __forceinline__ double dot_g(const LinArgs &a, const double v[3]) {
-    double s = 0.0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) s += a.grav[d] * v[d];
-    return s;
-}
-
-__device__ __forceinline__ const MatReg &region_of(const LinArgs &a, int cell) {
-    return a.regs[a.region ? a.region[cell] : 0];
-}
-
-// Scalar type S of the flux arithmetic: double, or float for the FP32 flux form, where the
-// pressure differences are still taken in double (they cancel values of order 1e5 Pa) and
-// the face fluxes are accumulated into the double residual and Jacobian.
-template <class S>
-__device__ __forceinline__ Dual<S> cst(double x) {
-    return Dual<S>{static_cast<S>(x), S(0), S(0)};
-}
-template <class S>
-__device__ __forceinline__ Dual<S> narrow(const Ev &x);
-template <>
-__device__ __forceinline__ Ev narrow<double>(const Ev &x) {
-    return x;
-}
-template <>
-__device__ __forceinline__ Evf narrow<float>(const Ev &x) {
-    return to_f32(x);
-}
-__device__ __forceinline__ Ev widen(const Ev &x) { return x; }
-__device__ __forceinline__ Ev widen(const Evf &x) { return to_f64(x); }
-
-// ImmiscibleIntensiveQuantities::update (immiscibleintensivequantities.hh:87-145)
-template <class T>
-__device__ __forceinline__ IQ<T> intensive(const LinArgs &a, int cell, T p0, T Sw) {
-    const MatReg &r = region_of(a, cell);
-    double pe = a.pe ? a.pe[cell] : 0.0;
-    IQ<T> q;
-    T pc = pcnw(r, pe, Sw);
-    T krw, krn;
-    relperm(r, Sw, krw, krn);
-    q.p[0] = p0;
-    q.p[1] = p0 + pc;
-    q.Sw = Sw;
-    q.lam[0] = krw / lift(krw, a.mu[0]);
-    q.lam[1] = krn / lift(krn, a.mu[1]);
-    return q;
-}
-
-// M_k = phi S_k rho_k (immisciblelocalresidual.hh:69-103)
-template <class T>
-__device__ __forceinline__ void storage(const LinArgs &a, double phi, T Sw, T M[2]) {
-    M[0] = (phi * Sw) * a.rho[0];
-    M[1] = (phi * (1.0 - Sw)) * a.rho[1];
-}
-
-// Interior face flux from the focus side: fvbasegradientcalculator.hh:195-240,
-// darcyfluxmodule.hh:185-324, :461-527, immisciblelocalresidual.hh:123-149.
-template <int DIM, class S>
-__device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in,
-                                              const IQ<double> &ex, double Kin, double Kex,
-                                              const int ijk[3], int dir, Ev F[2]) {
-    using D = Dual<S>;
-    const Grid &g = a.g;
-    constexpr int dim = DIM;
-    int q[3] = {ijk[0], ijk[1], ijk[2]};
-    q[dir >> 1] += (dir & 1) ? 1 : -1;
-    double posIn[3], posEx[3], posFace[3];
-    centre(g, ijk, posIn);
-    centre(g, q, posEx);
-    face_centre(g, ijk, dir, posFace);
-    double normal[3] = {0.0, 0.0, 0.0};
-    normal[dir >> 1] = (dir & 1) ? 1.0 : -1.0;
-
-    double distSq = 0.0;
-    for (int d = 0; d < dim; ++d) {
-        double t = posEx[d] - posIn[d];
-        distSq += t * t;
-    }
-    D grad[2][3];
-    for (int k = 0; k < 2; ++k) {
-        D deltay = narrow<S>(ev(ex.p[k]) - in.p[k]);
-        for (int d = 0; d < dim; ++d)
-            grad[k][d] = deltay * static_cast<S>((posEx[d] - posIn[d]) / distSq);
-    }
-    double dvIn[3], dvEx[3], dvTot[3];
-    for (int d = 0; d < 3; ++d) {
-        dvIn[d] = posIn[d] - posFace[d];
-        dvEx[d] = posEx[d] - posFace[d];
-        dvTot[d] = posEx[d] - posIn[d];
-    }
-    double absDistTotSq = 0.0;
-    for (int d = 0; d < dim; ++d) absDistTotSq += dvTot[d] * dvTot[d];
-    double gIn = dot_g<DIM>(a, dvIn), gEx = dot_g<DIM>(a, dvEx);
-    for (int k = 0; k < 2; ++k) {
-        D pStatIn = -cst<S>(a.rho[k]) * static_cast<S>(gIn);
-        D pStatEx = cst<S>(-a.rho[k] * gEx);
-        D qq = (pStatEx - pStatIn) / static_cast<S>(absDistTotSq);
-        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + cst<S>(dvTot[d]) * qq;
-    }
-    const S Kf = static_cast<S>(harmonic_mean(Kin, Kex));
-    for (int k = 0; k < 2; ++k) {
-        D tmp = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * static_cast<S>(normal[d]);
-        bool upIsEx = tmp.v > S(0);
-        D lam = upIsEx ? cst<S>(ex.lam[k]) : narrow<S>(in.lam[k]);
-        D v[3];
-        for (int r = 0; r < dim; ++r) {
-            v[r] = cst<S>(0.0);
-            for (int c = 0; c < dim; ++c) v[r] = v[r] + (r == c ? Kf : S(0)) * grad[k][c];
-        }
-        D mlam = -lam;
-        for (int r = 0; r < dim; ++r) v[r] = v[r] * mlam;
-        D vol = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) vol = vol + v[d] * static_cast<S>(normal[d]);
-        F[k] = widen(cst<S>(0.0) + vol * cst<S>(a.rho[k]));
-    }
-}
-
-// Dirichlet free-flow boundary: darcyfluxmodule.hh:332-453, :488-509,
-// fvbasegradientcalculator.hh:278-312, immiscibleboundaryratevector.hh:91-153.
-template <int DIM, class S>
-__device__ __forceinline__ void freeflow_flux(const LinArgs &a, const IQ<Ev> &in, int cell,
-                                              double Kc, const int ijk[3], int dir,
-                                              double pwB, double SwB, Ev B[2]) {
-    using D = Dual<S>;
-    const Grid &g = a.g;
-    constexpr int dim = DIM;
-    const MatReg &r = region_of(a, cell);
-    double pe = a.pe ? a.pe[cell] : 0.0;
-    double pcB = pcnw(r, pe, SwB);
-    double pB[2] = {pwB, (pwB + pcB) - 0.0};
-    double krB[2];
-    relperm(r, SwB, krB[0], krB[1]);
-    double posIn[3], posFace[3];
-    centre(g, ijk, posIn);
-    face_centre(g, ijk, dir, posFace);
-    double normal[3] = {0.0, 0.0, 0.0};
-    normal[dir >> 1] = (dir & 1) ? 1.0 : -1.0;
-    double distSq = 0.0;
-    for (int d = 0; d < dim; ++d) {
-        double t = posFace[d] - posIn[d];
-        distSq += t * t;
-    }
-    D grad[2][3];
-    for (int k = 0; k < 2; ++k) {
-        D deltay = narrow<S>(pB[k] - in.p[k]);
-        for (int d = 0; d < dim; ++d)
-            grad[k][d] = deltay * static_cast<S>((posFace[d] - posIn[d]) / distSq);
-    }
-    double dvIn[3];
-    for (int d = 0; d < 3; ++d) dvIn[d] = posIn[d] - posFace[d];
-    double absDistSq = 0.0;
-    for (int d = 0; d < dim; ++d) absDistSq += dvIn[d] * dvIn[d];
-    double gTimesDist = dot_g<DIM>(a, dvIn);
-    for (int k = 0; k < 2; ++k) {
-        D pStatIn = static_cast<S>(-gTimesDist) * cst<S>(a.rho[k]);
-        D qq = pStatIn / static_cast<S>(absDistSq);
-        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + cst<S>(dvIn[d]) * qq;
-    }
-    const S Ks = static_cast<S>(Kc);
-    for (int k = 0; k < 2; ++k) {
-        D tmp = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * static_cast<S>(normal[d]);
-        bool upIsBoundary = tmp.v > S(0);
-        D mob = upIsBoundary ? cst<S>(krB[k] / a.mu[k]) : narrow<S>(in.lam[k]);
-        D v[3];
-        for (int rr = 0; rr < dim; ++rr) {
-            v[rr] = cst<S>(0.0);
-            for (int c = 0; c < dim; ++c) v[rr] = v[rr] + (rr == c ? Ks : S(0)) * grad[k][c];
-        }
-        D mmob = -mob;
-        for (int rr = 0; rr < dim; ++rr) v[rr] = v[rr] * mmob;
-        D vol = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) vol = vol + v[d] * static_cast<S>(normal[d]);
-        // density by pressure comparison: both candidates are the same constant here
-        double dens = a.rho[k];
-        B[k] = widen(cst<S>(0.0) + vol * cst<S>(dens));
-    }
-}
-
-__device__ __forceinline__ long long boundary_face(const LinArgs &a, const int ijk[3], int dir) {
-    const Grid &g = a.g;
-    int ax = dir >> 1;
-    long long loc;
-    if (ax == 0) loc = ijk[1] + (long long)g.ny * ijk[2];
-    else if (ax == 1) loc = ijk[0] + (long long)g.nx * ijk[2];
-    else loc = ijk[0] + (long long)g.nx * ijk[1];
-    return a.side_off[dir] + loc;
-}
-
-__device__ __forceinline__ void store_block(double *J, int slot, int N, int cell, const Ev r[2]) {
-    double2 *b = reinterpret_cast<double2 *>(J + ((size_t)slot * N + cell) * 4);
-    b[0] = make_double2(r[0].d0, r[0].d1);
-    b[1] = make_double2(r[1].d0, r[1].d1);
-}
-
-// first pass: iqv[cell] = (p_w, p_n, lambda_w, lambda_n) values, iqd[cell] = derivatives
-// of (p_n, lambda_w, lambda_n) w.r.t. the cell's (p_w, S_w)
 __global__ void __launch_bounds__(256) k_iq(LinArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.g.N) return;
-    const double2 xi = reinterpret_cast<const double2 *>(a.x)[i];
-    const IQ<Ev> q = intensive<Ev>(a, i, Ev{xi.x, 1.0, 0.0}, Ev{xi.y, 0.0, 1.0});
-    double4 v;
-    v.x = q.p[0].v;
-    v.y = q.p[1].v;
-    v.z = q.lam[0].v;
-    v.w = q.lam[1].v;
-    reinterpret_cast<double4 *>(a.iqv)[i] = v;
-    double2 *d = reinterpret_cast<double2 *>(a.iqd + (size_t)i * 6);
-    d[0] = make_double2(q.p[1].d0, q.p[1].d1);
-    d[1] = make_double2(q.lam[0].d0, q.lam[0].d1);
-    d[2] = make_double2(q.lam[1].d0, q.lam[1].d1);
-}
-
-__device__ __forceinline__ IQ<double> iq_values(const LinArgs &a, int cell) {
-    const double4 v = reinterpret_cast<const double4 *>(a.iqv)[cell];
-    IQ<double> q;
-    q.p[0] = v.x;
-    q.p[1] = v.y;
-    q.Sw = 0.0;  // not used by the flux
-    q.lam[0] = v.z;
-    q.lam[1] = v.w;
-    return q;
+    iq_cell(a, i);
 }
 
 template <int DIM, class S>
 __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
-    const Grid &g = a.g;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.N) return;
-    int ijk[3];
-    cell_coords(g, i, ijk);
-    const double2 xi = reinterpret_cast<const double2 *>(a.x)[i];
-    const double Ki = a.K[i];
-    IQ<Ev> in;
-    {
-        const double4 v = reinterpret_cast<const double4 *>(a.iqv)[i];
-        const double2 *d = reinterpret_cast<const double2 *>(a.iqd + (size_t)i * 6);
-        const double2 d0 = d[0], d1 = d[1], d2 = d[2];
-        in.p[0] = Ev{xi.x, 1.0, 0.0};
-        in.Sw = Ev{xi.y, 0.0, 1.0};
-        in.p[1] = Ev{v.y, d0.x, d0.y};
-        in.lam[0] = Ev{v.z, d1.x, d1.y};
-        in.lam[1] = Ev{v.w, d2.x, d2.y};
-    }
-    Ev res0[2] = {ev(0.0), ev(0.0)};
-    constexpr int nfaces = 2 * DIM;
-    // fluxes over the interior faces, intersection order (fvbaselocalresidual.hh:295-367)
-#pragma unroll
-    for (int dir = 0; dir < nfaces; ++dir) {
-        if (!has_neighbour(g, ijk, dir)) continue;
-        int nb = i + ((dir & 1) ? 1 : -1) * stride_of(g, dir >> 1);
-        const IQ<double> ex = iq_values(a, nb);
-        Ev F[2];
-        interior_flux<DIM, S>(a, in, ex, Ki, a.K[nb], ijk, dir, F);
-        double alpha = 1.0;
-        alpha *= g.area[dir >> 1];
-        Ev rj[2];
-        for (int k = 0; k < 2; ++k) {
-            F[k] = F[k] * alpha;
-            res0[k] = res0[k] + F[k];
-            rj[k] = (ev(0.0) - F[k]) / g.V;
-        }
-        store_block(a.J, slot_of_dir(g, dir ^ 1), g.N, nb, rj);
-    }
-    // storage (fvbaselocalresidual.hh:494-629)
-    {
-        const double phi = a.phi[i];
-        Ev M[2];
-        storage(a, phi, in.Sw, M);
-        double Mold[2];
-        double2 *sto = reinterpret_cast<double2 *>(a.sto);
-        if (a.enable_cache) {
-            if (a.newton_iter == 0) {
-                if (a.recycle) {
-                    Mold[0] = M[0].v;
-                    Mold[1] = M[1].v;
-                } else {
-                    const double2 xo = reinterpret_cast<const double2 *>(a.x1)[i];
-                    double Mo[2];
-                    storage<double>(a, phi, xo.y, Mo);
-                    Mold[0] = Mo[0];
-                    Mold[1] = Mo[1];
-                }
-                sto[i] = make_double2(Mold[0], Mold[1]);
-            } else {
-                double2 s = sto[i];
-                Mold[0] = s.x;
-                Mold[1] = s.y;
-            }
-        } else {
-            const double2 xo = reinterpret_cast<const double2 *>(a.x1)[i];
-            double Mo[2];
-            storage<double>(a, phi, xo.y, Mo);
-            Mold[0] = Mo[0];
-            Mold[1] = Mo[1];
-        }
-        double scv = g.V * 1.0;
-        for (int k = 0; k < 2; ++k) {
-            Ev tmp = M[k] - Mold[k];
-            tmp = tmp * (scv / a.dt);
-            res0[k] = res0[k] + tmp;
-        }
-    }
-    // boundary segments (fvbaselocalresidual.hh:425-487)
-#pragma unroll
-    for (int dir = 0; dir < nfaces; ++dir) {
-        if (has_neighbour(g, ijk, dir)) continue;
-        long long bf = boundary_face(a, ijk, dir);
-        int t = a.bc_type[bf];
-        Ev Bv[2] = {ev(0.0), ev(0.0)};
-        if (t == 1) {
-            freeflow_flux<DIM, S>(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf], a.bc_val[2 * bf + 1], Bv);
-        } else if (t == 2) {
-            Bv[0] = ev(a.bc_val[2 * bf]);
-            Bv[1] = ev(a.bc_val[2 * bf + 1]);
-        }
-        double mult = g.area[dir >> 1] * 1.0;
-        for (int k = 0; k < 2; ++k) res0[k] = res0[k] + Bv[k] * mult;
-    }
-    // volumetric residual (fvbaselocalresidual.hh:170-186)
-    for (int k = 0; k < 2; ++k) res0[k] = res0[k] / g.V;
-    reinterpret_cast<double2 *>(a.R)[i] = make_double2(res0[0].v, res0[1].v);
-    store_block(a.J, g.diag, g.N, i, res0);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.g.N) return;
+    linearize_cell<DIM, S>(a, i);
 }
 
 __global__ void __launch_bounds__(256) k_storage(LinArgs a, const double *x, double *M) {

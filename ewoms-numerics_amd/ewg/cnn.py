@@ -271,18 +271,6 @@ def warm_stats(ctx):
 
 
 # ---------------------------------------------------------------------- training data
-def ensemble_member(k, seed):
-    """Realization k of the ensemble family: size from {32^3, 48^3, 64x64x32}, sigma_lnK
-    ~ U[0.5, 2], correlation length ~ U[4, 12] cells, injection ~ U[5e-4, 2e-3]
-    kg/(m^2 s) over a top centre patch of nx/8 x nx/8 cells."""
-    rng = np.random.default_rng([seed, k])
-    sizes = [(32, 32, 32), (48, 48, 48), (64, 64, 32)]
-    nx, ny, nz = sizes[int(rng.integers(len(sizes)))]
-    return dict(nx=nx, ny=ny, nz=nz, seed=int(rng.integers(1, 2**31 - 1)),
-                sigma_lnK=float(rng.uniform(0.5, 2.0)), corr_cells=float(rng.uniform(4.0, 12.0)),
-                rate=float(rng.uniform(5e-4, 2e-3)), patch=max(2, nx // 8))
-
-
 def harvest(n_pairs, steps_per_member=20, seed=1, crop=(32, 32, 32), skip_first=True,
             top_fraction=0.75, log=None):
     """(state, converged update) pairs from plain FP64 runs of ensemble members, each cut
@@ -297,11 +285,9 @@ def harvest(n_pairs, steps_per_member=20, seed=1, crop=(32, 32, 32), skip_first=
     X, Kc, Pc, DT, DX = [], [], [], [], []
     k = 0
     while len(X) < n_pairs:
-        m = ensemble_member(k, seed)
+        P = problems.ensemble_member(k, seed, sizes=problems.ENSEMBLE_SIZES[:3],
+                                     steps=steps_per_member)
         k += 1
-        P = problems.heterogeneous_3d(m["nx"], m["ny"], m["nz"], seed=m["seed"],
-                                      sigma_lnK=m["sigma_lnK"], corr_cells=m["corr_cells"],
-                                      rate=m["rate"], patch=m["patch"], steps=steps_per_member)
         ctx = GpuContext(P, default_params(newton_tolerance=1e-6, max_time_step_size=P.dt))
         K3 = P.K.reshape(P.nz, P.ny, P.nx)
         F3 = P.phi.reshape(P.nz, P.ny, P.nx)
@@ -332,7 +318,7 @@ def harvest(n_pairs, steps_per_member=20, seed=1, crop=(32, 32, 32), skip_first=
                 break
         ctx.close()
         if log:
-            log(f"harvest: member {k - 1} {m['nx']}x{m['ny']}x{m['nz']} -> {len(X)} pairs")
+            log(f"harvest: member {k - 1} {P.nx}x{P.ny}x{P.nz} -> {len(X)} pairs")
     return {"x": np.stack(X), "K": np.stack(Kc), "phi": np.stack(Pc), "dt": np.array(DT),
             "dx": np.stack(DX)}
 

@@ -225,6 +225,23 @@ def heterogeneous_3d(nx=128, ny=128, nz=64, seed=20261203, sigma_lnK=1.0, corr_c
                          "h": h, "rate": rate, "patch": patch, "steps": steps})
 
 
+ENSEMBLE_SIZES = ((32, 32, 32), (48, 48, 48), (64, 64, 32), (96, 96, 48))
+
+
+def ensemble_member(k, seed, sizes=ENSEMBLE_SIZES, steps=40) -> Problem:
+    """Realization k of the ensemble family of config 3 (the config-1 physics): grid size
+    drawn uniformly from `sizes`, sigma_lnK ~ U[0.5, 2], correlation length ~ U[4, 12]
+    cells, injection rate ~ U[5e-4, 2e-3] kg/(m^2 s) over a top centre patch of nx/8 x
+    nx/8 cells, its own seed for the permeability and porosity fields."""
+    rng = np.random.default_rng([seed, k])
+    nx, ny, nz = sizes[int(rng.integers(len(sizes)))]
+    return heterogeneous_3d(nx, ny, nz, seed=int(rng.integers(1, 2**31 - 1)),
+                            sigma_lnK=float(rng.uniform(0.5, 2.0)),
+                            corr_cells=float(rng.uniform(4.0, 12.0)),
+                            rate=float(rng.uniform(5e-4, 2e-3)), patch=max(2, nx // 8),
+                            steps=steps)
+
+
 def perturbed_state(problem: Problem, seed=7, dS=0.3, dp=500.0):
     """A non-trivial state for linearization parity: saturation and pressure perturbed
     away from the initial hydrostatic state (keeps S_w in (0.02, 1.05))."""

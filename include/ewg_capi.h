@@ -103,7 +103,8 @@ typedef struct ewg_params {
                                           wavefront each (default); 1 one launch per
                                           hyperplane x+y+z = const; 2 tile wavefront, generic
                                           sweep kernel; 3 tile wavefront, one wavefront per
-                                          tile */
+                                          tile. The tile schedules need their buffers, which
+                                          ewg_create allocates unless it is given 1 */
     int32_t kernel_events;             /* 1: bracket sampled launches with device events (the
                                           first preconditioner apply and SpMV of each batch
                                           of BiCGStab iterations, every factorization and

@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     path = ewg.library_path()
     assert os.path.exists(path), "build libewg.so first (__graft_entry__.build())"
     lib = C.CDLL(path)
-    for header in ("ewg_capi.h", "ewg_cnn.h"):
+    for header in ("ewg_capi.h", "ewg_cnn.h", "ewg_ensemble.h"):
         names = _declared(header)
         assert names
         missing = [n for n in names if not hasattr(lib, n)]

@@ -1,0 +1,137 @@
+"""GPU: the ensemble solver (include/ewg_ensemble.h) against single contexts and the oracle.
+
+Each member of an ensemble must reproduce what its own context does with the same
+parameters: the same Newton iterations and the same states up to the summation order of
+the BiCGStab dot products (saturations within 1e-7: BiCGStab iterates that differ in the
+last bits can stop one iteration apart, within the Newton tolerance); with the FP64
+preconditioner and Newton tolerance 1e-8 the members match the CPU oracle like the single
+context does (identical Newton iterations, saturations within 1e-8).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 12, 8, 31), (24, 16, 8, 32), (32, 8, 6, 33), (16, 16, 16, 34)]
+
+
+def _problems(steps=3):
+    from ewg import problems
+    return [problems.heterogeneous_3d(nx, ny, nz, seed=s, patch=4, steps=steps)
+            for nx, ny, nz, s in SHAPES]
+
+
+def _single_runs(probs, params, steps):
+    import ewg
+    out = []
+    for P in probs:
+        c = ewg.GpuContext(P, params)
+        its = 0
+        for _ in range(steps):
+            rc, it, _ = c.newton(P.dt)
+            assert rc == 0
+            its += it
+            c.advance_time_level()
+        out.append((its, c.stats()["linear_iterations"], c.solution(0)))
+        c.close()
+    return out
+
+
+def test_members_match_single_contexts():
+    import ewg
+    from ewg import ensemble
+    probs = _problems()
+    prm = ensemble.member_params()
+    ctxs = [ewg.GpuContext(P, prm) for P in probs]
+    ens = ensemble.Ensemble(ctxs)
+    print("ensemble of", [(P.nx, P.ny, P.nz) for P in probs], flush=True)
+    sec = ens.run(probs[0].dt, 3)
+    ref = _single_runs(probs, prm, 3)
+    for i, P in enumerate(probs):
+        st = ens.stats(i)
+        its, lits, x = ref[i]
+        xe = ctxs[i].solution(0)
+        dS = np.abs(xe[1::2] - x[1::2]).max()
+        print(f"member {i}: {st}; single: newton {its} bicgstab {lits}; max |dS| {dS:.2e}",
+              flush=True)
+        assert st["status"] == 0 and st["steps"] == 3
+        assert st["newton_iterations"] == its
+        assert abs(st["linear_iterations"] - lits) <= 2
+        assert dS < 1e-7
+    print(f"3 steps in {sec:.3f} s", flush=True)
+
+
+def test_fp64_members_match_the_oracle():
+    import ewg
+    from ewg import ensemble
+    from oracle.pyoracle import OracleContext
+    probs = _problems()[:2]
+    prm = ensemble.member_params(precond_fp32=0, newton_tolerance=1e-8)
+    ctxs = [ewg.GpuContext(P, prm) for P in probs]
+    ens = ensemble.Ensemble(ctxs)
+    ens.run(probs[0].dt, 3)
+    for i, P in enumerate(probs):
+        o = OracleContext(P, ewg.default_params(newton_tolerance=1e-8))
+        its = 0
+        for _ in range(3):
+            rc, it, _ = o.newton(P.dt)
+            assert rc == 0
+            its += it
+            o.advance_time_level()
+        st = ens.stats(i)
+        dS = np.abs(ctxs[i].solution(0)[1::2] - o.solution(0)[1::2]).max()
+        print(f"member {i}: ensemble newton {st['newton_iterations']}, oracle {its}; "
+              f"max |dS| {dS:.2e}", flush=True)
+        assert st["newton_iterations"] == its
+        assert dS < 1e-8
+
+
+def test_failing_member_is_isolated():
+    """A member whose Newton solve fails (here: NewtonMaxError below its first error)
+    keeps its state and stops; the others are unaffected."""
+    import ewg
+    from ewg import ensemble
+    probs = _problems()
+    prm = ensemble.member_params()
+    bad = ensemble.member_params(newton_max_error=1e-9)
+    ctxs = [ewg.GpuContext(P, bad if i == 1 else prm) for i, P in enumerate(probs)]
+    x_before = ctxs[1].solution(0)
+    ens = ensemble.Ensemble(ctxs)
+    ens.run(probs[0].dt, 2)
+    ref = _single_runs([probs[0], probs[2]], prm, 2)
+    st = [ens.stats(i) for i in range(4)]
+    print(st, flush=True)
+    assert st[1]["status"] == ewg.capi.EWG_NUMERICAL_ISSUE and st[1]["steps"] == 0
+    assert np.array_equal(ctxs[1].solution(0), x_before)
+    for i, (its, _, x) in zip((0, 2), ref):
+        assert st[i]["status"] == 0 and st[i]["steps"] == 2
+        assert st[i]["newton_iterations"] == its
+        assert np.abs(ctxs[i].solution(0)[1::2] - x[1::2]).max() < 1e-7
+
+
+def test_tuner_and_graph_replay():
+    """The tuner's trial steps leave the state as it was; one run of three steps and
+    three runs of one step (replays of different graph captures, the same launch shape)
+    give identical states."""
+    import ewg
+    from ewg import ensemble
+    probs = _problems()
+    prm = ensemble.member_params()
+    t_ctx = [ewg.GpuContext(P, prm) for P in probs]
+    et = ensemble.Ensemble(t_ctx)
+    x0 = [c.solution(0) for c in t_ctx]
+    t = et.tune(probs[0].dt)
+    print("tuning", t, flush=True)
+    assert t["threads"] in (256, 512, 1024)
+    assert all(c["ms"] > 0 for c in t["candidates"])
+    assert all(np.array_equal(c.solution(0), x) for c, x in zip(t_ctx, x0))
+    assert all(et.stats(i)["steps"] == 0 for i in range(4))
+    a = [ewg.GpuContext(P, prm) for P in probs]
+    b = [ewg.GpuContext(P, prm) for P in probs]
+    ea, eb = ensemble.Ensemble(a), ensemble.Ensemble(b)
+    ea.run(probs[0].dt, 3)
+    for _ in range(3):
+        eb.run(probs[0].dt, 1)
+    for i in range(4):
+        assert ea.stats(i)["steps"] == eb.stats(i)["steps"] == 3
+        assert np.array_equal(a[i].solution(0), b[i].solution(0))
