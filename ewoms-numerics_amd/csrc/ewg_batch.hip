@@ -32,13 +32,14 @@ struct Member {
     double *L, *D;                     // FP64 factors
     float *L32, *D32, *U32, *w32;      // FP32 preconditioner (fp32 = 1)
     double *r, *p, *v, *y, *z, *dx;    // BiCGStab vectors
+    double *xs;                        // the time step's start state (retries)
     double newton_tol, newton_max_error, lin_rel_tol, lin_abs_tol, lin_max_error;
-    int newton_max_it, lin_max_it, fp32, pad_;
+    int newton_max_it, lin_max_it, fp32, max_div;
 };
 
 struct Outcome {
     long long steps, newton, linear, solves;
-    int status, pad_;
+    int status, divisions;
     double last_error;
 };
 
@@ -276,15 +277,14 @@ k_ensemble(const Member *members, const int *order, int nsteps, double dt, int c
     const Member &m = members[id];
     Outcome o = out[id];  // every thread holds the same copy; thread 0 writes it back
     if (o.status != 0) return;
-    if (threadIdx.x == 0) {
-        la = m.lin;
-        la.dt = dt;
-    }
+    if (threadIdx.x == 0) la = m.lin;
     __syncthreads();
     const Solver<NT> S{m, la, red};
     const Grid &g = la.g;
     const int n = 2 * g.N;
-    for (int step = 0; step < nsteps && o.status == 0; ++step) {
+    // one Newton solve of dts from x0 = x1 (newtonmethod.hh:308-517)
+    auto solve = [&](double dts) -> int {
+        if (threadIdx.x == 0) la.dt = dts;
         int it = 0, rc = 0;
         double err = 1e100, lerr = 1e100;
         while (proceed(it, err, lerr, m)) {
@@ -336,17 +336,492 @@ k_ensemble(const Member *members, const int *order, int nsteps, double dt, int c
         }
         if (rc == 0 && !(err <= m.newton_tol)) rc = EWG_NOT_CONVERGED;
         o.last_error = err;
+        return rc;
+    };
+    for (int step = 0; step < nsteps && o.status == 0; ++step) {
+        // a failed solve is retried from the step's start with 2, 4, ... substeps
+        // (FvBaseProblem::timeIntegration halves the step, fvbaseproblem.hh:507-553)
+        for (int i = threadIdx.x; i < n; i += NT) m.xs[i] = m.x1[i];
+        __syncthreads();
+        int rc = 0;
+        for (int level = 0; level <= m.max_div; ++level) {
+            const int nsub = 1 << level;
+            rc = 0;
+            for (int j = 0; j < nsub && rc == 0; ++j) {
+                rc = solve(dt / nsub);
+                if (rc == 0)
+                    for (int i = threadIdx.x; i < n; i += NT) m.x1[i] = m.x0[i];
+                __syncthreads();
+            }
+            if (rc == 0) break;
+            for (int i = threadIdx.x; i < n; i += NT) m.x0[i] = m.x1[i] = m.xs[i];
+            __syncthreads();
+            if (level < m.max_div) o.divisions++;
+        }
         if (rc != 0 || !commit) {
-            // keep the last converged state
-            for (int i = threadIdx.x; i < n; i += NT) m.x0[i] = m.x1[i];
+            // the state the step started from
+            for (int i = threadIdx.x; i < n; i += NT) m.x0[i] = m.x1[i] = m.xs[i];
             if (rc != 0) o.status = rc;
         } else {
-            for (int i = threadIdx.x; i < n; i += NT) m.x1[i] = m.x0[i];
             o.steps++;
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) out[id] = o;
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Batched mode: every launch covers the cells of all members (block -> member maps), so
+// the whole GPU works on each phase; reductions are per member (block partials summed in
+// a fixed order by one block per member); the ILU(0) sweeps run hyperplane by
+// hyperplane for all members at once. Kernels return at once for members that do not
+// take part (act[m] = 0, or a Krylov status other than running).
+// ---------------------------------------------------------------------------------------
+constexpr int BT = 256;
+constexpr int KS_IDLE = 4, KS_MAXIT = 5;
+
+struct Maps {
+    const int *cmem, *coff;  // cell launches: block -> member, member -> first block (n+1)
+    const int *lmem, *loff;  // hyperplane launches
+};
+
+enum VecId { V_R = 0, V_P, V_V, V_Y, V_Z, V_DX, V_X0, V_X1, V_XCUR, V_RES, V_XS };
+__device__ __forceinline__ double *vec(const Member &m, int k) {
+    switch (k) {
+        case V_R: return m.r;
+        case V_P: return m.p;
+        case V_V: return m.v;
+        case V_Y: return m.y;
+        case V_Z: return m.z;
+        case V_DX: return m.dx;
+        case V_X0: return m.x0;
+        case V_X1: return m.x1;
+        case V_XCUR: return m.xcur;
+        case V_XS: return m.xs;
+        default: return m.lin.R;
+    }
+}
+
+__device__ __forceinline__ double bt_sum(double v) {
+    __shared__ double r[BT / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < BT / 64; ++k) t += r[k];
+    return t;
+}
+__device__ __forceinline__ double bt_max(double v) {
+    __shared__ double r[BT / 64];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < BT / 64; ++k) t = fmax(t, r[k]);
+    return t;
+}
+// one block per member: reductions of the member's block partials
+__device__ __forceinline__ double range_sum(const double *part, int b0, int b1) {
+    double s = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += BT) s += part[b];
+    return bt_sum(s);
+}
+__device__ __forceinline__ double range_max(const double *part, int b0, int b1) {
+    double s = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += BT) s = fmax(s, part[b]);
+    return bt_max(s);
+}
+__device__ __forceinline__ int range_any(const int *part, int b0, int b1) {
+    int v = 0;
+    for (int b = b0 + threadIdx.x; b < b1; b += BT) v |= part[b];
+    return __syncthreads_or(v);
+}
+
+__global__ void __launch_bounds__(BT) k_b_copy(const Member *mem, Maps mp, const int *act, int dst, int src) {
+    const int id = mp.cmem[blockIdx.x];
+    if (!act[id]) return;
+    const Member &m = mem[id];
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    if (i >= m.lin.g.N) return;
+    const double *a = vec(m, src);
+    double *b = vec(m, dst);
+    b[2 * i] = a[2 * i];
+    b[2 * i + 1] = a[2 * i + 1];
+}
+
+// the two passes of the linearization (ewg_lin_device.h) with the member's arguments
+template <int PASS>
+__global__ void __launch_bounds__(BT) k_b_lin(const Member *mem, Maps mp, const int *act, const int *niter,
+                                              const double *dts) {
+    __shared__ LinArgs la;
+    const int id = mp.cmem[blockIdx.x];
+    if (!act[id]) return;
+    const Member &m = mem[id];
+    if (threadIdx.x == 0) {
+        la = m.lin;
+        la.dt = dts[id];
+        la.newton_iter = niter[id];
+    }
+    __syncthreads();
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    if (i >= la.g.N) return;
+    if (PASS == 0) lin::iq_cell(la, i);
+    else if (la.g.dim == 3) lin::linearize_cell<3, double>(la, i);
+    else lin::linearize_cell<2, double>(la, i);
+}
+
+__global__ void __launch_bounds__(BT) k_b_maxabs(const Member *mem, Maps mp, const int *act, int v, double *part) {
+    const int id = mp.cmem[blockIdx.x];
+    if (!act[id]) return;
+    const Member &m = mem[id];
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    double e = 0.0;
+    if (i < m.lin.g.N) {
+        const double *x = vec(m, v);
+        e = fmax(fabs(x[2 * i]), fabs(x[2 * i + 1]));
+    }
+    e = bt_max(e);
+    if (threadIdx.x == 0) part[blockIdx.x] = e;
+}
+
+__global__ void __launch_bounds__(BT) k_b_reduce(Maps mp, const int *act, const double *part, int op, double *out) {
+    const int id = blockIdx.x;
+    if (!act[id]) return;
+    const double r = op == 0 ? range_sum(part, mp.coff[id], mp.coff[id + 1])
+                             : range_max(part, mp.coff[id], mp.coff[id + 1]);
+    if (threadIdx.x == 0) out[id] = r;
+}
+
+__global__ void __launch_bounds__(BT) k_b_factor(const Member *mem, Maps mp, const int *act, int level, int *err) {
+    const int id = mp.lmem[blockIdx.x];
+    if (!act[id]) return;
+    const Member &m = mem[id];
+    const Grid &g = m.lin.g;
+    if (level >= g.nlevels) return;
+    int ijk[3];
+    const int i = cell::level_cell(g, level, (blockIdx.x - mp.loff[id]) * BT + threadIdx.x, ijk);
+    if (i < 0) return;
+    if (!cell::ilu_factor_cell<double>(g, i, ijk, m.lin.J, m.L, m.D, static_cast<double *>(nullptr)))
+        atomicExch(&err[id], 1);
+}
+
+__global__ void __launch_bounds__(BT) k_b_round(const Member *mem, Maps mp, const int *act) {
+    const int id = mp.cmem[blockIdx.x];
+    if (!act[id]) return;
+    const Member &m = mem[id];
+    if (!m.fp32) return;
+    const Grid &g = m.lin.g;
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    if (i >= g.N) return;
+    const double *Aup = m.lin.J + (size_t)(g.diag + 1) * g.N * 4;
+    for (int s = 0; s < g.dim; ++s)
+        for (int e = 0; e < 4; ++e) {
+            const size_t k = ((size_t)s * g.N + i) * 4 + e;
+            m.L32[k] = static_cast<float>(m.L[k]);
+            m.U32[k] = static_cast<float>(Aup[k]);
+        }
+    for (int e = 0; e < 4; ++e) m.D32[(size_t)i * 4 + e] = static_cast<float>(m.D[(size_t)i * 4 + e]);
+}
+
+__global__ void __launch_bounds__(BT) k_b_fwd(const Member *mem, Maps mp, const KrylovState *ks, int level, int src, int dst) {
+    const int id = mp.lmem[blockIdx.x];
+    if (ks[id].status != KS_RUN) return;
+    const Member &m = mem[id];
+    const Grid &g = m.lin.g;
+    if (level >= g.nlevels) return;
+    int ijk[3];
+    const int i = cell::level_cell(g, level, (blockIdx.x - mp.loff[id]) * BT + threadIdx.x, ijk);
+    if (i < 0) return;
+    if (m.fp32) cell::ilu_fwd_cell<float>(g, i, ijk, m.L32, vec(m, src), m.w32);
+    else cell::ilu_fwd_cell<double>(g, i, ijk, m.L, vec(m, src), vec(m, dst));
+}
+
+__global__ void __launch_bounds__(BT) k_b_bwd(const Member *mem, Maps mp, const KrylovState *ks, int level, int dst) {
+    const int id = mp.lmem[blockIdx.x];
+    if (ks[id].status != KS_RUN) return;
+    const Member &m = mem[id];
+    const Grid &g = m.lin.g;
+    if (level >= g.nlevels) return;
+    int ijk[3];
+    const int i = cell::level_cell(g, level, (blockIdx.x - mp.loff[id]) * BT + threadIdx.x, ijk);
+    if (i < 0) return;
+    double *out = vec(m, dst);
+    if (m.fp32) cell::ilu_bwd_cell<float, float>(g, i, ijk, m.U32, 0, m.D32, m.w32, out);
+    else cell::ilu_bwd_cell<double, double>(g, i, ijk, m.lin.J, g.diag + 1, m.D, out, out);
+}
+
+// y = A x with sum_i a_i (A x)_i and sum_i b_i (A x)_i per block (b < 0: none)
+__global__ void __launch_bounds__(BT) k_b_spmv(const Member *mem, Maps mp, const KrylovState *ks, int x, int y, int a, int b,
+                                               double *partA, double *partB) {
+    const int id = mp.cmem[blockIdx.x];
+    if (ks[id].status != KS_RUN) return;
+    const Member &m = mem[id];
+    const Grid &g = m.lin.g;
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    double sa = 0.0, sb = 0.0;
+    if (i < g.N) {
+        const double2 yv = cell::spmv_cell(g, i, m.lin.J, vec(m, x));
+        reinterpret_cast<double2 *>(vec(m, y))[i] = yv;
+        const double2 av = reinterpret_cast<const double2 *>(vec(m, a))[i];
+        sa += av.x * yv.x + av.y * yv.y;
+        if (b >= 0) {
+            const double2 bv = reinterpret_cast<const double2 *>(vec(m, b))[i];
+            sb += bv.x * yv.x + bv.y * yv.y;
+        }
+    }
+    sa = bt_sum(sa);
+    if (b >= 0) sb = bt_sum(sb);
+    if (threadIdx.x == 0) {
+        partA[blockIdx.x] = sa;
+        if (b >= 0) partB[blockIdx.x] = sb;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_init(const Member *mem, Maps mp, const int *act, double *partMax, double *partDot) {
+    const int id = mp.cmem[blockIdx.x];
+    if (!act[id]) return;
+    const Member &m = mem[id];
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    double mx = 0.0, d = 0.0;
+    if (i < m.lin.g.N) {
+        for (int c = 2 * i; c < 2 * i + 2; ++c) {
+            const double bi = m.lin.R[c];
+            m.r[c] = bi;
+            m.dx[c] = 0.0;
+            m.p[c] = 0.0;
+            m.v[c] = 0.0;
+            mx = fmax(mx, fabs(bi));
+            d += bi * bi;
+        }
+    }
+    mx = bt_max(mx);
+    d = bt_sum(d);
+    if (threadIdx.x == 0) {
+        partMax[blockIdx.x] = mx;
+        partDot[blockIdx.x] = d;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_ks_init(const Member *mem, Maps mp, const int *act, const int *err, KrylovState *ks,
+                                                  const double *partMax, const double *partDot) {
+    const int id = blockIdx.x;
+    if (!act[id]) {
+        if (threadIdx.x == 0) ks[id].status = KS_IDLE;
+        return;
+    }
+    const Member &m = mem[id];
+    const double mx = range_max(partMax, mp.coff[id], mp.coff[id + 1]);
+    const double d = range_sum(partDot, mp.coff[id], mp.coff[id + 1]);
+    if (threadIdx.x == 0) {
+        KrylovState &k = ks[id];
+        k.rel_tol = m.lin_rel_tol;
+        k.abs_tol = m.lin_abs_tol;
+        k.max_resid = m.lin_max_error;
+        const double e = fmax(mx, kBreakdownEps);
+        k.resid_err = e;
+        k.init_err = e;
+        k.last_err = e;
+        k.stagnates = 0;
+        k.rho = 1.0;
+        k.alpha = 1.0;
+        k.omega = 1.0;
+        k.rho_i = d;
+        k.iterations = 0;
+        const double red = e / fmax(1e-20, e);
+        k.status = err[id] ? KS_BREAK : ((red <= m.lin_rel_tol || e <= m.lin_abs_tol) ? KS_CONV : KS_RUN);
+    }
+}
+
+__device__ __forceinline__ void b_criterion(KrylovState &k, double err, int stagnates) {
+    k.last_err = k.resid_err;
+    k.resid_err = err;
+    k.stagnates = stagnates;
+    const double red = err / fmax(1e-20, k.init_err);
+    const bool conv = red <= k.rel_tol || err <= k.abs_tol;
+    const bool failed = !conv && (stagnates || err > k.max_resid);
+    if (conv) k.status = KS_CONV;
+    else if (failed) k.status = KS_FAIL;
+}
+
+__global__ void __launch_bounds__(BT) k_b_ks_beta(const Member *mem, Maps mp, KrylovState *ks, const double *partRho) {
+    const int id = blockIdx.x;
+    KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const double rho_i = k.iterations == 0 ? k.rho_i : range_sum(partRho, mp.coff[id], mp.coff[id + 1]);
+    if (threadIdx.x == 0) {
+        if (k.iterations >= mem[id].lin_max_it) {
+            k.status = KS_MAXIT;
+            return;
+        }
+        if (fabs(k.rho) <= kBreakdownEps || fabs(k.omega) <= kBreakdownEps) {
+            k.status = KS_BREAK;
+            return;
+        }
+        k.beta = (rho_i / k.rho) * (k.alpha / k.omega);
+        k.rho = rho_i;
+        k.rho_i = rho_i;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_p_update(const Member *mem, Maps mp, const KrylovState *ks) {
+    const int id = mp.cmem[blockIdx.x];
+    const KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const Member &m = mem[id];
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    if (i >= m.lin.g.N) return;
+    const double omega = k.omega, beta = k.beta;
+    for (int c = 2 * i; c < 2 * i + 2; ++c) {
+        double tmp = m.v[c];
+        tmp *= omega;
+        tmp -= m.p[c];
+        tmp *= -beta;
+        double pi = m.r[c];
+        pi += tmp;
+        m.p[c] = pi;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_ks_alpha(Maps mp, KrylovState *ks, const double *partDen) {
+    const int id = blockIdx.x;
+    KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const double den = range_sum(partDen, mp.coff[id], mp.coff[id + 1]);
+    if (threadIdx.x == 0) {
+        k.denom = den;
+        if (fabs(den) <= kBreakdownEps) {
+            k.status = KS_BREAK;
+            return;
+        }
+        const double alpha = k.rho / den;
+        k.alpha = alpha;
+        if (fabs(alpha) <= kBreakdownEps) k.status = KS_BREAK;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_hs_update(const Member *mem, Maps mp, const KrylovState *ks, double *partMax, int *partAny) {
+    const int id = mp.cmem[blockIdx.x];
+    const KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const Member &m = mem[id];
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    const double alpha = k.alpha;
+    double mx = 0.0;
+    bool nz = false;
+    if (i < m.lin.g.N) {
+        for (int c = 2 * i; c < 2 * i + 2; ++c) {
+            const double yi = m.y[c];
+            double tmp = yi;
+            tmp *= alpha;
+            tmp += m.dx[c];
+            m.dx[c] = tmp;
+            tmp = m.v[c];
+            tmp *= alpha;
+            const double ri = m.r[c] - tmp;
+            m.r[c] = ri;
+            mx = fmax(mx, fabs(ri));
+            nz |= (yi != 0.0);
+        }
+    }
+    mx = bt_max(mx);
+    const int any = __syncthreads_or(nz);
+    if (threadIdx.x == 0) {
+        partMax[blockIdx.x] = mx;
+        partAny[blockIdx.x] = any;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_ks_crit1(Maps mp, KrylovState *ks, const double *partMax, const int *partAny) {
+    const int id = blockIdx.x;
+    KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const double mx = range_max(partMax, mp.coff[id], mp.coff[id + 1]);
+    const int any = range_any(partAny, mp.coff[id], mp.coff[id + 1]);
+    if (threadIdx.x == 0) b_criterion(k, mx, any == 0);
+}
+
+__global__ void __launch_bounds__(BT) k_b_ks_omega(Maps mp, KrylovState *ks, const double *partTT, const double *partTS) {
+    const int id = blockIdx.x;
+    KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const double tt = range_sum(partTT, mp.coff[id], mp.coff[id + 1]);
+    const double ts = range_sum(partTS, mp.coff[id], mp.coff[id + 1]);
+    if (threadIdx.x == 0) {
+        if (fabs(tt) <= kBreakdownEps) {
+            k.status = KS_BREAK;
+            return;
+        }
+        const double omega = ts / tt;
+        k.omega = omega;
+        if (fabs(omega) <= kBreakdownEps) k.status = KS_BREAK;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_xr_update(const Member *mem, Maps mp, const KrylovState *ks, double *partRho, int *partAny) {
+    const int id = mp.cmem[blockIdx.x];
+    const KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const Member &m = mem[id];
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    const double omega = k.omega;
+    double d = 0.0;
+    bool nz = false;
+    if (i < m.lin.g.N) {
+        for (int c = 2 * i; c < 2 * i + 2; ++c) {
+            const double zi = m.z[c];
+            m.dx[c] += omega * zi;
+            const double ri = m.r[c] + -omega * m.y[c];
+            m.r[c] = ri;
+            d += m.lin.R[c] * ri;
+            nz |= (zi != 0.0);
+        }
+    }
+    d = bt_sum(d);
+    const int any = __syncthreads_or(nz);
+    if (threadIdx.x == 0) {
+        partRho[blockIdx.x] = d;
+        partAny[blockIdx.x] = any;
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_b_ks_crit2(Maps mp, KrylovState *ks, const int *partAny) {
+    const int id = blockIdx.x;
+    KrylovState &k = ks[id];
+    if (k.status != KS_RUN) return;
+    const int any = range_any(partAny, mp.coff[id], mp.coff[id + 1]);
+    if (threadIdx.x == 0) {
+        // the residual checked here is still s (the object r is updated afterwards)
+        b_criterion(k, k.resid_err, any == 0);
+        if (k.status == KS_RUN) k.iterations += 1;
+    }
+}
+
+// x0 = xcur - dx, with sum |dx| per block (finiteness of the update)
+__global__ void __launch_bounds__(BT) k_b_update(const Member *mem, Maps mp, const int *act, double *part) {
+    const int id = mp.cmem[blockIdx.x];
+    if (!act[id]) return;
+    const Member &m = mem[id];
+    const int i = (blockIdx.x - mp.coff[id]) * BT + threadIdx.x;
+    double s = 0.0;
+    if (i < m.lin.g.N) {
+        for (int c = 2 * i; c < 2 * i + 2; ++c) {
+            const double d = m.dx[c];
+            double nv = m.xcur[c];
+            nv -= d;
+            m.x0[c] = nv;
+            s += fabs(d);
+        }
+    }
+    s = bt_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(BT) k_b_zero(int n, int *v) {
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i < n) v[i] = 0;
 }
 
 }  // namespace batch
@@ -357,17 +832,30 @@ using namespace ewg::batch;
 
 struct ewg_ensemble {
     std::vector<ewg_ctx *> members;
+    std::vector<Member> host;
     std::vector<Outcome> outcome;
     Member *dmem = nullptr;
     int *dorder = nullptr;
     Outcome *dout = nullptr;
     hipStream_t stream = nullptr;
+    int mode = 1;  // 0: persistent (one workgroup per member), 1: batched launches
     int threads = 512;
     bool largest_first = true;
     hipGraphExec_t graph = nullptr;
     int graph_steps = -1, graph_threads = 0;
     bool graph_largest = true;
     double graph_dt = 0.0;
+    // batched mode
+    int nblkC = 0, nblkL = 0, lmax = 0;
+    int *cmem = nullptr, *coff = nullptr, *lmem = nullptr, *loff = nullptr;
+    int *act = nullptr, *niter = nullptr, *ilu_err = nullptr;
+    double *dts = nullptr;
+    std::vector<double *> xs;  // per member: the time step's start state
+    KrylovState *ks = nullptr;
+    double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *pres = nullptr;
+    int *partAny = nullptr;
+    hipGraphExec_t iter_graph = nullptr;  // one BiCGStab iteration of every member
+    std::vector<KrylovState> ks_host;
     std::string err;
 };
 
@@ -423,6 +911,277 @@ double now_s() {
         .count();
 }
 
+// ------------------------------------------------------------------------ batched mode
+template <class T>
+hipError_t dal(T **p, size_t count) {
+    return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+int batched_setup(ewg_ensemble *e) {
+    const int n = static_cast<int>(e->members.size());
+    std::vector<int> cmem, coff(n + 1), lmem, loff(n + 1);
+    for (int m = 0; m < n; ++m) {
+        const Grid &g = e->host[m].lin.g;
+        coff[m] = static_cast<int>(cmem.size());
+        cmem.insert(cmem.end(), (g.N + BT - 1) / BT, m);
+        loff[m] = static_cast<int>(lmem.size());
+        const int lc = g.dim == 3 ? g.ny * g.nz : g.ny;
+        lmem.insert(lmem.end(), (lc + BT - 1) / BT, m);
+        e->lmax = std::max(e->lmax, g.nlevels);
+    }
+    coff[n] = static_cast<int>(cmem.size());
+    loff[n] = static_cast<int>(lmem.size());
+    e->nblkC = coff[n];
+    e->nblkL = loff[n];
+    ENS_TRY(e, dal(&e->cmem, cmem.size()));
+    ENS_TRY(e, dal(&e->coff, coff.size()));
+    ENS_TRY(e, dal(&e->lmem, lmem.size()));
+    ENS_TRY(e, dal(&e->loff, loff.size()));
+    ENS_TRY(e, dal(&e->act, n));
+    ENS_TRY(e, dal(&e->niter, n));
+    ENS_TRY(e, dal(&e->ilu_err, n));
+    ENS_TRY(e, dal(&e->ks, n));
+    ENS_TRY(e, dal(&e->part0, e->nblkC));
+    ENS_TRY(e, dal(&e->part1, e->nblkC));
+    ENS_TRY(e, dal(&e->part2, e->nblkC));
+    ENS_TRY(e, dal(&e->partAny, e->nblkC));
+    ENS_TRY(e, dal(&e->pres, n));
+    ENS_TRY(e, dal(&e->dts, n));
+    ENS_TRY(e, hipMemcpy(e->cmem, cmem.data(), cmem.size() * sizeof(int), hipMemcpyHostToDevice));
+    ENS_TRY(e, hipMemcpy(e->coff, coff.data(), coff.size() * sizeof(int), hipMemcpyHostToDevice));
+    ENS_TRY(e, hipMemcpy(e->lmem, lmem.data(), lmem.size() * sizeof(int), hipMemcpyHostToDevice));
+    ENS_TRY(e, hipMemcpy(e->loff, loff.data(), loff.size() * sizeof(int), hipMemcpyHostToDevice));
+    e->ks_host.resize(n);
+    return EWG_OK;
+}
+
+Maps maps(const ewg_ensemble *e) { return Maps{e->cmem, e->coff, e->lmem, e->loff}; }
+
+// out = (LU)^-1 in, hyperplane by hyperplane for all members whose Krylov status runs
+void launch_apply(ewg_ensemble *e, int src, int dst) {
+    const Maps mp = maps(e);
+    for (int l = 0; l < e->lmax; ++l)
+        hipLaunchKernelGGL(k_b_fwd, dim3(e->nblkL), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, l, src, dst);
+    for (int l = e->lmax - 1; l >= 0; --l)
+        hipLaunchKernelGGL(k_b_bwd, dim3(e->nblkL), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, l, dst);
+}
+
+// one BiCGStab iteration of every member (bicgstabsolver.hh:191-322), captured once
+int iteration_graph(ewg_ensemble *e) {
+    if (e->iter_graph) return EWG_OK;
+    const int n = static_cast<int>(e->members.size());
+    const Maps mp = maps(e);
+    hipGraph_t gr;
+    ENS_TRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_b_ks_beta, dim3(n), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2);
+    hipLaunchKernelGGL(k_b_p_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks);
+    launch_apply(e, V_P, V_Y);
+    hipLaunchKernelGGL(k_b_spmv, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks,
+                       (int)V_Y, (int)V_V, (int)V_RES, -1, e->part0, static_cast<double *>(nullptr));
+    hipLaunchKernelGGL(k_b_ks_alpha, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part0);
+    hipLaunchKernelGGL(k_b_hs_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part1, e->partAny);
+    hipLaunchKernelGGL(k_b_ks_crit1, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part1, e->partAny);
+    launch_apply(e, V_R, V_Z);
+    hipLaunchKernelGGL(k_b_spmv, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks,
+                       (int)V_Z, (int)V_Y, (int)V_Y, (int)V_R, e->part0, e->part1);
+    hipLaunchKernelGGL(k_b_ks_omega, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part0, e->part1);
+    hipLaunchKernelGGL(k_b_xr_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2, e->partAny);
+    hipLaunchKernelGGL(k_b_ks_crit2, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->partAny);
+    hipError_t h = hipStreamEndCapture(e->stream, &gr);
+    ENS_TRY(e, h);
+    h = hipGraphInstantiate(&e->iter_graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    ENS_TRY(e, h);
+    return EWG_OK;
+}
+
+bool proceed_h(int it, double err, double lerr, const Member &m) {
+    if (it < 1) return true;
+    if (err <= m.newton_tol) return false;
+    if (it >= m.newton_max_it) return err * 4.0 < lerr;
+    return true;
+}
+
+// one Newton solve (newtonmethod.hh:308-517) of dts[m] for every member with part[m],
+// from x0 = x1; rc[m] = 0 or the EWG_* code of the failure; err[m] = final error
+int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vector<double> &dts,
+                   std::vector<int> &rc, std::vector<double> &err) {
+    const int n = static_cast<int>(e->members.size());
+    const Maps mp = maps(e);
+    hipStream_t s = e->stream;
+    enum { RUN = 0, DONE = 1, FAILED = 2, OUT = 3 };
+    std::vector<int> state(n, OUT), it(n, 0), act(n, 0);
+    std::vector<double> lerr(n, 1e100), res(n, 0.0);
+    for (int m = 0; m < n; ++m) {
+        rc[m] = 0;
+        err[m] = 1e100;
+        if (part[m]) state[m] = RUN;
+    }
+    auto upload = [&](const std::vector<int> &v, int *d) -> int {
+        ENS_TRY(e, hipMemcpyAsync(d, v.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+        return EWG_OK;
+    };
+    auto fail = [&](int m, int code) {
+        state[m] = FAILED;
+        rc[m] = code;
+    };
+    if (int r = iteration_graph(e)) return r;
+    ENS_TRY(e, hipMemcpyAsync(e->dts, dts.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    while (true) {
+        bool any = false;
+        for (int m = 0; m < n; ++m) {
+            act[m] = 0;
+            if (state[m] != RUN) continue;
+            if (!proceed_h(it[m], err[m], lerr[m], e->host[m])) {
+                state[m] = DONE;
+                continue;
+            }
+            act[m] = 1;
+            any = true;
+            lerr[m] = err[m];  // beginIteration_
+        }
+        if (!any) break;
+        if (int r = upload(act, e->act)) return r;
+        if (int r = upload(it, e->niter)) return r;
+        hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, (int)V_XCUR, (int)V_X0);
+        hipLaunchKernelGGL(k_b_lin<0>, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->niter, e->dts);
+        hipLaunchKernelGGL(k_b_lin<1>, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->niter, e->dts);
+        hipLaunchKernelGGL(k_b_maxabs, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, (int)V_RES, e->part0);
+        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(BT), 0, s, mp, e->act, e->part0, 1, e->pres);
+        ENS_TRY(e, hipMemcpyAsync(res.data(), e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        ENS_TRY(e, hipStreamSynchronize(s));
+        bool solve = false;
+        for (int m = 0; m < n; ++m) {
+            if (!act[m]) continue;
+            e->outcome[m].newton++;
+            lerr[m] = err[m];  // preSolve_
+            err[m] = res[m];
+            act[m] = 0;
+            if (err[m] > e->host[m].newton_max_error) {
+                fail(m, EWG_NUMERICAL_ISSUE);
+            } else if (!proceed_h(it[m], err[m], lerr[m], e->host[m])) {
+                ++it[m];
+            } else {
+                act[m] = 1;
+                solve = true;
+            }
+        }
+        if (!solve) continue;
+        if (int r = upload(act, e->act)) return r;
+        hipLaunchKernelGGL(k_b_zero, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ilu_err);
+        for (int l = 0; l < e->lmax; ++l)
+            hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), dim3(BT), 0, s, e->dmem, mp, e->act, l, e->ilu_err);
+        hipLaunchKernelGGL(k_b_round, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act);
+        hipLaunchKernelGGL(k_b_init, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0, e->part1);
+        hipLaunchKernelGGL(k_b_ks_init, dim3(n), dim3(BT), 0, s, e->dmem, mp, e->act, e->ilu_err, e->ks, e->part0, e->part1);
+        ENS_TRY(e, hipGetLastError());
+        // iterations are replayed speculatively, the states read after 1, 2, 4, then every
+        // 8 of them (members that left the running state skip the rest)
+        int batch = 1;
+        while (true) {
+            ENS_TRY(e, hipMemcpyAsync(e->ks_host.data(), e->ks, n * sizeof(KrylovState), hipMemcpyDeviceToHost, s));
+            ENS_TRY(e, hipStreamSynchronize(s));
+            bool running = false;
+            for (int m = 0; m < n; ++m) running |= e->ks_host[m].status == KS_RUN;
+            if (!running) break;
+            for (int q = 0; q < batch; ++q) ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
+            batch = std::min(batch * 2, 8);
+        }
+        bool upd = false;
+        for (int m = 0; m < n; ++m) {
+            if (!act[m]) continue;
+            const KrylovState &k = e->ks_host[m];
+            e->outcome[m].linear += k.iterations;
+            e->outcome[m].solves++;
+            if (k.status == KS_CONV) {
+                upd = true;
+                continue;
+            }
+            act[m] = 0;
+            fail(m, k.status == KS_BREAK ? EWG_NUMERICAL_ISSUE : EWG_NOT_CONVERGED);
+        }
+        if (!upd) continue;
+        if (int r = upload(act, e->act)) return r;
+        hipLaunchKernelGGL(k_b_update, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0);
+        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(BT), 0, s, mp, e->act, e->part0, 0, e->pres);
+        ENS_TRY(e, hipMemcpyAsync(res.data(), e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        ENS_TRY(e, hipStreamSynchronize(s));
+        for (int m = 0; m < n; ++m) {
+            if (!act[m]) continue;
+            if (!std::isfinite(res[m])) fail(m, EWG_NUMERICAL_ISSUE);
+            else ++it[m];
+        }
+    }
+    for (int m = 0; m < n; ++m) {
+        if (state[m] == DONE && !(err[m] <= e->host[m].newton_tol)) fail(m, EWG_NOT_CONVERGED);
+        if (part[m]) e->outcome[m].last_error = err[m];
+    }
+    return EWG_OK;
+}
+
+// dst <- src for the members with sel[m] (act buffer reused)
+int batched_copy(ewg_ensemble *e, const std::vector<int> &sel, int dst, int src) {
+    const int n = static_cast<int>(e->members.size());
+    ENS_TRY(e, hipMemcpyAsync(e->act, sel.data(), n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, maps(e), e->act, dst, src);
+    ENS_TRY(e, hipGetLastError());
+    ENS_TRY(e, hipStreamSynchronize(e->stream));
+    return EWG_OK;
+}
+
+// one time step of dt for every member still running; a failed solve is retried from
+// the step's start with 2, 4, ... substeps (FvBaseProblem::timeIntegration halves the
+// step, fvbaseproblem.hh:507-553); commit = 0: return to the start state afterwards
+int batched_step(ewg_ensemble *e, double dt, bool commit) {
+    const int n = static_cast<int>(e->members.size());
+    std::vector<int> in(n, 0), level(n, 0), sub(n, 0), done(n, 0), fin(n, 0), rc(n, 0);
+    std::vector<double> err(n, 0.0), dts(n, dt);
+    for (int m = 0; m < n; ++m) in[m] = e->outcome[m].status == 0;
+    if (int r = batched_copy(e, in, V_XS, V_X1)) return r;
+    while (true) {
+        std::vector<int> part(n, 0);
+        bool any = false;
+        for (int m = 0; m < n; ++m) {
+            part[m] = in[m] && !done[m];
+            any |= part[m] != 0;
+            dts[m] = dt / (double)(1 << level[m]);
+        }
+        if (!any) break;
+        if (int r = batched_newton(e, part, dts, rc, err)) return r;
+        std::vector<int> ok(n, 0), bad(n, 0);
+        for (int m = 0; m < n; ++m) {
+            if (!part[m]) continue;
+            if (rc[m] == 0) {
+                ok[m] = 1;
+                if (++sub[m] == (1 << level[m])) done[m] = 1;
+            } else {
+                bad[m] = 1;
+                sub[m] = 0;
+                if (level[m] < e->host[m].max_div) {
+                    ++level[m];
+                    e->outcome[m].divisions++;
+                } else {
+                    done[m] = 1;
+                    fin[m] = rc[m];
+                }
+            }
+        }
+        if (int r = batched_copy(e, ok, V_X1, V_X0)) return r;   // substep accepted
+        if (int r = batched_copy(e, bad, V_X0, V_XS)) return r;  // back to the start
+        if (int r = batched_copy(e, bad, V_X1, V_XS)) return r;
+    }
+    std::vector<int> back(n, 0);
+    for (int m = 0; m < n; ++m) {
+        if (!in[m]) continue;
+        if (fin[m]) e->outcome[m].status = fin[m];
+        else if (commit) e->outcome[m].steps++;
+        back[m] = fin[m] != 0 || !commit;
+    }
+    if (int r = batched_copy(e, back, V_X0, V_XS)) return r;
+    if (int r = batched_copy(e, back, V_X1, V_XS)) return r;
+    return EWG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -462,9 +1221,20 @@ int ewg_ensemble_create(ewg_ctx *const *members, int32_t n, ewg_ensemble **out) 
         m.lin_max_error = c->prm.linear_max_error;
         m.lin_max_it = c->prm.linear_max_iterations;
         m.fp32 = c->prm.precond_fp32 ? 1 : 0;
+        m.max_div = std::max(0, std::min(c->prm.max_time_step_divisions, 16));
     }
     auto *e = new ewg_ensemble();
     e->members.assign(members, members + n);
+    e->xs.assign(n, nullptr);
+    for (int i = 0; i < n; ++i) {
+        if (hipMalloc(reinterpret_cast<void **>(&e->xs[i]), 2 * (size_t)host[i].lin.g.N * sizeof(double)) !=
+            hipSuccess) {
+            ewg_ensemble_destroy(e);
+            return EWG_DEVICE_ERROR;
+        }
+        host[i].xs = e->xs[i];
+    }
+    e->host = host;
     e->outcome.assign(n, Outcome{});
     auto fail = [&](hipError_t) {
         ewg_ensemble_destroy(e);
@@ -481,6 +1251,7 @@ int ewg_ensemble_create(ewg_ctx *const *members, int32_t n, ewg_ensemble **out) 
     for (ewg_ctx *c : e->members)
         if ((h = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(h);
     if (upload_order(e, true) != EWG_OK) return fail(hipErrorUnknown);
+    if (batched_setup(e) != EWG_OK) return fail(hipErrorUnknown);
     *out = e;
     return EWG_OK;
 }
@@ -492,6 +1263,13 @@ void ewg_ensemble_destroy(ewg_ensemble *e) {
     if (e->dmem) (void)hipFree(e->dmem);
     if (e->dorder) (void)hipFree(e->dorder);
     if (e->dout) (void)hipFree(e->dout);
+    if (e->iter_graph) (void)hipGraphExecDestroy(e->iter_graph);
+    void *bufs[] = {e->cmem, e->coff, e->lmem, e->loff, e->act, e->niter, e->ilu_err, e->ks,
+                    e->part0, e->part1, e->part2, e->pres, e->partAny, e->dts};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (double *b : e->xs)
+        if (b) (void)hipFree(b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -502,44 +1280,57 @@ int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
     if (!e || !(dt > 0.0)) return EWG_INVALID_ARGUMENT;
     const int n = static_cast<int>(e->members.size());
     ewg_ensemble_tuning t{};
-    const int shapes[3] = {256, 512, 1024};
     double best = 1e300;
     hipEvent_t e0, e1;
     ENS_TRY(e, hipEventCreate(&e0));
     ENS_TRY(e, hipEventCreate(&e1));
-    for (int lf = 1; lf >= 0; --lf)
-        for (int s = 0; s < 3; ++s) {
-            const int k = t.ncandidates++;
-            t.cand_threads[k] = shapes[s];
-            t.cand_largest_first[k] = lf;
-            if (int rc = upload_order(e, lf != 0)) return rc;
-            ENS_TRY(e, hipMemcpyAsync(e->dout, e->outcome.data(), n * sizeof(Outcome),
+    const std::vector<Outcome> saved = e->outcome;
+    const int cand[4][2] = {{EWG_ENSEMBLE_BATCHED, 0}, {EWG_ENSEMBLE_PERSISTENT, 256},
+                            {EWG_ENSEMBLE_PERSISTENT, 512}, {EWG_ENSEMBLE_PERSISTENT, 1024}};
+    for (const auto &c : cand) {
+        const int k = t.ncandidates++;
+        t.cand_mode[k] = c[0];
+        t.cand_threads[k] = c[1];
+        t.cand_largest_first[k] = 1;
+        bool ok = true;
+        double ms = 0.0;
+        if (c[0] == EWG_ENSEMBLE_BATCHED) {
+            const double t0 = now_s();
+            if (int rc = batched_step(e, dt, false)) return rc;
+            ms = 1e3 * (now_s() - t0);
+            for (int i = 0; i < n; ++i) ok &= e->outcome[i].status == 0 || saved[i].status != 0;
+            e->outcome = saved;
+        } else {
+            if (int rc = upload_order(e, true)) return rc;
+            ENS_TRY(e, hipMemcpyAsync(e->dout, saved.data(), n * sizeof(Outcome),
                                       hipMemcpyHostToDevice, e->stream));
             ENS_TRY(e, hipEventRecord(e0, e->stream));
-            if (int rc = launch(e, shapes[s], 1, dt, 0)) return rc;
+            if (int rc = launch(e, c[1], 1, dt, 0)) return rc;
             ENS_TRY(e, hipEventRecord(e1, e->stream));
             std::vector<Outcome> trial(n);
             ENS_TRY(e, hipMemcpyAsync(trial.data(), e->dout, n * sizeof(Outcome),
                                       hipMemcpyDeviceToHost, e->stream));
             ENS_TRY(e, hipStreamSynchronize(e->stream));
-            float ms = 0.0f;
-            ENS_TRY(e, hipEventElapsedTime(&ms, e0, e1));
-            bool ok = true;
-            for (int i = 0; i < n; ++i) ok &= trial[i].status == 0 || e->outcome[i].status != 0;
-            t.cand_ms[k] = ok ? ms : -1.0;
-            if (ok && ms < best) {
-                best = ms;
-                t.threads = shapes[s];
-                t.largest_first = lf;
-            }
+            float f = 0.0f;
+            ENS_TRY(e, hipEventElapsedTime(&f, e0, e1));
+            ms = f;
+            for (int i = 0; i < n; ++i) ok &= trial[i].status == 0 || saved[i].status != 0;
         }
+        t.cand_ms[k] = ok ? ms : -1.0;
+        if (ok && ms < best) {
+            best = ms;
+            t.mode = c[0];
+            t.threads = c[1];
+            t.largest_first = 1;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    ENS_TRY(e, hipMemcpy(e->dout, e->outcome.data(), n * sizeof(Outcome), hipMemcpyHostToDevice));
+    ENS_TRY(e, hipMemcpy(e->dout, saved.data(), n * sizeof(Outcome), hipMemcpyHostToDevice));
     if (best >= 1e300) return set_err(e, EWG_NUMERICAL_ISSUE, "every candidate failed");
-    e->threads = t.threads;
-    e->largest_first = t.largest_first != 0;
-    if (int rc = upload_order(e, e->largest_first)) return rc;
+    e->mode = t.mode;
+    if (t.mode == EWG_ENSEMBLE_PERSISTENT) e->threads = t.threads;
+    e->largest_first = true;
     if (out) *out = t;
     return EWG_OK;
 }
@@ -547,6 +1338,17 @@ int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
 int ewg_ensemble_run(ewg_ensemble *e, double dt, int32_t steps, double *seconds) {
     if (!e || !(dt > 0.0) || steps < 1) return EWG_INVALID_ARGUMENT;
     const int n = static_cast<int>(e->members.size());
+    if (e->mode == EWG_ENSEMBLE_BATCHED) {
+        ENS_TRY(e, hipStreamSynchronize(e->stream));
+        const double t0 = now_s();
+        for (int k = 0; k < steps; ++k)
+            if (int rc = batched_step(e, dt, true)) return rc;
+        if (seconds) *seconds = now_s() - t0;
+        ENS_TRY(e, hipMemcpy(e->dout, e->outcome.data(), n * sizeof(Outcome), hipMemcpyHostToDevice));
+        return EWG_OK;
+    }
+    ENS_TRY(e, hipMemcpy(e->dout, e->outcome.data(), n * sizeof(Outcome), hipMemcpyHostToDevice));
+    if (int rc = upload_order(e, e->largest_first)) return rc;
     if (!e->graph || e->graph_steps != steps || e->graph_dt != dt ||
         e->graph_threads != e->threads || e->graph_largest != e->largest_first) {
         if (e->graph) {
@@ -576,6 +1378,21 @@ int ewg_ensemble_run(ewg_ensemble *e, double dt, int32_t steps, double *seconds)
     return EWG_OK;
 }
 
+/* choose the execution mode without tuning (EWG_ENSEMBLE_*; threads for the persistent
+   mode: 256, 512 or 1024) */
+int ewg_ensemble_set_mode(ewg_ensemble *e, int32_t mode, int32_t threads) {
+    if (!e) return EWG_INVALID_ARGUMENT;
+    if (mode == EWG_ENSEMBLE_BATCHED) {
+        e->mode = mode;
+        return EWG_OK;
+    }
+    if (mode != EWG_ENSEMBLE_PERSISTENT || (threads != 256 && threads != 512 && threads != 1024))
+        return set_err(e, EWG_INVALID_ARGUMENT, "mode / threads");
+    e->mode = mode;
+    e->threads = threads;
+    return EWG_OK;
+}
+
 int ewg_ensemble_stats(const ewg_ensemble *e, int32_t i, ewg_ensemble_member_stats *out) {
     if (!e || !out || i < 0 || i >= static_cast<int>(e->members.size())) return EWG_INVALID_ARGUMENT;
     const Outcome &o = e->outcome[i];
@@ -584,7 +1401,7 @@ int ewg_ensemble_stats(const ewg_ensemble *e, int32_t i, ewg_ensemble_member_sta
     out->linear_iterations = o.linear;
     out->linear_solves = o.solves;
     out->status = o.status;
-    out->pad_ = 0;
+    out->step_divisions = o.divisions;
     out->last_error = o.last_error;
     return EWG_OK;
 }

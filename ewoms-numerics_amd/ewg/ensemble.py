@@ -1,6 +1,7 @@
 """ewg.ensemble -- binding of include/ewg_ensemble.h: realizations of uneven size
-time-stepped together in one persistent launch (one workgroup per member), replayed from
-a HIP graph, with the launch shape chosen by the library's tuner.
+time-stepped together, either as batched launches over the cells of all members (each
+BiCGStab iteration of all members replayed from a HIP graph) or in one persistent launch
+(one workgroup per member), the mode and launch shape chosen by the library's tuner.
 
 `python -m ewg.ensemble` measures the ensemble workload of config 3 (256 members of the
 config-1 physics, sizes from {32^3, 48^3, 64x64x32, 96x96x48}, FP32 preconditioner):
@@ -25,15 +26,20 @@ _vp = C.c_void_p
 class MemberStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("newton_iterations", C.c_int64),
                 ("linear_iterations", C.c_int64), ("linear_solves", C.c_int64),
-                ("status", C.c_int32), ("pad_", C.c_int32), ("last_error", C.c_double)]
+                ("status", C.c_int32), ("step_divisions", C.c_int32),
+                ("last_error", C.c_double)]
 
 
 MAXC = 8
 
 
+PERSISTENT, BATCHED = 0, 1
+MODES = {PERSISTENT: "persistent", BATCHED: "batched"}
+
+
 class Tuning(C.Structure):
-    _fields_ = [("threads", C.c_int32), ("largest_first", C.c_int32),
-                ("ncandidates", C.c_int32), ("pad_", C.c_int32),
+    _fields_ = [("mode", C.c_int32), ("threads", C.c_int32), ("largest_first", C.c_int32),
+                ("ncandidates", C.c_int32), ("cand_mode", C.c_int32 * MAXC),
                 ("cand_threads", C.c_int32 * MAXC), ("cand_largest_first", C.c_int32 * MAXC),
                 ("cand_ms", C.c_double * MAXC)]
 
@@ -50,6 +56,7 @@ def _lib():
             "ewg_ensemble_destroy": (None, [_vp]),
             "ewg_ensemble_last_error": (C.c_char_p, [_vp]),
             "ewg_ensemble_tune": (C.c_int, [_vp, C.c_double, C.POINTER(Tuning)]),
+            "ewg_ensemble_set_mode": (C.c_int, [_vp, C.c_int32, C.c_int32]),
             "ewg_ensemble_run": (C.c_int, [_vp, C.c_double, C.c_int32,
                                            C.POINTER(C.c_double)]),
             "ewg_ensemble_stats": (C.c_int, [_vp, C.c_int32, C.POINTER(MemberStats)]),
@@ -102,10 +109,12 @@ class Ensemble:
     def tune(self, dt):
         t = Tuning()
         self._check(self.lib.ewg_ensemble_tune(self.h, dt, C.byref(t)))
-        return {"threads": t.threads, "largest_first": bool(t.largest_first),
-                "candidates": [{"threads": t.cand_threads[k],
-                                "largest_first": bool(t.cand_largest_first[k]),
+        return {"mode": MODES[t.mode], "threads": t.threads,
+                "candidates": [{"mode": MODES[t.cand_mode[k]], "threads": t.cand_threads[k],
                                 "ms": t.cand_ms[k]} for k in range(t.ncandidates)]}
+
+    def set_mode(self, mode, threads=512):
+        self._check(self.lib.ewg_ensemble_set_mode(self.h, mode, threads))
 
     def run(self, dt, steps):
         sec = C.c_double(0)
@@ -115,7 +124,7 @@ class Ensemble:
     def stats(self, i):
         s = MemberStats()
         self._check(self.lib.ewg_ensemble_stats(self.h, i, C.byref(s)))
-        return {n: getattr(s, n) for n, _ in s._fields_ if n != "pad_"}
+        return {n: getattr(s, n) for n, _ in s._fields_}
 
 
 def build_members(n, seed, steps=40, sizes=problems.ENSEMBLE_SIZES, **params):
@@ -131,12 +140,21 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--dt", type=float, default=500.0)
+    ap.add_argument("--tune", action="store_true",
+                    help="time every mode on one step first (the persistent candidates take "
+                         "long on large members)")
+    ap.add_argument("--mode", choices=("batched", "persistent"), default="batched")
+    ap.add_argument("--threads", type=int, default=512)
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
     probs, ctxs = build_members(args.members, args.seed, args.steps)
     ens = Ensemble(ctxs)
     t1 = time.perf_counter()
-    tuning = ens.tune(args.dt)
+    if args.tune:
+        tuning = ens.tune(args.dt)
+    else:
+        ens.set_mode(BATCHED if args.mode == "batched" else PERSISTENT, args.threads)
+        tuning = {"mode": args.mode, "threads": args.threads, "candidates": []}
     t2 = time.perf_counter()
     sec = ens.run(args.dt, args.steps)
     st = [ens.stats(i) for i in range(len(probs))]
@@ -149,6 +167,7 @@ def main(argv=None):
         "members": len(probs), "cells": int(cells.sum()), "steps": args.steps,
         "seconds": sec, "completed_steps": int(done.sum()),
         "failed_members": int(sum(1 for s in st if s["status"] != 0)),
+        "step_divisions": int(sum(s["step_divisions"] for s in st)),
         "newton_iterations": int(sum(s["newton_iterations"] for s in st)),
         "linear_iterations": int(sum(s["linear_iterations"] for s in st)),
         "sizes": {f"{a}x{b}x{c}": int(sum(1 for P in probs if (P.nx, P.ny, P.nz) == (a, b, c)))

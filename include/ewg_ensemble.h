@@ -8,17 +8,19 @@
  * tile buffers) and, for the FP32 preconditioner, params.precond_fp32 = 1. Every member
  * keeps its own parameters (tolerances, limits).
  *
- * ewg_ensemble_run advances all members by `steps` implicit-Euler steps of dt in one
- * persistent launch: one workgroup per member runs that member's whole time loop
- * (Newton with the reference's rules, BiCGStab with its criterion, ILU(0) in natural
- * order by hyperplanes), so Newton and Krylov convergence are decided per member on the
- * device, and members run independently to the end. The launch is captured once in a
- * HIP graph per (steps, dt, launch shape) and replayed. A member whose Newton solve fails
- * keeps its last converged state and stops (its status says why); the reference would
- * retry with half the time step, which a lock-step ensemble does not do.
+ * ewg_ensemble_run advances all members by `steps` implicit-Euler steps of dt (Newton
+ * with the reference's rules, BiCGStab with its criterion, ILU(0) in natural order by
+ * hyperplanes), with each member's Newton and Krylov convergence its own, in one of two
+ * modes (below): in the persistent mode one launch, captured once in a HIP graph per
+ * (steps, dt, launch shape) and replayed, runs the whole time loop of every member. A
+ * failed solve is retried from the step's start with 2, 4, ... substeps (up to
+ * params.max_time_step_divisions halvings), as the reference halves a failed step, so the
+ * member stays at the same time level as the others; a member whose step still fails
+ * keeps its last converged state and stops (its status says why).
  *
- * ewg_ensemble_tune times the candidate launch shapes on one step from the current state
- * (restoring it afterwards) and keeps the fastest for the following runs.
+ * ewg_ensemble_tune times the candidates (the batched mode; the persistent mode with
+ * 256, 512 and 1024 threads per member) on one step from the current state, restoring it
+ * afterwards, and keeps the fastest for the following runs (default: batched).
  */
 #ifndef EWG_ENSEMBLE_H
 #define EWG_ENSEMBLE_H
@@ -39,17 +41,26 @@ typedef struct ewg_ensemble_member_stats {
     int64_t linear_iterations; /* BiCGStab iterations */
     int64_t linear_solves;
     int32_t status;            /* 0 running; else the EWG_* code of the failed solve */
-    int32_t pad_;
+    int32_t step_divisions;    /* time steps halved after a failed solve */
     double last_error;         /* Newton error of the last solve */
 } ewg_ensemble_member_stats;
 
 #define EWG_ENSEMBLE_MAX_CANDIDATES 8
 
+/* Execution modes: EWG_ENSEMBLE_PERSISTENT runs every member's whole time loop in one
+ * workgroup of one launch; EWG_ENSEMBLE_BATCHED runs each phase (linearization, norms,
+ * ILU(0) hyperplanes, each BiCGStab step) as one launch over the cells of all members,
+ * with per-member reductions and per-member Newton and Krylov states, one BiCGStab
+ * iteration of all members replayed from a HIP graph. */
+#define EWG_ENSEMBLE_PERSISTENT 0
+#define EWG_ENSEMBLE_BATCHED 1
+
 typedef struct ewg_ensemble_tuning {
-    int32_t threads;           /* workgroup size chosen (threads per member) */
-    int32_t largest_first;     /* 1: members ordered by decreasing cell count */
+    int32_t mode;              /* EWG_ENSEMBLE_* chosen */
+    int32_t threads;           /* persistent mode: workgroup size (threads per member) */
+    int32_t largest_first;     /* persistent mode: members ordered by decreasing size */
     int32_t ncandidates;
-    int32_t pad_;
+    int32_t cand_mode[EWG_ENSEMBLE_MAX_CANDIDATES];
     int32_t cand_threads[EWG_ENSEMBLE_MAX_CANDIDATES];
     int32_t cand_largest_first[EWG_ENSEMBLE_MAX_CANDIDATES];
     double cand_ms[EWG_ENSEMBLE_MAX_CANDIDATES];  /* one step; < 0: candidate failed */
@@ -61,6 +72,8 @@ void ewg_ensemble_destroy(ewg_ensemble *ens);
 const char *ewg_ensemble_last_error(const ewg_ensemble *ens);
 
 int ewg_ensemble_tune(ewg_ensemble *ens, double dt, ewg_ensemble_tuning *out);
+/* Choose the mode without tuning (threads: 256, 512 or 1024 for the persistent mode). */
+int ewg_ensemble_set_mode(ewg_ensemble *ens, int32_t mode, int32_t threads);
 int ewg_ensemble_run(ewg_ensemble *ens, double dt, int32_t steps, double *seconds);
 int ewg_ensemble_stats(const ewg_ensemble *ens, int32_t i,
                        ewg_ensemble_member_stats *out);

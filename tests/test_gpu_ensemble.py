@@ -37,14 +37,16 @@ def _single_runs(probs, params, steps):
     return out
 
 
-def test_members_match_single_contexts():
+@pytest.mark.parametrize("mode", ["batched", "persistent"])
+def test_members_match_single_contexts(mode):
     import ewg
     from ewg import ensemble
     probs = _problems()
     prm = ensemble.member_params()
     ctxs = [ewg.GpuContext(P, prm) for P in probs]
     ens = ensemble.Ensemble(ctxs)
-    print("ensemble of", [(P.nx, P.ny, P.nz) for P in probs], flush=True)
+    ens.set_mode(ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT, 512)
+    print(mode, "ensemble of", [(P.nx, P.ny, P.nz) for P in probs], flush=True)
     sec = ens.run(probs[0].dt, 3)
     ref = _single_runs(probs, prm, 3)
     for i, P in enumerate(probs):
@@ -61,7 +63,8 @@ def test_members_match_single_contexts():
     print(f"3 steps in {sec:.3f} s", flush=True)
 
 
-def test_fp64_members_match_the_oracle():
+@pytest.mark.parametrize("mode", ["batched", "persistent"])
+def test_fp64_members_match_the_oracle(mode):
     import ewg
     from ewg import ensemble
     from oracle.pyoracle import OracleContext
@@ -69,6 +72,7 @@ def test_fp64_members_match_the_oracle():
     prm = ensemble.member_params(precond_fp32=0, newton_tolerance=1e-8)
     ctxs = [ewg.GpuContext(P, prm) for P in probs]
     ens = ensemble.Ensemble(ctxs)
+    ens.set_mode(ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT, 512)
     ens.run(probs[0].dt, 3)
     for i, P in enumerate(probs):
         o = OracleContext(P, ewg.default_params(newton_tolerance=1e-8))
@@ -86,33 +90,38 @@ def test_fp64_members_match_the_oracle():
         assert dS < 1e-8
 
 
-def test_failing_member_is_isolated():
-    """A member whose Newton solve fails (here: NewtonMaxError below its first error)
-    keeps its state and stops; the others are unaffected."""
+@pytest.mark.parametrize("mode", ["batched", "persistent"])
+def test_failing_member_is_isolated(mode):
+    """A member whose Newton solve fails (here: NewtonMaxError below its first error) is
+    retried with halved steps (two halvings allowed here), then keeps its state and
+    stops; the others are unaffected."""
     import ewg
     from ewg import ensemble
     probs = _problems()
     prm = ensemble.member_params()
-    bad = ensemble.member_params(newton_max_error=1e-9)
+    bad = ensemble.member_params(newton_max_error=1e-9, max_time_step_divisions=2)
     ctxs = [ewg.GpuContext(P, bad if i == 1 else prm) for i, P in enumerate(probs)]
     x_before = ctxs[1].solution(0)
     ens = ensemble.Ensemble(ctxs)
+    ens.set_mode(ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT, 512)
     ens.run(probs[0].dt, 2)
     ref = _single_runs([probs[0], probs[2]], prm, 2)
     st = [ens.stats(i) for i in range(4)]
     print(st, flush=True)
     assert st[1]["status"] == ewg.capi.EWG_NUMERICAL_ISSUE and st[1]["steps"] == 0
+    assert st[1]["step_divisions"] == 2 and st[1]["newton_iterations"] == 3
     assert np.array_equal(ctxs[1].solution(0), x_before)
     for i, (its, _, x) in zip((0, 2), ref):
-        assert st[i]["status"] == 0 and st[i]["steps"] == 2
+        assert st[i]["status"] == 0 and st[i]["steps"] == 2 and st[i]["step_divisions"] == 0
         assert st[i]["newton_iterations"] == its
         assert np.abs(ctxs[i].solution(0)[1::2] - x[1::2]).max() < 1e-7
 
 
-def test_tuner_and_graph_replay():
+@pytest.mark.parametrize("mode", ["batched", "persistent"])
+def test_tuner_and_graph_replay(mode):
     """The tuner's trial steps leave the state as it was; one run of three steps and
-    three runs of one step (replays of different graph captures, the same launch shape)
-    give identical states."""
+    three runs of one step (graph replays; the same launch shape) give identical
+    states."""
     import ewg
     from ewg import ensemble
     probs = _problems()
@@ -122,13 +131,16 @@ def test_tuner_and_graph_replay():
     x0 = [c.solution(0) for c in t_ctx]
     t = et.tune(probs[0].dt)
     print("tuning", t, flush=True)
-    assert t["threads"] in (256, 512, 1024)
+    assert t["mode"] in ("batched", "persistent")
     assert all(c["ms"] > 0 for c in t["candidates"])
     assert all(np.array_equal(c.solution(0), x) for c, x in zip(t_ctx, x0))
     assert all(et.stats(i)["steps"] == 0 for i in range(4))
     a = [ewg.GpuContext(P, prm) for P in probs]
     b = [ewg.GpuContext(P, prm) for P in probs]
     ea, eb = ensemble.Ensemble(a), ensemble.Ensemble(b)
+    m = ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT
+    ea.set_mode(m, 512)
+    eb.set_mode(m, 512)
     ea.run(probs[0].dt, 3)
     for _ in range(3):
         eb.run(probs[0].dt, 1)
