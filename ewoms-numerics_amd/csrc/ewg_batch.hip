@@ -855,6 +855,9 @@ struct ewg_ensemble {
     double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *pres = nullptr;
     int *partAny = nullptr;
     hipGraphExec_t iter_graph = nullptr;  // one BiCGStab iteration of every member
+    // members with the tiles' wavefront layout (params.ilu_schedule = 0, 3D, all of one
+    // precision): the preconditioner applications run as batched tile-wavefront sweeps
+    void *wl_py = nullptr, *wl_rz = nullptr;
     std::vector<KrylovState> ks_host;
     std::string err;
 };
@@ -952,13 +955,52 @@ int batched_setup(ewg_ensemble *e) {
     ENS_TRY(e, hipMemcpy(e->lmem, lmem.data(), lmem.size() * sizeof(int), hipMemcpyHostToDevice));
     ENS_TRY(e, hipMemcpy(e->loff, loff.data(), loff.size() * sizeof(int), hipMemcpyHostToDevice));
     e->ks_host.resize(n);
+    // batched tile-wavefront sweeps when every member carries the layout
+    bool wl = true;
+    for (int m = 0; m < n; ++m) {
+        const ewg_ctx *c = e->members[m];
+        wl &= c->g.dim == 3 && c->prm.ilu_schedule == 0 && c->wf.ticket != nullptr &&
+              c->g.N <= kWfMaxCells && c->wl64.L != nullptr &&
+              (!c->prm.precond_fp32 || c->wl32.L != nullptr) &&
+              (c->prm.precond_fp32 != 0) == (e->members[0]->prm.precond_fp32 != 0);
+    }
+    if (wl) {
+        const bool fp32 = e->members[0]->prm.precond_fp32 != 0;
+        std::vector<WlMember> py(n), rz(n);
+        for (int m = 0; m < n; ++m) {
+            const ewg_ctx *c = e->members[m];
+            WlMember w{};
+            w.g = c->g;
+            w.sync = c->wf;
+            if (fp32) {
+                w.L = c->wl32.L; w.U = c->wl32.U; w.D = c->wl32.D; w.w = c->wl32.w;
+            } else {
+                w.L = c->wl64.L; w.U = c->wl64.U; w.D = c->wl64.D; w.w = c->wl64.w;
+            }
+            w.status = &e->ks[m].status;  // device address of the member's Krylov status
+            py[m] = w;
+            py[m].d = e->host[m].p;
+            py[m].out = e->host[m].y;
+            rz[m] = w;
+            rz[m].d = e->host[m].r;
+            rz[m].out = e->host[m].z;
+        }
+        e->wl_py = wl_batch_create(py, fp32, e->stream);
+        e->wl_rz = wl_batch_create(rz, fp32, e->stream);
+        if (!e->wl_py || !e->wl_rz) return set_err(e, EWG_DEVICE_ERROR, "wavefront batch");
+    }
     return EWG_OK;
 }
 
 Maps maps(const ewg_ensemble *e) { return Maps{e->cmem, e->coff, e->lmem, e->loff}; }
 
-// out = (LU)^-1 in, hyperplane by hyperplane for all members whose Krylov status runs
+// out = (LU)^-1 in for all members whose Krylov status runs: tile-wavefront sweeps over
+// the members' wavefront layouts, else hyperplane by hyperplane
 void launch_apply(ewg_ensemble *e, int src, int dst) {
+    if (e->wl_py) {
+        wl_batch_apply(src == V_P ? e->wl_py : e->wl_rz, e->stream);
+        return;
+    }
     const Maps mp = maps(e);
     for (int l = 0; l < e->lmax; ++l)
         hipLaunchKernelGGL(k_b_fwd, dim3(e->nblkL), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, l, src, dst);
@@ -1071,7 +1113,18 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
         hipLaunchKernelGGL(k_b_zero, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ilu_err);
         for (int l = 0; l < e->lmax; ++l)
             hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), dim3(BT), 0, s, e->dmem, mp, e->act, l, e->ilu_err);
-        hipLaunchKernelGGL(k_b_round, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act);
+        if (e->wl_py) {
+            for (int m = 0; m < n; ++m) {
+                if (!act[m]) continue;
+                ewg_ctx *c = e->members[m];
+                if (c->prm.precond_fp32)
+                    wl_pack32(c->g, c->wf, c->L64, c->D64, c->J, c->wl32, s);
+                else
+                    wl_pack64(c->g, c->wf, c->L64, c->D64, c->J, c->wl64, s);
+            }
+        } else {
+            hipLaunchKernelGGL(k_b_round, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act);
+        }
         hipLaunchKernelGGL(k_b_init, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0, e->part1);
         hipLaunchKernelGGL(k_b_ks_init, dim3(n), dim3(BT), 0, s, e->dmem, mp, e->act, e->ilu_err, e->ks, e->part0, e->part1);
         ENS_TRY(e, hipGetLastError());
@@ -1264,6 +1317,8 @@ void ewg_ensemble_destroy(ewg_ensemble *e) {
     if (e->dorder) (void)hipFree(e->dorder);
     if (e->dout) (void)hipFree(e->dout);
     if (e->iter_graph) (void)hipGraphExecDestroy(e->iter_graph);
+    wl_batch_free(e->wl_py);
+    wl_batch_free(e->wl_rz);
     void *bufs[] = {e->cmem, e->coff, e->lmem, e->loff, e->act, e->niter, e->ilu_err, e->ks,
                     e->part0, e->part1, e->part2, e->pres, e->partAny, e->dts};
     for (void *b : bufs)

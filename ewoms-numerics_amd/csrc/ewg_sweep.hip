@@ -14,6 +14,8 @@
 //     (one store and two loads per step).
 // The arithmetic per row is the one of the hyperplane kernels (ascending columns, no
 // fused multiply-add), so both schedules agree to the last bit up to the sign of zeros.
+#include <vector>
+
 #include "ewg_internal.h"
 #include "ewg_wavefront.h"
 
@@ -418,7 +420,7 @@ __device__ __forceinline__ void lds_st(int *p, int v) {
 }
 
 template <class T, bool FWD, int PD>
-__global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
+__device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     constexpr int TX = 8, TY = 8, NT = 64;
     static_assert(PD < kRing, "the edge wave must be able to run ahead of the compute wave");
     __shared__ P2<T> xch[2][NT];
@@ -765,6 +767,24 @@ __global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
     }
 }
 
+template <class T, bool FWD, int PD>
+__global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
+    sweep_ws_body<T, FWD, PD>(a);
+}
+
+// the same sweep for several realizations in one launch: block -> realization by tmem,
+// each block then takes a tile ticket of its realization as the single launch does
+template <class T, bool FWD, int PD>
+__global__ void __launch_bounds__(192) k_sweep_ws_batch(const WsArgs<T> *args, const int *tmem) {
+    const WsArgs<T> a = args[tmem[blockIdx.x]];
+    sweep_ws_body<T, FWD, PD>(a);
+}
+
+__global__ void k_reset_tickets(unsigned *const *tickets, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) *tickets[i] = 0u;
+}
+
 __global__ void k_sweep_reset_ticket(unsigned *ticket) { *ticket = 0u; }
 
 // Copy the factors into the wavefront layout: one thread per layout entry. FWD: L (3
@@ -918,6 +938,97 @@ void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, 
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
                 double *v, const int *status, hipStream_t s) {
     wl_apply<float>(g, sync, wl, d, v, status, s);
+}
+
+struct WlBatch {
+    int n = 0, ntiles = 0;
+    bool fp32 = false;
+    void *fwd = nullptr, *bwd = nullptr;  // WsArgs<T>[n]
+    int *tmem = nullptr;                  // block -> realization
+    unsigned **tickets = nullptr;         // [n]
+};
+
+template <class T>
+static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t s) {
+    std::vector<WsArgs<T>> f(m.size()), bw(m.size());
+    std::vector<int> tmem;
+    std::vector<unsigned *> tk(m.size());
+    for (size_t i = 0; i < m.size(); ++i) {
+        const Grid &g = m[i].g;
+        const size_t entries = wl_entries(g, m[i].sync);
+        WsArgs<T> a{};
+        a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nxny = g.nxny; a.N = g.N;
+        a.nsteps = g.nz + 14;
+        a.wl_bytes_blk = (unsigned)(entries * 4 * sizeof(T));
+        a.wl_bytes_vec = (unsigned)(entries * 2 * sizeof(T));
+        a.d = m[i].d;
+        const T *L = static_cast<const T *>(m[i].L), *U = static_cast<const T *>(m[i].U);
+        a.blk[0] = L; a.blk[1] = L + entries * 4; a.blk[2] = L + 2 * entries * 4;
+        a.Dinv = nullptr; a.w = static_cast<T *>(m[i].w); a.out = nullptr;
+        a.status = m[i].status; a.sync = m[i].sync;
+        f[i] = a;
+        WsArgs<T> c = a;
+        c.d = nullptr;
+        c.blk[0] = U; c.blk[1] = U + entries * 4; c.blk[2] = U + 2 * entries * 4;
+        c.Dinv = static_cast<const T *>(m[i].D);
+        c.out = m[i].out;
+        bw[i] = c;
+        tmem.insert(tmem.end(), m[i].sync.ntiles, (int)i);
+        tk[i] = m[i].sync.ticket;
+    }
+    b.ntiles = (int)tmem.size();
+    const size_t bytes = m.size() * sizeof(WsArgs<T>);
+    if (hipMalloc(&b.fwd, bytes) != hipSuccess || hipMalloc(&b.bwd, bytes) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&b.tmem), tmem.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&b.tickets), m.size() * sizeof(unsigned *)) != hipSuccess)
+        return -1;
+    if (hipMemcpyAsync(b.fwd, f.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b.bwd, bw.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b.tmem, tmem.data(), tmem.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b.tickets, tk.data(), m.size() * sizeof(unsigned *), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+void *wl_batch_create(const std::vector<WlMember> &members, bool fp32, hipStream_t s) {
+    auto *b = new WlBatch();
+    b->n = (int)members.size();
+    b->fp32 = fp32;
+    const int rc = fp32 ? wl_batch_fill<float>(*b, members, s) : wl_batch_fill<double>(*b, members, s);
+    if (rc) {
+        wl_batch_free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void wl_batch_apply(void *h, hipStream_t s) {
+    auto *b = static_cast<WlBatch *>(h);
+    const dim3 rg((b->n + 255) / 256), rb(256);
+    hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
+    if (b->fp32)
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, true, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+                           static_cast<const WsArgs<float> *>(b->fwd), b->tmem);
+    else
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, true, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+                           static_cast<const WsArgs<double> *>(b->fwd), b->tmem);
+    hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
+    if (b->fp32)
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, false, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+                           static_cast<const WsArgs<float> *>(b->bwd), b->tmem);
+    else
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, false, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+                           static_cast<const WsArgs<double> *>(b->bwd), b->tmem);
+}
+
+void wl_batch_free(void *h) {
+    auto *b = static_cast<WlBatch *>(h);
+    if (!b) return;
+    void *p[] = {b->fwd, b->bwd, b->tmem, b->tickets};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    delete b;
 }
 
 }  // namespace ewg

@@ -1,6 +1,8 @@
 // ewg_wavefront.h -- persistent tile-wavefront launches for the 3D ILU(0) recurrences.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 #include "ewg_internal.h"
@@ -75,5 +77,22 @@ void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, 
                 double *v, const int *status, hipStream_t s);
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
                 double *v, const int *status, hipStream_t s);
+
+// Several realizations' preconditioner applications v_i = (L_i U_i)^-1 d_i in one pair
+// of sweep launches (each realization with its own layout, tickets and edge buffers; its
+// blocks take tickets of their own realization, so the schedule inside each is the
+// single launch's). Realizations whose status is not running skip.
+struct WlMember {
+    Grid g;
+    WfSync sync;
+    const void *L, *U, *D;  // WlPrecond<T> arrays
+    void *w;
+    const double *d;
+    double *out;
+    const int *status;
+};
+void *wl_batch_create(const std::vector<WlMember> &members, bool fp32, hipStream_t s);
+void wl_batch_apply(void *batch, hipStream_t s);
+void wl_batch_free(void *batch);
 
 }  // namespace ewg

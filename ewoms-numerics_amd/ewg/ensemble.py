@@ -70,9 +70,10 @@ def _lib():
 
 
 def member_params(**overrides):
-    """Parameters of an ensemble member: the ensemble runs its own hyperplane ILU(0)
-    schedule (ilu_schedule = 1: no tile buffers), FP32 preconditioner by default."""
-    kw = dict(newton_tolerance=1e-6, linear_tolerance=1e-3, precond_fp32=1, ilu_schedule=1)
+    """Parameters of an ensemble member: FP32 preconditioner by default; with
+    ilu_schedule = 0 (default) the batched mode sweeps the members' tile-wavefront
+    layouts, with ilu_schedule = 1 (no tile buffers) their hyperplanes."""
+    kw = dict(newton_tolerance=1e-6, linear_tolerance=1e-3, precond_fp32=1, ilu_schedule=0)
     kw.update(overrides)
     return default_params(**kw)
 
@@ -145,9 +146,12 @@ def main(argv=None):
                          "long on large members)")
     ap.add_argument("--mode", choices=("batched", "persistent"), default="batched")
     ap.add_argument("--threads", type=int, default=512)
+    ap.add_argument("--ilu-schedule", type=int, default=0, choices=(0, 1),
+                    help="0: tile-wavefront sweeps in the batched mode; 1: hyperplane sweeps")
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
-    probs, ctxs = build_members(args.members, args.seed, args.steps)
+    probs, ctxs = build_members(args.members, args.seed, args.steps,
+                                ilu_schedule=args.ilu_schedule)
     ens = Ensemble(ctxs)
     t1 = time.perf_counter()
     if args.tune:
@@ -172,7 +176,8 @@ def main(argv=None):
         "linear_iterations": int(sum(s["linear_iterations"] for s in st)),
         "sizes": {f"{a}x{b}x{c}": int(sum(1 for P in probs if (P.nx, P.ny, P.nz) == (a, b, c)))
                   for (a, b, c) in problems.ENSEMBLE_SIZES},
-        "tuning": tuning, "setup_s": t1 - t0, "tune_s": t2 - t1,
+        "tuning": tuning, "ilu_schedule": args.ilu_schedule,
+        "setup_s": t1 - t0, "tune_s": t2 - t1,
         "config": {"workload": "config-3 ensemble: FP64 state, FP32 ILU(0) preconditioner, "
                                f"dt {args.dt:g} s, Newton tol 1e-6", "seed": args.seed},
     }

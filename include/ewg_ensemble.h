@@ -3,10 +3,13 @@
  *
  * No counterpart in the reference: eWoms runs one simulation per process
  * (Simulator::run, utils/simulator.hh:660-831). Each member here is an ordinary context
- * (ewg_create, ewg_set_static, ewg_set_boundary, ewg_set_solution), created with
- * params.ilu_schedule = 1 (the ensemble solver has its own ILU(0) schedule and needs no
- * tile buffers) and, for the FP32 preconditioner, params.precond_fp32 = 1. Every member
- * keeps its own parameters (tolerances, limits).
+ * (ewg_create, ewg_set_static, ewg_set_boundary, ewg_set_solution), with
+ * params.precond_fp32 = 1 for the FP32 preconditioner. When every member is a 3D context
+ * created with params.ilu_schedule = 0 (the tiles' wavefront layout) and all use one
+ * preconditioner precision, the batched mode applies the preconditioner as tile-wavefront
+ * sweeps over all members at once (two launches per application); otherwise (e.g.
+ * ilu_schedule = 1, no tile buffers) hyperplane by hyperplane. Every member keeps its own
+ * parameters (tolerances, limits).
  *
  * ewg_ensemble_run advances all members by `steps` implicit-Euler steps of dt (Newton
  * with the reference's rules, BiCGStab with its criterion, ILU(0) in natural order by

@@ -37,16 +37,18 @@ def _single_runs(probs, params, steps):
     return out
 
 
-@pytest.mark.parametrize("mode", ["batched", "persistent"])
-def test_members_match_single_contexts(mode):
+@pytest.mark.parametrize("mode,schedule", [("batched", 0), ("batched", 1), ("persistent", 0)])
+def test_members_match_single_contexts(mode, schedule):
+    """schedule 0: the batched mode's tile-wavefront sweeps; 1: its hyperplane sweeps."""
     import ewg
     from ewg import ensemble
     probs = _problems()
-    prm = ensemble.member_params()
+    prm = ensemble.member_params(ilu_schedule=schedule)
     ctxs = [ewg.GpuContext(P, prm) for P in probs]
     ens = ensemble.Ensemble(ctxs)
     ens.set_mode(ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT, 512)
-    print(mode, "ensemble of", [(P.nx, P.ny, P.nz) for P in probs], flush=True)
+    print(mode, "schedule", schedule, "ensemble of", [(P.nx, P.ny, P.nz) for P in probs],
+          flush=True)
     sec = ens.run(probs[0].dt, 3)
     ref = _single_runs(probs, prm, 3)
     for i, P in enumerate(probs):
@@ -63,13 +65,13 @@ def test_members_match_single_contexts(mode):
     print(f"3 steps in {sec:.3f} s", flush=True)
 
 
-@pytest.mark.parametrize("mode", ["batched", "persistent"])
-def test_fp64_members_match_the_oracle(mode):
+@pytest.mark.parametrize("mode,schedule", [("batched", 0), ("batched", 1), ("persistent", 0)])
+def test_fp64_members_match_the_oracle(mode, schedule):
     import ewg
     from ewg import ensemble
     from oracle.pyoracle import OracleContext
     probs = _problems()[:2]
-    prm = ensemble.member_params(precond_fp32=0, newton_tolerance=1e-8)
+    prm = ensemble.member_params(precond_fp32=0, newton_tolerance=1e-8, ilu_schedule=schedule)
     ctxs = [ewg.GpuContext(P, prm) for P in probs]
     ens = ensemble.Ensemble(ctxs)
     ens.set_mode(ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT, 512)
