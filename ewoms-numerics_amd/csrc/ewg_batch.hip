@@ -979,6 +979,9 @@ int batched_setup(ewg_ensemble *e) {
             }
             w.status = &e->ks[m].status;  // device address of the member's Krylov status
             py[m] = w;
+            py[m].A = c->J;               // the first batch also factorizes
+            py[m].err = e->ilu_err + m;
+            py[m].active = e->act + m;
             py[m].d = e->host[m].p;
             py[m].out = e->host[m].y;
             rz[m] = w;
@@ -1111,18 +1114,12 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
         if (!solve) continue;
         if (int r = upload(act, e->act)) return r;
         hipLaunchKernelGGL(k_b_zero, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ilu_err);
-        for (int l = 0; l < e->lmax; ++l)
-            hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), dim3(BT), 0, s, e->dmem, mp, e->act, l, e->ilu_err);
         if (e->wl_py) {
-            for (int m = 0; m < n; ++m) {
-                if (!act[m]) continue;
-                ewg_ctx *c = e->members[m];
-                if (c->prm.precond_fp32)
-                    wl_pack32(c->g, c->wf, c->L64, c->D64, c->J, c->wl32, s);
-                else
-                    wl_pack64(c->g, c->wf, c->L64, c->D64, c->J, c->wl64, s);
-            }
+            // every member's factorization in one tile-wavefront launch, into its layout
+            wl_batch_factor(e->wl_py, s);
         } else {
+            for (int l = 0; l < e->lmax; ++l)
+                hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), dim3(BT), 0, s, e->dmem, mp, e->act, l, e->ilu_err);
             hipLaunchKernelGGL(k_b_round, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act);
         }
         hipLaunchKernelGGL(k_b_init, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0, e->part1);

@@ -157,6 +157,14 @@ bool use_wavefront_layout(const ewg_ctx *c) {
 // factors, the same for every schedule).
 void precond_factor(ewg_ctx *c) {
     const bool fp32 = c->prm.precond_fp32;
+    if (use_wavefront_layout(c)) {
+        // straight into the sweeps' layout (no natural-order factors)
+        if (fp32)
+            wl_factor32(c->g, c->wf, c->J, c->wl32, c->ilu_err, c->stream);
+        else
+            wl_factor64(c->g, c->wf, c->J, c->wl64, c->ilu_err, c->stream);
+        return;
+    }
     if (use_wavefront(c))
         wf_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->wf, c->stream);
     else
@@ -302,6 +310,11 @@ int ensure_fp32(ewg_ctx *c) {
         HIP_TRY(c, dalloc(&c->wl32.U, 3 * E * 4));
         HIP_TRY(c, dalloc(&c->wl32.D, E * 4));
         HIP_TRY(c, dalloc(&c->wl32.w, E * 2));
+        // the layout's entries without a cell stay zero (the factorization writes cells)
+        HIP_TRY(c, hipMemset(c->wl32.L, 0, 3 * E * 4 * sizeof(float)));
+        HIP_TRY(c, hipMemset(c->wl32.U, 0, 3 * E * 4 * sizeof(float)));
+        HIP_TRY(c, hipMemset(c->wl32.D, 0, E * 4 * sizeof(float)));
+        HIP_TRY(c, hipMemset(c->wl32.w, 0, E * 2 * sizeof(float)));
     }
     return EWG_OK;
 }
@@ -573,6 +586,12 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
             ALLOC(c->wl64.U, 3 * E * 4);
             ALLOC(c->wl64.D, E * 4);
             ALLOC(c->wl64.w, E * 2);
+            // the layout's entries without a cell stay zero (the factorization writes cells)
+            if ((e = hipMemset(c->wl64.L, 0, 3 * E * 4 * sizeof(double))) != hipSuccess ||
+                (e = hipMemset(c->wl64.U, 0, 3 * E * 4 * sizeof(double))) != hipSuccess ||
+                (e = hipMemset(c->wl64.D, 0, E * 4 * sizeof(double))) != hipSuccess ||
+                (e = hipMemset(c->wl64.w, 0, E * 2 * sizeof(double))) != hipSuccess)
+                return fail("wl64", e);
         }
     }
 #undef ALLOC

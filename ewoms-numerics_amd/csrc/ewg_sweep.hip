@@ -16,6 +16,7 @@
 // fused multiply-add), so both schedules agree to the last bit up to the sign of zeros.
 #include <vector>
 
+#include "ewg_cell_ops.h"
 #include "ewg_internal.h"
 #include "ewg_wavefront.h"
 
@@ -827,6 +828,427 @@ k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t 
     }
 }
 
+
+// ----------------------------------------------------------------------------------
+// The factorization straight into the wavefront layout, warp-specialized like the
+// sweeps (natural order, i.e. the forward sweep's schedule):
+//   wave 0 computes Dinv_i = inv(A_ii - sum_j (A_ij Dinv_j) A_ji) over the -z, -y, -x
+//          neighbours j (Dinv_j from its own register, the LDS exchange or the edge
+//          ring), stores L_ij = A_ij Dinv_j in the forward layout (zeros where there is
+//          no cell or no neighbour) and Dinv_i at the cell's backward-layout entry;
+//   wave 1 and wave 2 move the tile edges' Dinv between tiles as in the sweeps (four
+//          64-bit words per value);
+//   wave 3 copies the cell's upper blocks A_i,+x/+y/+z into the backward layout; it
+//          does not depend on the recurrence.
+// Stored as T: FP64, or FP32 rounded to nearest from the FP64 factors. The arithmetic
+// per row is ilu_factor_cell's (ascending columns, no fused multiply-add), so every
+// schedule produces the same factors bit for bit. The backward layout's entries without
+// a cell are never written here (they are zero from allocation on).
+// ----------------------------------------------------------------------------------
+template <class T>
+struct FwsArgs {
+    int nx, ny, nz, nxny, N, diag;
+    int nsteps;            // local steps per tile: 14 + nz
+    const double *A;       // Jacobian, slot-major [2 diag + 1][N][4]
+    T *L;                  // forward layout: 3 arrays (-z, -y, -x) of entries x 4
+    T *U;                  // backward layout: 3 arrays (+x, +y, +z)
+    T *D;                  // backward layout: Dinv
+    size_t entries;
+    int *err;              // singular pivot (|det| < 1e-30)
+    const int *active;     // null, or skip the launch unless *active
+    WfSync sync;
+};
+
+struct B4 {
+    double a[4];
+};
+
+// one exchanged Dinv: 4 coherent 64-bit words
+__device__ __forceinline__ void lde4(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned long long *w) {
+    const u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kCoh);
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, kCoh);
+    w[0] = (unsigned long long)u.x | ((unsigned long long)u.y << 32);
+    w[1] = (unsigned long long)u.z | ((unsigned long long)u.w << 32);
+    w[2] = (unsigned long long)v.x | ((unsigned long long)v.y << 32);
+    w[3] = (unsigned long long)v.z | ((unsigned long long)v.w << 32);
+}
+__device__ __forceinline__ void ste4(__amdgpu_buffer_rsrc_t r, unsigned off, const unsigned long long *w) {
+    u32x4 u, v;
+    u.x = (unsigned)w[0]; u.y = (unsigned)(w[0] >> 32); u.z = (unsigned)w[1]; u.w = (unsigned)(w[1] >> 32);
+    v.x = (unsigned)w[2]; v.y = (unsigned)(w[2] >> 32); v.z = (unsigned)w[3]; v.w = (unsigned)(w[3] >> 32);
+    __builtin_amdgcn_raw_buffer_store_b128(u, r, off, 0, kCoh);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off + 16u, 0, kCoh);
+}
+
+// a 2x2 FP64 block stored as T
+template <class T>
+__device__ __forceinline__ void stb(__amdgpu_buffer_rsrc_t r, unsigned off, const double *v) {
+    if constexpr (sizeof(T) == 8) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_double2(v[0], v[1])), r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_double2(v[2], v[3])), r, off + 16u, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b128(
+            __builtin_bit_cast(u32x4, make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
+                                                  static_cast<float>(v[2]), static_cast<float>(v[3]))),
+            r, off, 0, 0);
+    }
+}
+
+struct FIn {
+    double Aii[4];
+    double Aij[3][4];  // -z, -y, -x
+    double Aji[3][4];  // the neighbour's block towards this cell
+};
+
+template <class T, int PD>
+__device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
+    constexpr int TX = 8, TY = 8, NT = 64;
+    static_assert(PD < kRing, "the edge wave must be able to run ahead of the compute wave");
+    __shared__ B4 xch[2][NT];
+    __shared__ B4 ering[2][8][kRing];
+    __shared__ B4 outbox[2][8][kOut];
+    __shared__ int filled[16];
+    __shared__ int progress;
+    __shared__ int stepdone;
+    __shared__ int pubstep;
+    __shared__ int dummy_i[NT];
+    __shared__ B4 dummy_v[NT];
+    __shared__ unsigned s_tk;
+    if (a.active && *a.active == 0) return;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tk = atomicAdd(a.sync.ticket, 1u);
+    __syncthreads();
+    const unsigned tk = s_tk;
+    if (tk >= (unsigned)a.sync.ntiles) return;
+    const int2 tile = a.sync.order[tk];
+    const int tx = tile.x, ty = tile.y;
+    const int nz = a.nz, nsteps = a.nsteps;
+    const int ntx = a.sync.ntx;
+    const int tileId = ty * ntx + tx;
+    for (int i = tid; i < 2 * 8 * kRing; i += 256) (&ering[0][0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
+    for (int i = tid; i < 2 * NT; i += 256) (&xch[0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
+    if (tid < 16) {
+        const int dir = tid >> 3, e = tid & 7;
+        const bool valid = dir == 0 ? (tx > 0 && ty * TY + e < a.ny) : (ty > 0 && tx * TX + e < a.nx);
+        filled[tid] = valid ? 0 : nz;
+    }
+    if (tid == 0) {
+        progress = 0;
+        stepdone = 0;
+        pubstep = 0;
+    }
+    __syncthreads();
+    constexpr unsigned zw = kWfMaxWords * 8;
+    static_assert(kWfMaxWords == 4, "a Dinv block takes four edge words");
+    const __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
+    const __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
+    const size_t N = (size_t)a.N;
+    constexpr unsigned bB = 4 * sizeof(T);
+    const unsigned blk_bytes = (unsigned)(a.entries * bB);
+    const int wave = tid >> 6, l = tid & 63;
+
+    // the column (lx, ly) of the tile and its entry in the backward layout: the cell of
+    // local step tau sits at backward entry eb0 - 64 tau
+    const int lx = l % TX, ly = l / TX;
+    const int xs = tx * TX + lx, ys = ty * TY + ly;
+    const bool inplane = xs < a.nx && ys < a.ny;
+    const int c0 = xs + a.nx * ys;
+    int eb0 = 0;
+    if (inplane) {
+        const int xb = a.nx - 1 - xs, yb = a.ny - 1 - ys;
+        const int lxb = xb & 7, lyb = yb & 7;
+        const int tb = (yb >> 3) * ntx + (xb >> 3);
+        eb0 = (tb * nsteps + nz - 1 + lx + ly + lxb + lyb) * 64 + lxb + 8 * lyb;
+    }
+
+    if (wave == 3) {
+        // ------------------------------------------------------------ upper-block copy
+        if (!inplane) return;
+        const __amdgpu_buffer_rsrc_t ru0 = rsrc(a.A + (size_t)(a.diag + 1) * N * 4, N * 32),
+                                     ru1 = rsrc(a.A + (size_t)(a.diag + 2) * N * 4, N * 32),
+                                     ru2 = rsrc(a.A + (size_t)(a.diag + 3) * N * 4, N * 32);
+        const __amdgpu_buffer_rsrc_t ro0 = rsrc(a.U, blk_bytes),
+                                     ro1 = rsrc(a.U + a.entries * 4, blk_bytes),
+                                     ro2 = rsrc(a.U + 2 * a.entries * 4, blk_bytes);
+#pragma unroll 4
+        for (int z = 0; z < nz; ++z) {
+            const unsigned oc = (unsigned)(c0 + z * a.nxny) * 32u;
+            const unsigned ob = (unsigned)(eb0 - (z + lx + ly) * 64) * bB;
+            double u0[4], u1[4], u2[4];
+            ldb<double>(ru0, oc, u0);
+            ldb<double>(ru1, oc, u1);
+            ldb<double>(ru2, oc, u2);
+            stb<T>(ro0, ob, u0);
+            stb<T>(ro1, ob, u1);
+            stb<T>(ro2, ob, u2);
+        }
+        return;
+    }
+    if (wave == 2) {
+        // ----------------------------------------------------------- publisher wave
+        const int e = l & 15, dir = e >> 3, idx = e & 7;
+        const bool valid = l < 16 && (dir == 0 ? (tx + 1 < ntx && ty * TY + idx < a.ny)
+                                               : (ty + 1 < a.sync.nty && tx * TX + idx < a.nx));
+        const unsigned gout = !valid ? 0u : (unsigned)((((size_t)tileId * 8 + idx) * nz) * zw);
+        const B4 *box = &outbox[dir][idx][0];
+        int zp = valid ? 0 : nz;
+        unsigned idle = 0;
+        int shown = 0;
+        while (true) {
+            int ps = zp < nz ? zp + 7 + idx : 0x3fffffff;
+            ps = min(ps, __shfl_xor(ps, 1));
+            ps = min(ps, __shfl_xor(ps, 2));
+            ps = min(ps, __shfl_xor(ps, 4));
+            ps = min(ps, __shfl_xor(ps, 8));
+            if (l == 0 && ps > shown) {
+                lds_st(&pubstep, ps);
+                shown = ps;
+            }
+            if (__builtin_amdgcn_ballot_w64(zp < nz) == 0ull) break;
+            const int done = lds_ld(&stepdone);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const bool ready = zp < nz && zp + 7 + idx < done;
+            const B4 v = box[zp & (kOut - 1)];
+            unsigned long long w[4];
+            __builtin_memcpy(w, v.a, sizeof(w));
+            ste4(rex, ready && dir == 0 ? gout + (unsigned)zp * zw : kFar, w);
+            ste4(rey, ready && dir == 1 ? gout + (unsigned)zp * zw : kFar, w);
+            zp += ready ? 1 : 0;
+            if (__builtin_amdgcn_ballot_w64(ready) != 0ull) {
+                idle = 0;
+            } else {
+                if (++idle >= kSpinLimit) {
+                    atomicExch(a.sync.err, 1);
+                    break;
+                }
+                if ((idle & 63) == 0 &&
+                    __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        return;
+    }
+    if (wave == 1) {
+        // ---------------------------------------------------------------- edge wave
+        const int e = l & 15, j = l >> 4;
+        const int dir = e >> 3, idx = e & 7;
+        const bool valid = dir == 0 ? (tx > 0 && ty * TY + idx < a.ny) : (ty > 0 && tx * TX + idx < a.nx);
+        const unsigned gbase =
+            !valid ? 0u
+            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nz) * zw)
+                       : (unsigned)((((size_t)(tileId - ntx) * TX + idx) * nz) * zw);
+        B4 *ring = &ering[dir][idx][0];
+        const unsigned long long sent[4] = {kSentinel, kSentinel, kSentinel, kSentinel};
+        int z = valid ? j : nz;
+        unsigned idle = 0;
+        int published = valid ? 0 : nz;
+        while (__builtin_amdgcn_ballot_w64(z < nz) != 0ull) {
+            const bool mine = z < nz;
+            const bool room = z - kRing + idx < lds_ld(&progress);
+            const unsigned go = mine ? gbase + (unsigned)z * zw : kFar;
+            unsigned long long vx[4], vy[4];
+            lde4(rex, dir == 0 ? go : kFar, vx);
+            lde4(rey, dir == 1 ? go : kFar, vy);
+            unsigned long long v[4];
+            bool got = true;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                v[k] = dir == 0 ? vx[k] : vy[k];
+                got &= v[k] != kSentinel;
+            }
+            const bool deliver = mine && room && got;
+            if (deliver) {
+                B4 b;
+                __builtin_memcpy(b.a, v, sizeof(v));
+                ring[z & (kRing - 1)] = b;
+                ste4(rex, dir == 0 ? go : kFar, sent);
+                ste4(rey, dir == 1 ? go : kFar, sent);
+                z += 4;
+            }
+            int zmin = z < nz ? z : nz;
+            zmin = min(zmin, __shfl_xor(zmin, 16));
+            zmin = min(zmin, __shfl_xor(zmin, 32));
+            const bool advance = j == 0 && zmin > published;
+            if (__builtin_amdgcn_ballot_w64(advance) != 0ull) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(kLgkm0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (advance) lds_st(&filled[e], zmin);
+                published = zmin > published ? zmin : published;
+            }
+            if (__builtin_amdgcn_ballot_w64(deliver) != 0ull) {
+                idle = 0;
+            } else {
+                if (++idle >= kSpinLimit) {
+                    atomicExch(a.sync.err, 1);
+                    break;
+                }
+                if ((idle & 63) == 0 &&
+                    __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        return;
+    }
+
+    // -------------------------------------------------------------------- compute wave
+    const bool prodX = inplane && lx == TX - 1 && tx + 1 < ntx;
+    const bool prodY = inplane && ly == TY - 1 && ty + 1 < a.sync.nty;
+    const __amdgpu_buffer_rsrc_t rAii = rsrc(a.A + (size_t)a.diag * N * 4, N * 32);
+    const __amdgpu_buffer_rsrc_t rAij0 = rsrc(a.A, N * 32), rAij1 = rsrc(a.A + N * 4, N * 32),
+                                 rAij2 = rsrc(a.A + 2 * N * 4, N * 32);
+    // row j's block towards this cell: +z, +y, +x slot of the -z, -y, -x neighbour
+    const __amdgpu_buffer_rsrc_t rAji0 = rsrc(a.A + (size_t)(a.diag + 3) * N * 4, N * 32),
+                                 rAji1 = rsrc(a.A + (size_t)(a.diag + 2) * N * 4, N * 32),
+                                 rAji2 = rsrc(a.A + (size_t)(a.diag + 1) * N * 4, N * 32);
+    const __amdgpu_buffer_rsrc_t rL0 = rsrc(a.L, blk_bytes), rL1 = rsrc(a.L + a.entries * 4, blk_bytes),
+                                 rL2 = rsrc(a.L + 2 * a.entries * 4, blk_bytes);
+    const __amdgpu_buffer_rsrc_t rD = rsrc(a.D, blk_bytes);
+    const unsigned wl0 = (unsigned)((tileId * nsteps) * 64 + l);
+    const bool hy = ys > 0, hx = xs > 0;
+
+    FIn ring[PD];
+    auto load = [&](FIn &in, int tau) {
+        const int z = tau - lx - ly;
+        const bool act = inplane && (unsigned)z < (unsigned)nz;
+        const int c = c0 + z * a.nxny;
+        const unsigned oc = act ? (unsigned)c * 32u : kFar;
+        ldb<double>(rAii, oc, in.Aii);
+        ldb<double>(rAij0, oc, in.Aij[0]);
+        ldb<double>(rAij1, oc, in.Aij[1]);
+        ldb<double>(rAij2, oc, in.Aij[2]);
+        ldb<double>(rAji0, act && z > 0 ? (unsigned)(c - a.nxny) * 32u : kFar, in.Aji[0]);
+        ldb<double>(rAji1, act && hy ? (unsigned)(c - a.nx) * 32u : kFar, in.Aji[1]);
+        ldb<double>(rAji2, act && hx ? (unsigned)(c - 1) * 32u : kFar, in.Aji[2]);
+    };
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
+        load(ring[k], k);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    B4 prev = B4{{0.0, 0.0, 0.0, 0.0}};
+    const bool hasL = lx > 0, hasD = ly > 0;
+    B4 *const oX = prodX ? &outbox[0][ly][0] : &dummy_v[l];
+    B4 *const oY = prodY ? &outbox[1][lx][0] : &dummy_v[l];
+    const int oMaskX = prodX ? kOut - 1 : 0, oMaskY = prodY ? kOut - 1 : 0;
+    int *const sdone = l == 0 ? &stepdone : &dummy_i[l];
+    const B4 *const eX = &ering[0][ly][0];
+    const B4 *const eY = &ering[1][lx][0];
+    const int *const fX = &filled[ly];
+    const int *const fY = &filled[8 + lx];
+    const bool edgeX = lx == 0, edgeY = ly == 0;
+    auto nbr = [&](int tau, B4 &vx, B4 &vy) {
+        const int z = tau - lx - ly;
+        const int rbuf = (tau + 1) & 1;
+        const int slot = z & (kRing - 1);
+        vx = hasL ? xch[rbuf][l - 1] : eX[slot];
+        vy = hasD ? xch[rbuf][l - TX] : eY[slot];
+    };
+    int cntX = lds_ld(fX), cntY = lds_ld(fY);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    B4 nx_, ny_;
+    nbr(0, nx_, ny_);
+    for (int tau0 = 0; tau0 < nsteps; tau0 += PD) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (l == 0) lds_st(&progress, tau0);
+        {
+            bool full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
+            if (__builtin_amdgcn_ballot_w64(full) != 0ull) {
+                unsigned spins = 0;
+                do {
+                    __builtin_amdgcn_s_sleep(1);
+                    full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
+                } while (__builtin_amdgcn_ballot_w64(full) != 0ull && ++spins < kSpinLimit);
+                if (full) atomicExch(a.sync.err, 1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+#pragma unroll
+        for (int k = 0; k < PD; ++k) {
+            const int tau = tau0 + k;
+            const int z = tau - lx - ly;
+            const bool zin = (unsigned)z < (unsigned)nz;
+            const bool active = inplane && zin && tau < nsteps;
+            const int wbuf = tau & 1;
+            bool wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
+            if (__builtin_amdgcn_ballot_w64(wait) != 0ull) {
+                unsigned spins = 0;
+                do {
+                    __builtin_amdgcn_s_sleep(1);
+                    cntX = lds_ld(fX);
+                    cntY = lds_ld(fY);
+                    wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
+                    ++spins;
+                    if ((spins & 63) == 0 &&
+                        __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                        break;
+                } while (__builtin_amdgcn_ballot_w64(wait) != 0ull && spins < kSpinLimit);
+                if (wait) atomicExch(a.sync.err, 1);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                nbr(tau, nx_, ny_);
+            }
+            const FIn &c = ring[k];
+            const bool has[3] = {z > 0, hy, hx};
+            const double *nb[3] = {prev.a, ny_.a, nx_.a};
+            double D[4] = {c.Aii[0], c.Aii[1], c.Aii[2], c.Aii[3]};
+            double Lv[3][4];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                Lv[s][0] = Lv[s][1] = Lv[s][2] = Lv[s][3] = 0.0;
+                if (has[s]) {
+                    double Bm[4];
+                    cell::bmul(c.Aij[s], nb[s], Lv[s]);
+                    cell::bmul(Lv[s], c.Aji[s], Bm);
+                    for (int e = 0; e < 4; ++e) D[e] -= Bm[e];
+                }
+                if (!active) Lv[s][0] = Lv[s][1] = Lv[s][2] = Lv[s][3] = 0.0;
+            }
+            double det = D[0] * D[3] - D[1] * D[2];
+            const bool sing = !(fabs(det) >= 1e-30);
+            if (active && sing) atomicExch(a.err, 1);
+            if (sing) det = 1.0;
+            const double det_1 = 1.0 / det;
+            B4 res;
+            res.a[0] = D[3] * det_1;
+            res.a[1] = -D[1] * det_1;
+            res.a[2] = -D[2] * det_1;
+            res.a[3] = D[0] * det_1;
+            oX[z & oMaskX] = res;
+            oY[z & oMaskY] = res;
+            const unsigned oL = tau < nsteps ? (wl0 + (unsigned)tau * 64u) * bB : kFar;
+            stb<T>(rL0, oL, Lv[0]);
+            stb<T>(rL1, oL, Lv[1]);
+            stb<T>(rL2, oL, Lv[2]);
+            stb<T>(rD, active ? (unsigned)(eb0 - tau * 64) * bB : kFar, res.a);
+            prev = res;
+            xch[wbuf][l] = res;
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            lds_st(sdone, tau + 1);
+            __builtin_amdgcn_wave_barrier();
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            cntX = lds_ld(fX);
+            cntY = lds_ld(fY);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            nbr(tau + 1, nx_, ny_);
+            load(ring[k], tau + PD);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+template <class T, int PD>
+__global__ void __launch_bounds__(256) k_factor_ws(FwsArgs<T> a) {
+    factor_ws_body<T, PD>(a);
+}
+
+template <class T, int PD>
+__global__ void __launch_bounds__(256) k_factor_ws_batch(const FwsArgs<T> *args, const int *tmem) {
+    const FwsArgs<T> a = args[tmem[blockIdx.x]];
+    factor_ws_body<T, PD>(a);
+}
 }  // namespace
 
 template <class T, bool FWD>
@@ -940,10 +1362,39 @@ void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, c
     wl_apply<float>(g, sync, wl, d, v, status, s);
 }
 
+template <class T>
+static FwsArgs<T> fws_args(const Grid &g, const WfSync &sync, const double *A, T *L, T *U, T *D,
+                           int *err) {
+    FwsArgs<T> f{};
+    f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N; f.diag = g.diag;
+    f.nsteps = g.nz + 14;
+    f.A = A; f.L = L; f.U = U; f.D = D;
+    f.entries = wl_entries(g, sync);
+    f.err = err;
+    f.active = nullptr;
+    f.sync = sync;
+    return f;
+}
+
+void wl_factor64(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<double> &wl,
+                 int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
+    hipLaunchKernelGGL((k_factor_ws<double, kFactorPD>), dim3(sync.ntiles), dim3(256), 0, s,
+                       fws_args<double>(g, sync, A, wl.L, wl.U, wl.D, err));
+}
+
+void wl_factor32(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<float> &wl,
+                 int *err, hipStream_t s) {
+    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
+    hipLaunchKernelGGL((k_factor_ws<float, kFactorPD>), dim3(sync.ntiles), dim3(256), 0, s,
+                       fws_args<float>(g, sync, A, wl.L, wl.U, wl.D, err));
+}
+
 struct WlBatch {
     int n = 0, ntiles = 0;
     bool fp32 = false;
     void *fwd = nullptr, *bwd = nullptr;  // WsArgs<T>[n]
+    void *fac = nullptr;                  // FwsArgs<T>[n] (members given a Jacobian)
     int *tmem = nullptr;                  // block -> realization
     unsigned **tickets = nullptr;         // [n]
 };
@@ -951,6 +1402,8 @@ struct WlBatch {
 template <class T>
 static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t s) {
     std::vector<WsArgs<T>> f(m.size()), bw(m.size());
+    std::vector<FwsArgs<T>> fa(m.size());
+    const bool factor = !m.empty() && m[0].A != nullptr;
     std::vector<int> tmem;
     std::vector<unsigned *> tk(m.size());
     for (size_t i = 0; i < m.size(); ++i) {
@@ -973,6 +1426,13 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
         c.Dinv = static_cast<const T *>(m[i].D);
         c.out = m[i].out;
         bw[i] = c;
+        if (factor) {
+            WfSync fs = m[i].sync;
+            fs.err = m[i].err;
+            fa[i] = fws_args<T>(g, fs, m[i].A, const_cast<T *>(L), const_cast<T *>(U),
+                                const_cast<T *>(static_cast<const T *>(m[i].D)), m[i].err);
+            fa[i].active = m[i].active;
+        }
         tmem.insert(tmem.end(), m[i].sync.ntiles, (int)i);
         tk[i] = m[i].sync.ticket;
     }
@@ -982,6 +1442,12 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
         hipMalloc(reinterpret_cast<void **>(&b.tmem), tmem.size() * sizeof(int)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&b.tickets), m.size() * sizeof(unsigned *)) != hipSuccess)
         return -1;
+    if (factor) {
+        const size_t fb = m.size() * sizeof(FwsArgs<T>);
+        if (hipMalloc(&b.fac, fb) != hipSuccess ||
+            hipMemcpyAsync(b.fac, fa.data(), fb, hipMemcpyHostToDevice, s) != hipSuccess)
+            return -1;
+    }
     if (hipMemcpyAsync(b.fwd, f.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b.bwd, bw.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b.tmem, tmem.data(), tmem.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1022,10 +1488,22 @@ void wl_batch_apply(void *h, hipStream_t s) {
                            static_cast<const WsArgs<double> *>(b->bwd), b->tmem);
 }
 
+void wl_batch_factor(void *h, hipStream_t s) {
+    auto *b = static_cast<WlBatch *>(h);
+    const dim3 rg((b->n + 255) / 256), rb(256);
+    hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
+    if (b->fp32)
+        hipLaunchKernelGGL((k_factor_ws_batch<float, kFactorPD>), dim3(b->ntiles), dim3(256), 0, s,
+                           static_cast<const FwsArgs<float> *>(b->fac), b->tmem);
+    else
+        hipLaunchKernelGGL((k_factor_ws_batch<double, kFactorPD>), dim3(b->ntiles), dim3(256), 0, s,
+                           static_cast<const FwsArgs<double> *>(b->fac), b->tmem);
+}
+
 void wl_batch_free(void *h) {
     auto *b = static_cast<WlBatch *>(h);
     if (!b) return;
-    void *p[] = {b->fwd, b->bwd, b->tmem, b->tickets};
+    void *p[] = {b->fwd, b->bwd, b->fac, b->tmem, b->tickets};
     for (void *q : p)
         if (q) (void)hipFree(q);
     delete b;

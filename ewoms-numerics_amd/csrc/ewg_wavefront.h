@@ -13,6 +13,7 @@ constexpr int kWfTX = 8;   // tile extent in x (lanes)
 constexpr int kWfTY = 8;   // tile extent in y
 constexpr int kWfPD = 8;   // prefetch depth (local steps) of each lane's operands
 constexpr int kWfPDFactor = 4;  // factorization operands are twice as wide
+constexpr int kFactorPD = 3;    // prefetch depth of the wavefront-layout factorization
 constexpr int kSweepPD = 6;     // prefetch depth of the triangular sweeps' operands
 constexpr int kSweepPE = 1;     // prefetch depth of their cross-tile words
 
@@ -73,6 +74,12 @@ void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double 
 void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
                const double *A, const WlPrecond<float> &wl, hipStream_t s);
 // v = (LU)^-1 d with one compute, one edge and one publisher wavefront per tile
+// the factorization straight into the layout (FP64 arithmetic; FP32 rounds the stored
+// factors and upper blocks); entries without a cell keep what they hold (zeros)
+void wl_factor64(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<double> &wl,
+                 int *err, hipStream_t s);
+void wl_factor32(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<float> &wl,
+                 int *err, hipStream_t s);
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
                 double *v, const int *status, hipStream_t s);
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
@@ -90,9 +97,13 @@ struct WlMember {
     const double *d;
     double *out;
     const int *status;
+    const double *A = nullptr;     // Jacobian: the batch also factorizes (wl_batch_factor)
+    int *err = nullptr;            // the realization's factorization error flag
+    const int *active = nullptr;   // factorize only when *active
 };
 void *wl_batch_create(const std::vector<WlMember> &members, bool fp32, hipStream_t s);
 void wl_batch_apply(void *batch, hipStream_t s);
+void wl_batch_factor(void *batch, hipStream_t s);
 void wl_batch_free(void *batch);
 
 }  // namespace ewg
