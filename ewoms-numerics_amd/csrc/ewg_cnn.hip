@@ -40,6 +40,7 @@ using u32x2 = __attribute__((ext_vector_type(2))) unsigned;
 constexpr unsigned kFar = 0x80000000u;  // offset that reads as zero (out of range)
 constexpr int kTapsPad = 28;
 constexpr int kWgParts = 128;           // cell partitions of the weight gradient
+constexpr int kConvPF = 4;              // convolution operand prefetch distance (K steps)
 
 struct Layer {
     int ci, co, cip, cipb;  // real in/out channels, K channels forward / backward data
@@ -89,6 +90,18 @@ struct ConvArgs {
     float *out_f;                // FWD_OUT: [ncells][2]; BWD_DATA: [ncells][32]
 };
 
+// tap t = 9 (dz + 1) + 3 (dy + 1) + (dx + 1); tap 27 is padding (never valid)
+__host__ __device__ constexpr int tap_dx(int t) { return t % 3 - 1; }
+__host__ __device__ constexpr int tap_dy(int t) { return (t / 3) % 3 - 1; }
+__host__ __device__ constexpr int tap_dz(int t) { return t / 9 - 1; }
+// neighbour bits a cell needs for tap t (bit 6: the cell exists; padding: unreachable)
+__host__ __device__ constexpr unsigned tap_mask(int t) {
+    return t >= 27 ? 0x80u
+                   : (tap_dx(t) < 0 ? 1u : tap_dx(t) > 0 ? 2u : 0u) |
+                         (tap_dy(t) < 0 ? 4u : tap_dy(t) > 0 ? 8u : 0u) |
+                         (tap_dz(t) < 0 ? 16u : tap_dz(t) > 0 ? 32u : 0u) | 64u;
+}
+
 // One wavefront computes 64 cells (two 32-cell column blocks) x 32 output rows; the
 // workgroup's four wavefronts share the layer's weights in LDS and stride over the grid.
 template <int CIP, int MODE>
@@ -113,9 +126,15 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
         const int co = 8 * (it >> 2) + 4 * g + (it & 3);
         bias[it] = a.bias ? a.bias[co] : 0.0f;
     }
+    // byte offset of each tap's neighbour relative to the cell (uniform over the wave)
+    const int sx = CIP * 2, sy = nx * CIP * 2, sz = nxny * CIP * 2;
+    // this lane's channel half (CIP = 32) or tap parity (CIP = 8), in bytes / elements
+    const unsigned lane_b = CIP == 32 ? 16u * g : 0u;
+    const int lane_w = CIP == 32 ? j * CIP + 8 * g : j * CIP + 256 * g;
     for (int tile = blockIdx.x * 4 + wave; tile < ntile; tile += gridDim.x * 4) {
         int cell[2];
-        unsigned nbm[2];  // bit d: neighbour in direction d (-x,+x,-y,+y,-z,+z) exists
+        unsigned nbm[2];   // bit d: neighbour in direction d (-x,+x,-y,+y,-z,+z) exists
+        unsigned base[2];  // byte offset of the lane's first channel at the cell
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             cell[t] = tile * 64 + t * 32 + j;
@@ -124,37 +143,42 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
             nbm[t] = in ? ((x > 0) | (x + 1 < nx) << 1 | (y > 0) << 2 | (y + 1 < ny) << 3 |
                            (z > 0) << 4 | (z + 1 < nz) << 5 | 1u << 6)
                         : 0u;
+            base[t] = (unsigned)(cell[t] * CIP) * 2u + lane_b;
         }
-        // operand addresses of step s for this lane: tap, first channel, validity
+        // operand offset of step s (compile-time after unrolling): K = 16 s + 8 g runs over
+        // (tap, channel); CIP = 32: tap s / 2 for both halves; CIP = 8: tap 2 s + g
         auto bofs = [&](int s, int t) -> unsigned {
-            const int kk = 16 * s + 8 * g;
-            const int tap = kk / CIP, ci0 = kk % CIP;
-            const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-            const unsigned m = nbm[t];
-            bool ok = (m >> 6) & 1u;
-            ok &= tap < 27;
-            ok &= dx < 0 ? (m & 1u) : dx > 0 ? ((m >> 1) & 1u) : 1u;
-            ok &= dy < 0 ? ((m >> 2) & 1u) : dy > 0 ? ((m >> 3) & 1u) : 1u;
-            ok &= dz < 0 ? ((m >> 4) & 1u) : dz > 0 ? ((m >> 5) & 1u) : 1u;
-            const int nb = cell[t] + dx + dy * nx + dz * nxny;
-            return ok ? (unsigned)(nb * CIP + ci0) * 2u : kFar;
-        };
-        auto wofs = [&](int s) -> int {
-            const int kk = 16 * s + 8 * g;
-            const int tap = kk / CIP, ci0 = kk % CIP;
-            return (tap * 32 + j) * CIP + ci0;
+            const int ta = CIP == 32 ? s / 2 : 2 * s, tb = CIP == 32 ? s / 2 : 2 * s + 1;
+            const unsigned ma = tap_mask(ta), mb = tap_mask(tb);
+            const int oa = tap_dx(ta) * sx + tap_dy(ta) * sy + tap_dz(ta) * sz;
+            const int ob = tap_dx(tb) * sx + tap_dy(tb) * sy + tap_dz(tb) * sz;
+            const unsigned m = CIP == 32 || g == 0 ? ma : mb;
+            const int o = CIP == 32 || g == 0 ? oa : ob;
+            const unsigned ch = CIP == 32 ? (unsigned)(s & 1) * 32u : 0u;
+            return (nbm[t] & m) == m ? base[t] + (unsigned)o + ch : kFar;
         };
         f32x16 acc0 = {}, acc1 = {};
-        bf16x8 b0 = ld8(rin, bofs(0, 0)), b1 = ld8(rin, bofs(0, 1));
+        // the operands of the next kConvPF steps are in flight while a step's products run
+        bf16x8 r0[kConvPF], r1[kConvPF];
+#pragma unroll
+        for (int k = 0; k < kConvPF; ++k) {
+            r0[k] = ld8(rin, bofs(k < NSTEP ? k : NSTEP - 1, 0));
+            r1[k] = ld8(rin, bofs(k < NSTEP ? k : NSTEP - 1, 1));
+        }
+#pragma unroll
         for (int s = 0; s < NSTEP; ++s) {
-            // next step's operands in flight while this step's products run
-            const int sn = s + 1 < NSTEP ? s + 1 : s;
-            const bf16x8 n0 = ld8(rin, bofs(sn, 0)), n1 = ld8(rin, bofs(sn, 1));
-            const bf16x8 wa = *reinterpret_cast<const bf16x8 *>(&wl[wofs(s)]);
+            const bf16x8 b0 = r0[s % kConvPF], b1 = r1[s % kConvPF];
+            const int sn = s + kConvPF;
+            if (sn < NSTEP) {
+                r0[s % kConvPF] = ld8(rin, bofs(sn, 0));
+                r1[s % kConvPF] = ld8(rin, bofs(sn, 1));
+            }
+            const int ws = CIP == 32 ? (s / 2) * 32 * CIP + (s & 1) * 16 : s * 512;
+            const bf16x8 wa = *reinterpret_cast<const bf16x8 *>(&wl[lane_w + ws]);
             acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, b0, acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, b1, acc1, 0, 0, 0);
-            b0 = n0;
-            b1 = n1;
+            // keep the prefetch distance as written (unbounded hoisting costs occupancy)
+            __builtin_amdgcn_sched_barrier(0);
         }
         // epilogue: lane holds cell j of each block, rows 8q + 4g + r in element 4q + r
 #pragma unroll
