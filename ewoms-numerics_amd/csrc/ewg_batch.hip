@@ -980,6 +980,7 @@ int batched_setup(ewg_ensemble *e) {
             w.status = &e->ks[m].status;  // device address of the member's Krylov status
             py[m] = w;
             py[m].A = c->J;               // the first batch also factorizes
+            py[m].Af = c->wl_af;
             py[m].err = e->ilu_err + m;
             py[m].active = e->act + m;
             py[m].d = e->host[m].p;

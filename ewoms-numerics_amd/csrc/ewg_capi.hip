@@ -50,7 +50,8 @@ void free_all(ewg_ctx *c) {
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
                     c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks, c->partAny,
                     c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wl64.L, c->wl64.U,
-                    c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w};
+                    c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w,
+                    c->wl_af};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->copy_dst) (void)hipFree(c->copy_dst);
@@ -160,9 +161,9 @@ void precond_factor(ewg_ctx *c) {
     if (use_wavefront_layout(c)) {
         // straight into the sweeps' layout (no natural-order factors)
         if (fp32)
-            wl_factor32(c->g, c->wf, c->J, c->wl32, c->ilu_err, c->stream);
+            wl_factor32(c->g, c->wf, c->J, c->wl_af, c->wl32, c->ilu_err, c->stream);
         else
-            wl_factor64(c->g, c->wf, c->J, c->wl64, c->ilu_err, c->stream);
+            wl_factor64(c->g, c->wf, c->J, c->wl_af, c->wl64, c->ilu_err, c->stream);
         return;
     }
     if (use_wavefront(c))
@@ -586,6 +587,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
             ALLOC(c->wl64.U, 3 * E * 4);
             ALLOC(c->wl64.D, E * 4);
             ALLOC(c->wl64.w, E * 2);
+            ALLOC(c->wl_af, 7 * E * 4);
             // the layout's entries without a cell stay zero (the factorization writes cells)
             if ((e = hipMemset(c->wl64.L, 0, 3 * E * 4 * sizeof(double))) != hipSuccess ||
                 (e = hipMemset(c->wl64.U, 0, 3 * E * 4 * sizeof(double))) != hipSuccess ||

@@ -57,6 +57,7 @@ struct ewg_ctx {
     // ILU(0) factors in the tiles' wavefront layout (schedule 0)
     WlPrecond<double> wl64{};
     WlPrecond<float> wl32{};
+    double *wl_af = nullptr;  // the factorization's operands in the forward layout
     // event-timed samples of the hot launches (params.kernel_events), by EWG_KERNEL_* id
     struct Sample {
         hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -837,9 +837,12 @@ k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t 
 //          ring), stores L_ij = A_ij Dinv_j in the forward layout (zeros where there is
 //          no cell or no neighbour) and Dinv_i at the cell's backward-layout entry;
 //   wave 1 and wave 2 move the tile edges' Dinv between tiles as in the sweeps (four
-//          64-bit words per value);
-//   wave 3 copies the cell's upper blocks A_i,+x/+y/+z into the backward layout; it
-//          does not depend on the recurrence.
+//          64-bit words per value).
+// The compute wave reads its operands from a copy of the Jacobian's blocks in the
+// forward layout (k_wl_pack_factor, a bandwidth-bound pass before the launch), so that
+// its loads are contiguous like the sweeps' (the natural-order blocks of a skewed tile
+// column touch a separate cache line per lane and load). The same pass copies the upper
+// blocks A_i,+x/+y/+z into the backward layout.
 // Stored as T: FP64, or FP32 rounded to nearest from the FP64 factors. The arithmetic
 // per row is ilu_factor_cell's (ascending columns, no fused multiply-add), so every
 // schedule produces the same factors bit for bit. The backward layout's entries without
@@ -850,6 +853,8 @@ struct FwsArgs {
     int nx, ny, nz, nxny, N, diag;
     int nsteps;            // local steps per tile: 14 + nz
     const double *A;       // Jacobian, slot-major [2 diag + 1][N][4]
+    double *Af;            // forward layout, 7 arrays of entries x 4: A_ii, A_ij (-z, -y,
+                           // -x), A_ji (the -z, -y, -x neighbour's block towards the cell)
     T *L;                  // forward layout: 3 arrays (-z, -y, -x) of entries x 4
     T *U;                  // backward layout: 3 arrays (+x, +y, +z)
     T *D;                  // backward layout: Dinv
@@ -925,8 +930,8 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     const int nz = a.nz, nsteps = a.nsteps;
     const int ntx = a.sync.ntx;
     const int tileId = ty * ntx + tx;
-    for (int i = tid; i < 2 * 8 * kRing; i += 256) (&ering[0][0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
-    for (int i = tid; i < 2 * NT; i += 256) (&xch[0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
+    for (int i = tid; i < 2 * 8 * kRing; i += 192) (&ering[0][0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
+    for (int i = tid; i < 2 * NT; i += 192) (&xch[0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
     if (tid < 16) {
         const int dir = tid >> 3, e = tid & 7;
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + e < a.ny) : (ty > 0 && tx * TX + e < a.nx);
@@ -942,7 +947,6 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     static_assert(kWfMaxWords == 4, "a Dinv block takes four edge words");
     const __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
     const __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
-    const size_t N = (size_t)a.N;
     constexpr unsigned bB = 4 * sizeof(T);
     const unsigned blk_bytes = (unsigned)(a.entries * bB);
     const int wave = tid >> 6, l = tid & 63;
@@ -952,7 +956,6 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     const int lx = l % TX, ly = l / TX;
     const int xs = tx * TX + lx, ys = ty * TY + ly;
     const bool inplane = xs < a.nx && ys < a.ny;
-    const int c0 = xs + a.nx * ys;
     int eb0 = 0;
     if (inplane) {
         const int xb = a.nx - 1 - xs, yb = a.ny - 1 - ys;
@@ -961,29 +964,6 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
         eb0 = (tb * nsteps + nz - 1 + lx + ly + lxb + lyb) * 64 + lxb + 8 * lyb;
     }
 
-    if (wave == 3) {
-        // ------------------------------------------------------------ upper-block copy
-        if (!inplane) return;
-        const __amdgpu_buffer_rsrc_t ru0 = rsrc(a.A + (size_t)(a.diag + 1) * N * 4, N * 32),
-                                     ru1 = rsrc(a.A + (size_t)(a.diag + 2) * N * 4, N * 32),
-                                     ru2 = rsrc(a.A + (size_t)(a.diag + 3) * N * 4, N * 32);
-        const __amdgpu_buffer_rsrc_t ro0 = rsrc(a.U, blk_bytes),
-                                     ro1 = rsrc(a.U + a.entries * 4, blk_bytes),
-                                     ro2 = rsrc(a.U + 2 * a.entries * 4, blk_bytes);
-#pragma unroll 4
-        for (int z = 0; z < nz; ++z) {
-            const unsigned oc = (unsigned)(c0 + z * a.nxny) * 32u;
-            const unsigned ob = (unsigned)(eb0 - (z + lx + ly) * 64) * bB;
-            double u0[4], u1[4], u2[4];
-            ldb<double>(ru0, oc, u0);
-            ldb<double>(ru1, oc, u1);
-            ldb<double>(ru2, oc, u2);
-            stb<T>(ro0, ob, u0);
-            stb<T>(ro1, ob, u1);
-            stb<T>(ro2, ob, u2);
-        }
-        return;
-    }
     if (wave == 2) {
         // ----------------------------------------------------------- publisher wave
         const int e = l & 15, dir = e >> 3, idx = e & 7;
@@ -1096,32 +1076,31 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     // -------------------------------------------------------------------- compute wave
     const bool prodX = inplane && lx == TX - 1 && tx + 1 < ntx;
     const bool prodY = inplane && ly == TY - 1 && ty + 1 < a.sync.nty;
-    const __amdgpu_buffer_rsrc_t rAii = rsrc(a.A + (size_t)a.diag * N * 4, N * 32);
-    const __amdgpu_buffer_rsrc_t rAij0 = rsrc(a.A, N * 32), rAij1 = rsrc(a.A + N * 4, N * 32),
-                                 rAij2 = rsrc(a.A + 2 * N * 4, N * 32);
-    // row j's block towards this cell: +z, +y, +x slot of the -z, -y, -x neighbour
-    const __amdgpu_buffer_rsrc_t rAji0 = rsrc(a.A + (size_t)(a.diag + 3) * N * 4, N * 32),
-                                 rAji1 = rsrc(a.A + (size_t)(a.diag + 2) * N * 4, N * 32),
-                                 rAji2 = rsrc(a.A + (size_t)(a.diag + 1) * N * 4, N * 32);
+    const size_t E4 = a.entries * 4;
+    const unsigned af_bytes = (unsigned)(a.entries * 32);
+    const __amdgpu_buffer_rsrc_t rAii = rsrc(a.Af, af_bytes);
+    const __amdgpu_buffer_rsrc_t rAij0 = rsrc(a.Af + E4, af_bytes), rAij1 = rsrc(a.Af + 2 * E4, af_bytes),
+                                 rAij2 = rsrc(a.Af + 3 * E4, af_bytes);
+    const __amdgpu_buffer_rsrc_t rAji0 = rsrc(a.Af + 4 * E4, af_bytes),
+                                 rAji1 = rsrc(a.Af + 5 * E4, af_bytes),
+                                 rAji2 = rsrc(a.Af + 6 * E4, af_bytes);
     const __amdgpu_buffer_rsrc_t rL0 = rsrc(a.L, blk_bytes), rL1 = rsrc(a.L + a.entries * 4, blk_bytes),
                                  rL2 = rsrc(a.L + 2 * a.entries * 4, blk_bytes);
     const __amdgpu_buffer_rsrc_t rD = rsrc(a.D, blk_bytes);
     const unsigned wl0 = (unsigned)((tileId * nsteps) * 64 + l);
-    const bool hy = ys > 0, hx = xs > 0;
 
     FIn ring[PD];
+    // operands of local step tau: this lane's forward-layout entry of that step (steps
+    // past the tile's last read out of range)
     auto load = [&](FIn &in, int tau) {
-        const int z = tau - lx - ly;
-        const bool act = inplane && (unsigned)z < (unsigned)nz;
-        const int c = c0 + z * a.nxny;
-        const unsigned oc = act ? (unsigned)c * 32u : kFar;
-        ldb<double>(rAii, oc, in.Aii);
-        ldb<double>(rAij0, oc, in.Aij[0]);
-        ldb<double>(rAij1, oc, in.Aij[1]);
-        ldb<double>(rAij2, oc, in.Aij[2]);
-        ldb<double>(rAji0, act && z > 0 ? (unsigned)(c - a.nxny) * 32u : kFar, in.Aji[0]);
-        ldb<double>(rAji1, act && hy ? (unsigned)(c - a.nx) * 32u : kFar, in.Aji[1]);
-        ldb<double>(rAji2, act && hx ? (unsigned)(c - 1) * 32u : kFar, in.Aji[2]);
+        const unsigned o = tau < nsteps ? (wl0 + (unsigned)tau * 64u) * 32u : kFar;
+        ldb<double>(rAii, o, in.Aii);
+        ldb<double>(rAij0, o, in.Aij[0]);
+        ldb<double>(rAij1, o, in.Aij[1]);
+        ldb<double>(rAij2, o, in.Aij[2]);
+        ldb<double>(rAji0, o, in.Aji[0]);
+        ldb<double>(rAji1, o, in.Aji[1]);
+        ldb<double>(rAji2, o, in.Aji[2]);
     };
 #pragma unroll
     for (int k = 0; k < PD; ++k) {
@@ -1191,20 +1170,20 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
                 nbr(tau, nx_, ny_);
             }
             const FIn &c = ring[k];
-            const bool has[3] = {z > 0, hy, hx};
             const double *nb[3] = {prev.a, ny_.a, nx_.a};
+            // Branch-free: an absent neighbour's blocks are zero in the packed operands
+            // (and so are all blocks of an entry without a cell), and every neighbour
+            // value is finite, so its L_ij is bmul's +0 and D - (+0) is D (also for D =
+            // -0): the same bits as skipping the neighbour.
             double D[4] = {c.Aii[0], c.Aii[1], c.Aii[2], c.Aii[3]};
             double Lv[3][4];
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
-                Lv[s][0] = Lv[s][1] = Lv[s][2] = Lv[s][3] = 0.0;
-                if (has[s]) {
-                    double Bm[4];
-                    cell::bmul(c.Aij[s], nb[s], Lv[s]);
-                    cell::bmul(Lv[s], c.Aji[s], Bm);
-                    for (int e = 0; e < 4; ++e) D[e] -= Bm[e];
-                }
-                if (!active) Lv[s][0] = Lv[s][1] = Lv[s][2] = Lv[s][3] = 0.0;
+                double Bm[4];
+                cell::bmul(c.Aij[s], nb[s], Lv[s]);
+                cell::bmul(Lv[s], c.Aji[s], Bm);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) D[e] -= Bm[e];
             }
             double det = D[0] * D[3] - D[1] * D[2];
             const bool sing = !(fabs(det) >= 1e-30);
@@ -1240,14 +1219,72 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
 }
 
 template <class T, int PD>
-__global__ void __launch_bounds__(256) k_factor_ws(FwsArgs<T> a) {
+__global__ void __launch_bounds__(192) k_factor_ws(FwsArgs<T> a) {
     factor_ws_body<T, PD>(a);
 }
 
 template <class T, int PD>
-__global__ void __launch_bounds__(256) k_factor_ws_batch(const FwsArgs<T> *args, const int *tmem) {
+__global__ void __launch_bounds__(192) k_factor_ws_batch(const FwsArgs<T> *args, const int *tmem) {
     const FwsArgs<T> a = args[tmem[blockIdx.x]];
     factor_ws_body<T, PD>(a);
+}
+
+// The factorization's operands in the forward layout (entry p: the cell of that tile,
+// local step and lane; zeros where there is no cell or no neighbour) and the upper
+// blocks in the backward layout (entry p of the reflected grid), one thread per entry.
+template <class T>
+__device__ __forceinline__ void pack_factor_entry(const FwsArgs<T> &a, unsigned p) {
+    if (p >= a.entries) return;
+    const int nsteps = a.nsteps, ntx = a.sync.ntx;
+    const int lane = p & 63, q = p >> 6;
+    const int step = q % nsteps, tileId = q / nsteps;
+    const int lx = lane & 7, ly = lane >> 3;
+    const int x = (tileId % ntx) * 8 + lx, y = (tileId / ntx) * 8 + ly, z = step - lx - ly;
+    const bool ok = x < a.nx && y < a.ny && z >= 0 && z < a.nz;
+    const size_t N = (size_t)a.N, E4 = a.entries * 4;
+    const double *A = a.A;
+    auto blk = [&](bool has, size_t slot, long long cell, double *o) {
+        for (int e = 0; e < 4; ++e) o[e] = has ? A[(slot * N + (size_t)cell) * 4 + e] : 0.0;
+    };
+    auto put = [&](double *dst, const double *v) {
+        *reinterpret_cast<double2 *>(dst + (size_t)p * 4) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(dst + (size_t)p * 4 + 2) = make_double2(v[2], v[3]);
+    };
+    const long long c = ok ? x + (long long)a.nx * (y + (long long)a.ny * z) : 0;
+    const long long str[3] = {(long long)a.nxny, (long long)a.nx, 1};
+    const bool has[3] = {z > 0, y > 0, x > 0};
+    double v[4];
+    blk(ok, a.diag, c, v);
+    put(a.Af, v);
+    for (int s = 0; s < 3; ++s) {
+        blk(ok, s, c, v);
+        put(a.Af + (1 + s) * E4, v);
+        // the lower neighbour's block towards this cell: its +z, +y, +x slot
+        blk(ok && has[s], a.diag + 3 - s, c - str[s], v);
+        put(a.Af + (4 + s) * E4, v);
+    }
+    // backward layout: the reflected cell's upper blocks (+x, +y, +z), stored as T
+    const long long cb = ok ? (a.nx - 1 - x) + (long long)a.nx * ((a.ny - 1 - y) + (long long)a.ny * (a.nz - 1 - z)) : 0;
+    for (int s = 0; s < 3; ++s) {
+        blk(ok, a.diag + 1 + s, cb, v);
+        T *dst = a.U + s * E4 + (size_t)p * 4;
+        for (int e = 0; e < 4; ++e) dst[e] = static_cast<T>(v[e]);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_wl_pack_factor(FwsArgs<T> a) {
+    pack_factor_entry<T>(a, blockIdx.x * 256u + threadIdx.x);
+}
+
+// several realizations: block -> realization by pmem, its first block at poff
+template <class T>
+__global__ void __launch_bounds__(256)
+k_wl_pack_factor_batch(const FwsArgs<T> *args, const int *pmem, const int *poff) {
+    const int m = pmem[blockIdx.x];
+    const FwsArgs<T> &a = args[m];
+    if (a.active && *a.active == 0) return;
+    pack_factor_entry<T>(a, (blockIdx.x - poff[m]) * 256u + threadIdx.x);
 }
 }  // namespace
 
@@ -1363,12 +1400,12 @@ void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, c
 }
 
 template <class T>
-static FwsArgs<T> fws_args(const Grid &g, const WfSync &sync, const double *A, T *L, T *U, T *D,
-                           int *err) {
+static FwsArgs<T> fws_args(const Grid &g, const WfSync &sync, const double *A, double *Af, T *L,
+                           T *U, T *D, int *err) {
     FwsArgs<T> f{};
     f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N; f.diag = g.diag;
     f.nsteps = g.nz + 14;
-    f.A = A; f.L = L; f.U = U; f.D = D;
+    f.A = A; f.Af = Af; f.L = L; f.U = U; f.D = D;
     f.entries = wl_entries(g, sync);
     f.err = err;
     f.active = nullptr;
@@ -1376,18 +1413,24 @@ static FwsArgs<T> fws_args(const Grid &g, const WfSync &sync, const double *A, T
     return f;
 }
 
-void wl_factor64(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<double> &wl,
-                 int *err, hipStream_t s) {
+template <class T>
+static void wl_factor(const Grid &g, const WfSync &sync, const double *A, double *Af,
+                      const WlPrecond<T> &wl, int *err, hipStream_t s) {
+    const FwsArgs<T> f = fws_args<T>(g, sync, A, Af, wl.L, wl.U, wl.D, err);
+    hipLaunchKernelGGL(k_wl_pack_factor<T>, dim3((unsigned)((f.entries + 255) / 256)), dim3(256), 0,
+                       s, f);
     hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
-    hipLaunchKernelGGL((k_factor_ws<double, kFactorPD>), dim3(sync.ntiles), dim3(256), 0, s,
-                       fws_args<double>(g, sync, A, wl.L, wl.U, wl.D, err));
+    hipLaunchKernelGGL((k_factor_ws<T, kFactorPD>), dim3(sync.ntiles), dim3(192), 0, s, f);
 }
 
-void wl_factor32(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<float> &wl,
-                 int *err, hipStream_t s) {
-    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
-    hipLaunchKernelGGL((k_factor_ws<float, kFactorPD>), dim3(sync.ntiles), dim3(256), 0, s,
-                       fws_args<float>(g, sync, A, wl.L, wl.U, wl.D, err));
+void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
+                 const WlPrecond<double> &wl, int *err, hipStream_t s) {
+    wl_factor<double>(g, sync, A, Af, wl, err, s);
+}
+
+void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
+                 const WlPrecond<float> &wl, int *err, hipStream_t s) {
+    wl_factor<float>(g, sync, A, Af, wl, err, s);
 }
 
 struct WlBatch {
@@ -1395,6 +1438,8 @@ struct WlBatch {
     bool fp32 = false;
     void *fwd = nullptr, *bwd = nullptr;  // WsArgs<T>[n]
     void *fac = nullptr;                  // FwsArgs<T>[n] (members given a Jacobian)
+    int npack = 0;                        // blocks of the operand packing
+    int *pmem = nullptr, *poff = nullptr; // packing block -> realization, its first block
     int *tmem = nullptr;                  // block -> realization
     unsigned **tickets = nullptr;         // [n]
 };
@@ -1404,7 +1449,7 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
     std::vector<WsArgs<T>> f(m.size()), bw(m.size());
     std::vector<FwsArgs<T>> fa(m.size());
     const bool factor = !m.empty() && m[0].A != nullptr;
-    std::vector<int> tmem;
+    std::vector<int> tmem, pmem, poff(m.size());
     std::vector<unsigned *> tk(m.size());
     for (size_t i = 0; i < m.size(); ++i) {
         const Grid &g = m[i].g;
@@ -1429,9 +1474,11 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
         if (factor) {
             WfSync fs = m[i].sync;
             fs.err = m[i].err;
-            fa[i] = fws_args<T>(g, fs, m[i].A, const_cast<T *>(L), const_cast<T *>(U),
+            fa[i] = fws_args<T>(g, fs, m[i].A, m[i].Af, const_cast<T *>(L), const_cast<T *>(U),
                                 const_cast<T *>(static_cast<const T *>(m[i].D)), m[i].err);
             fa[i].active = m[i].active;
+            poff[i] = (int)pmem.size();
+            pmem.insert(pmem.end(), (size_t)((entries + 255) / 256), (int)i);
         }
         tmem.insert(tmem.end(), m[i].sync.ntiles, (int)i);
         tk[i] = m[i].sync.ticket;
@@ -1444,8 +1491,13 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
         return -1;
     if (factor) {
         const size_t fb = m.size() * sizeof(FwsArgs<T>);
+        b.npack = (int)pmem.size();
         if (hipMalloc(&b.fac, fb) != hipSuccess ||
-            hipMemcpyAsync(b.fac, fa.data(), fb, hipMemcpyHostToDevice, s) != hipSuccess)
+            hipMalloc(reinterpret_cast<void **>(&b.pmem), pmem.size() * sizeof(int)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&b.poff), poff.size() * sizeof(int)) != hipSuccess ||
+            hipMemcpyAsync(b.fac, fa.data(), fb, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(b.pmem, pmem.data(), pmem.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(b.poff, poff.data(), poff.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
             return -1;
     }
     if (hipMemcpyAsync(b.fwd, f.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1491,19 +1543,25 @@ void wl_batch_apply(void *h, hipStream_t s) {
 void wl_batch_factor(void *h, hipStream_t s) {
     auto *b = static_cast<WlBatch *>(h);
     const dim3 rg((b->n + 255) / 256), rb(256);
+    if (b->fp32)
+        hipLaunchKernelGGL(k_wl_pack_factor_batch<float>, dim3(b->npack), dim3(256), 0, s,
+                           static_cast<const FwsArgs<float> *>(b->fac), b->pmem, b->poff);
+    else
+        hipLaunchKernelGGL(k_wl_pack_factor_batch<double>, dim3(b->npack), dim3(256), 0, s,
+                           static_cast<const FwsArgs<double> *>(b->fac), b->pmem, b->poff);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_factor_ws_batch<float, kFactorPD>), dim3(b->ntiles), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_factor_ws_batch<float, kFactorPD>), dim3(b->ntiles), dim3(192), 0, s,
                            static_cast<const FwsArgs<float> *>(b->fac), b->tmem);
     else
-        hipLaunchKernelGGL((k_factor_ws_batch<double, kFactorPD>), dim3(b->ntiles), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_factor_ws_batch<double, kFactorPD>), dim3(b->ntiles), dim3(192), 0, s,
                            static_cast<const FwsArgs<double> *>(b->fac), b->tmem);
 }
 
 void wl_batch_free(void *h) {
     auto *b = static_cast<WlBatch *>(h);
     if (!b) return;
-    void *p[] = {b->fwd, b->bwd, b->fac, b->tmem, b->tickets};
+    void *p[] = {b->fwd, b->bwd, b->fac, b->tmem, b->tickets, b->pmem, b->poff};
     for (void *q : p)
         if (q) (void)hipFree(q);
     delete b;

@@ -75,11 +75,12 @@ void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double 
                const double *A, const WlPrecond<float> &wl, hipStream_t s);
 // v = (LU)^-1 d with one compute, one edge and one publisher wavefront per tile
 // the factorization straight into the layout (FP64 arithmetic; FP32 rounds the stored
-// factors and upper blocks); entries without a cell keep what they hold (zeros)
-void wl_factor64(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<double> &wl,
-                 int *err, hipStream_t s);
-void wl_factor32(const Grid &g, const WfSync &sync, const double *A, const WlPrecond<float> &wl,
-                 int *err, hipStream_t s);
+// factors and upper blocks); entries without a cell keep what they hold (zeros). Af:
+// scratch for the operands in the forward layout, 7 x wl_entries x 4 doubles
+void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
+                 const WlPrecond<double> &wl, int *err, hipStream_t s);
+void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
+                 const WlPrecond<float> &wl, int *err, hipStream_t s);
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
                 double *v, const int *status, hipStream_t s);
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
@@ -98,6 +99,7 @@ struct WlMember {
     double *out;
     const int *status;
     const double *A = nullptr;     // Jacobian: the batch also factorizes (wl_batch_factor)
+    double *Af = nullptr;          // the realization's operand scratch (see wl_factor64)
     int *err = nullptr;            // the realization's factorization error flag
     const int *active = nullptr;   // factorize only when *active
 };
