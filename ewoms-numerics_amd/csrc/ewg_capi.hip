@@ -46,7 +46,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(ewg_ctx *c) {
     void *ptrs[] = {c->x0, c->x1, c->xcur, c->sto, c->R, c->J, c->iqv, c->iqd, c->K, c->phi,
                     c->pe, c->region,
-                    c->regs, c->bc_type, c->bc_val, c->L64, c->D64, c->L32, c->D32, c->U32,
+                    c->regs, c->bc_type, c->bc_val, c->bc_aux, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
                     c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks, c->partAny,
                     c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wl64.L, c->wl64.U,
@@ -83,6 +83,7 @@ LinArgs lin_args(ewg_ctx *c, double dt, int newton_iter) {
     a.regs = c->regs;
     a.bc_type = c->bc_type;
     a.bc_val = c->bc_val;
+    a.bc_aux = c->static_set && c->bc_set ? c->bc_aux : nullptr;
     for (int s = 0; s < 7; ++s) a.side_off[s] = c->side_off[s];
     a.dt = dt;
     a.newton_iter = newton_iter;
@@ -532,6 +533,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->regs, regs.size());
     ALLOC(c->bc_type, c->B);
     ALLOC(c->bc_val, 2 * c->B);
+    ALLOC(c->bc_aux, 3 * c->B);
     ALLOC(c->L64, (size_t)g.dim * N * 4);
     ALLOC(c->D64, (size_t)N * 4);
     ALLOC(c->kr, n2);
@@ -610,6 +612,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     if ((e = hipMemset(c->sto, 0, n2 * sizeof(double))) != hipSuccess) return fail("sto", e);
     if ((e = hipMemset(c->bc_type, 0, c->B * sizeof(int))) != hipSuccess) return fail("bc", e);
     if ((e = hipMemset(c->bc_val, 0, 2 * c->B * sizeof(double))) != hipSuccess) return fail("bcv", e);
+    if ((e = hipMemset(c->bc_aux, 0, 3 * c->B * sizeof(double))) != hipSuccess) return fail("bca", e);
     if ((e = hipMemcpy(c->regs, regs.data(), regs.size() * sizeof(MatReg), hipMemcpyHostToDevice)) !=
         hipSuccess)
         return fail("regs", e);
@@ -699,6 +702,10 @@ int ewg_set_static(ewg_ctx *c, const double *K, const double *phi, const int32_t
         c->pe = nullptr;
     }
     c->static_set = true;
+    if (c->bc_set) {
+        launch_bc_aux(lin_args(c, 1.0, 0), c->bc_aux, c->stream);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return EWG_OK;
 }
 
@@ -709,6 +716,11 @@ int ewg_set_boundary(ewg_ctx *c, const int32_t *type, const double *values) {
             return set_err(c, EWG_INVALID_ARGUMENT, "unknown boundary type");
     HIP_TRY(c, hipMemcpy(c->bc_type, type, c->B * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->bc_val, values, 2 * c->B * sizeof(double), hipMemcpyHostToDevice));
+    c->bc_set = true;
+    if (c->static_set) {
+        launch_bc_aux(lin_args(c, 1.0, 0), c->bc_aux, c->stream);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return EWG_OK;
 }
 

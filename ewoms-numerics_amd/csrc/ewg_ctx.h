@@ -37,6 +37,8 @@ struct ewg_ctx {
     MatReg *regs = nullptr;
     int *bc_type = nullptr;
     double *bc_val = nullptr;
+    double *bc_aux = nullptr;  // material-law values of the free-flow boundary states
+    bool bc_set = false;
     // intensive quantities of the linearization's first pass
     double *iqv = nullptr, *iqd = nullptr;
     // preconditioner

@@ -42,6 +42,8 @@ struct LinArgs {
     const MatReg *regs;
     const int *bc_type;
     const double *bc_val;
+    const double *bc_aux; // per boundary face: p_c, lambda_w, lambda_n of the free-flow state
+                          // (k_bc_aux; nullptr: evaluated in the flux)
     long long side_off[7];
     double dt;
     int newton_iter;
@@ -73,6 +75,7 @@ constexpr int kRedThreads = 256;
 // linearization
 void launch_linearize(const LinArgs &a, hipStream_t s);
 void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s);
+void launch_bc_aux(const LinArgs &a, double *aux, hipStream_t s);  // per-face boundary values
 
 // ILU(0)
 void launch_ilu_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,

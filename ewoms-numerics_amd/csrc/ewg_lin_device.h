@@ -103,6 +103,37 @@ __device__ __forceinline__ Evf narrow<float>(const Ev &x) {
 __device__ __forceinline__ Ev widen(const Ev &x) { return x; }
 __device__ __forceinline__ Ev widen(const Evf &x) { return to_f64(x); }
 
+// x / d where d is usually a power of two (grid spacings, cell volumes): then x times the
+// exact reciprocal, which rounds the same real number once as the division does (so the
+// bits are the division's); otherwise the division. d is uniform over the launch, so the
+// branch does not diverge.
+__device__ __forceinline__ bool pow2_recip(double d, double &r) {
+    const unsigned long long b = __double_as_longlong(d);
+    const unsigned e = (unsigned)(b >> 52) & 0x7FFu;
+    if ((b & 0xFFFFFFFFFFFFFull) != 0ull || e < 1u || e > 2045u) return false;
+    r = __longlong_as_double((b & 0x8000000000000000ull) | ((unsigned long long)(2046u - e) << 52));
+    return true;
+}
+__device__ __forceinline__ bool pow2_recip(float d, float &r) {
+    const unsigned b = __float_as_uint(d);
+    const unsigned e = (b >> 23) & 0xFFu;
+    if ((b & 0x7FFFFFu) != 0u || e < 1u || e > 253u) return false;
+    r = __uint_as_float((b & 0x80000000u) | ((254u - e) << 23));
+    return true;
+}
+template <class T>
+__device__ __forceinline__ T div_p2(T x, T d) {
+    T r;
+    if (pow2_recip(d, r)) return x * r;
+    return x / d;
+}
+template <class T>
+__device__ __forceinline__ Dual<T> div_p2(const Dual<T> &x, T d) {
+    T r;
+    if (pow2_recip(d, r)) return Dual<T>{x.v * r, x.d0 * r, x.d1 * r};
+    return x / d;
+}
+
 // ImmiscibleIntensiveQuantities::update (immiscibleintensivequantities.hh:87-145)
 template <class T>
 __device__ __forceinline__ IQ<T> intensive(const LinArgs &a, int cell, T p0, T Sw) {
@@ -129,6 +160,14 @@ __device__ __forceinline__ void storage(const LinArgs &a, double phi, T Sw, T M[
 
 // Interior face flux from the focus side: fvbasegradientcalculator.hh:195-240,
 // darcyfluxmodule.hh:185-324, :461-527, immisciblelocalresidual.hh:123-149.
+//
+// The reference evaluates the gradient, the Darcy velocity K grad and the normal flux as
+// dim-vectors with a permeability tensor. On this structured Cartesian grid the face
+// normal and the centre-to-centre vector lie on the face's axis and the permeability is
+// K I, so every other component is a sum of products with an exact zero: only the axis
+// component is evaluated here, with the same operations in the same order. Its nonzero
+// values are therefore the reference's bit for bit (dropping "+ (+-0)" terms can change
+// only the sign of a zero result, which no comparison or nonzero value depends on).
 template <int DIM, class S>
 __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in,
                                               const IQ<double> &ex, double Kin, double Kex,
@@ -136,25 +175,19 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
     using D = Dual<S>;
     const Grid &g = a.g;
     constexpr int dim = DIM;
+    const int ax = dir >> 1;
     int q[3] = {ijk[0], ijk[1], ijk[2]};
-    q[dir >> 1] += (dir & 1) ? 1 : -1;
+    q[ax] += (dir & 1) ? 1 : -1;
     double posIn[3], posEx[3], posFace[3];
     centre(g, ijk, posIn);
     centre(g, q, posEx);
     face_centre(g, ijk, dir, posFace);
-    double normal[3] = {0.0, 0.0, 0.0};
-    normal[dir >> 1] = (dir & 1) ? 1.0 : -1.0;
+    const S normal = (dir & 1) ? S(1) : S(-1);  // the axis component of the unit normal
 
     double distSq = 0.0;
     for (int d = 0; d < dim; ++d) {
         double t = posEx[d] - posIn[d];
         distSq += t * t;
-    }
-    D grad[2][3];
-    for (int k = 0; k < 2; ++k) {
-        D deltay = narrow<S>(ev(ex.p[k]) - in.p[k]);
-        for (int d = 0; d < dim; ++d)
-            grad[k][d] = deltay * static_cast<S>((posEx[d] - posIn[d]) / distSq);
     }
     double dvIn[3], dvEx[3], dvTot[3];
     for (int d = 0; d < 3; ++d) {
@@ -165,92 +198,90 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
     double absDistTotSq = 0.0;
     for (int d = 0; d < dim; ++d) absDistTotSq += dvTot[d] * dvTot[d];
     double gIn = dot_g<DIM>(a, dvIn), gEx = dot_g<DIM>(a, dvEx);
-    for (int k = 0; k < 2; ++k) {
-        D pStatIn = -cst<S>(a.rho[k]) * static_cast<S>(gIn);
-        D pStatEx = cst<S>(-a.rho[k] * gEx);
-        D qq = (pStatEx - pStatIn) / static_cast<S>(absDistTotSq);
-        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + cst<S>(dvTot[d]) * qq;
-    }
     const S Kf = static_cast<S>(harmonic_mean(Kin, Kex));
+    const S gscale = static_cast<S>(div_p2(posEx[ax] - posIn[ax], distSq));
     for (int k = 0; k < 2; ++k) {
-        D tmp = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * static_cast<S>(normal[d]);
-        bool upIsEx = tmp.v > S(0);
-        D lam = upIsEx ? cst<S>(ex.lam[k]) : narrow<S>(in.lam[k]);
-        D v[3];
-        for (int r = 0; r < dim; ++r) {
-            v[r] = cst<S>(0.0);
-            for (int c = 0; c < dim; ++c) v[r] = v[r] + (r == c ? Kf : S(0)) * grad[k][c];
-        }
-        D mlam = -lam;
-        for (int r = 0; r < dim; ++r) v[r] = v[r] * mlam;
-        D vol = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) vol = vol + v[d] * static_cast<S>(normal[d]);
+        // gradient along the axis: two-point difference plus the hydrostatic correction
+        const D deltay = narrow<S>(ev(ex.p[k]) - in.p[k]);
+        D grad = deltay * gscale;
+        const D pStatIn = -cst<S>(a.rho[k]) * static_cast<S>(gIn);
+        const D pStatEx = cst<S>(-a.rho[k] * gEx);
+        const D qq = div_p2(pStatEx - pStatIn, static_cast<S>(absDistTotSq));
+        grad = grad + cst<S>(dvTot[ax]) * qq;
+        // upstream mobility by the sign of grad . n, Darcy velocity -lambda K grad
+        const bool upIsEx = grad.v * normal > S(0);
+        const D lam = upIsEx ? cst<S>(ex.lam[k]) : narrow<S>(in.lam[k]);
+        D v = Kf * grad;
+        v = v * (-lam);
+        const D vol = v * normal;
         F[k] = widen(cst<S>(0.0) + vol * cst<S>(a.rho[k]));
     }
 }
 
 // Dirichlet free-flow boundary: darcyfluxmodule.hh:332-453, :488-509,
-// fvbasegradientcalculator.hh:278-312, immiscibleboundaryratevector.hh:91-153.
+// fvbasegradientcalculator.hh:278-312, immiscibleboundaryratevector.hh:91-153. As for the
+// interior faces, only the axis component of the gradient and velocity is evaluated
+// (the others are sums of products with exact zeros). The boundary state's capillary
+// pressure and mobilities come from a.bc_aux when it is set (computed once per boundary
+// face by bc_aux_cell with the same functions), else they are evaluated here.
 template <int DIM, class S>
 __device__ __forceinline__ void freeflow_flux(const LinArgs &a, const IQ<Ev> &in, int cell,
                                               double Kc, const int ijk[3], int dir,
-                                              double pwB, double SwB, Ev B[2]) {
+                                              double pwB, double SwB, long long bf, Ev B[2]) {
     using D = Dual<S>;
     const Grid &g = a.g;
     constexpr int dim = DIM;
-    const MatReg &r = region_of(a, cell);
-    double pe = a.pe ? a.pe[cell] : 0.0;
-    double pcB = pcnw(r, pe, SwB);
+    const int ax = dir >> 1;
+    double pcB, mobB[2];
+    if (a.bc_aux) {
+        pcB = a.bc_aux[3 * bf];
+        mobB[0] = a.bc_aux[3 * bf + 1];
+        mobB[1] = a.bc_aux[3 * bf + 2];
+    } else {
+        const MatReg &r = region_of(a, cell);
+        const double pe = a.pe ? a.pe[cell] : 0.0;
+        pcB = pcnw(r, pe, SwB);
+        double krB[2];
+        relperm(r, SwB, krB[0], krB[1]);
+        mobB[0] = krB[0] / a.mu[0];
+        mobB[1] = krB[1] / a.mu[1];
+    }
     double pB[2] = {pwB, (pwB + pcB) - 0.0};
-    double krB[2];
-    relperm(r, SwB, krB[0], krB[1]);
     double posIn[3], posFace[3];
     centre(g, ijk, posIn);
     face_centre(g, ijk, dir, posFace);
-    double normal[3] = {0.0, 0.0, 0.0};
-    normal[dir >> 1] = (dir & 1) ? 1.0 : -1.0;
+    const S normal = (dir & 1) ? S(1) : S(-1);
     double distSq = 0.0;
     for (int d = 0; d < dim; ++d) {
         double t = posFace[d] - posIn[d];
         distSq += t * t;
-    }
-    D grad[2][3];
-    for (int k = 0; k < 2; ++k) {
-        D deltay = narrow<S>(pB[k] - in.p[k]);
-        for (int d = 0; d < dim; ++d)
-            grad[k][d] = deltay * static_cast<S>((posFace[d] - posIn[d]) / distSq);
     }
     double dvIn[3];
     for (int d = 0; d < 3; ++d) dvIn[d] = posIn[d] - posFace[d];
     double absDistSq = 0.0;
     for (int d = 0; d < dim; ++d) absDistSq += dvIn[d] * dvIn[d];
     double gTimesDist = dot_g<DIM>(a, dvIn);
-    for (int k = 0; k < 2; ++k) {
-        D pStatIn = static_cast<S>(-gTimesDist) * cst<S>(a.rho[k]);
-        D qq = pStatIn / static_cast<S>(absDistSq);
-        for (int d = 0; d < dim; ++d) grad[k][d] = grad[k][d] + cst<S>(dvIn[d]) * qq;
-    }
+    const S gscale = static_cast<S>(div_p2(posFace[ax] - posIn[ax], distSq));
     const S Ks = static_cast<S>(Kc);
     for (int k = 0; k < 2; ++k) {
-        D tmp = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) tmp = tmp + grad[k][d] * static_cast<S>(normal[d]);
-        bool upIsBoundary = tmp.v > S(0);
-        D mob = upIsBoundary ? cst<S>(krB[k] / a.mu[k]) : narrow<S>(in.lam[k]);
-        D v[3];
-        for (int rr = 0; rr < dim; ++rr) {
-            v[rr] = cst<S>(0.0);
-            for (int c = 0; c < dim; ++c) v[rr] = v[rr] + (rr == c ? Ks : S(0)) * grad[k][c];
-        }
-        D mmob = -mob;
-        for (int rr = 0; rr < dim; ++rr) v[rr] = v[rr] * mmob;
-        D vol = cst<S>(0.0);
-        for (int d = 0; d < dim; ++d) vol = vol + v[d] * static_cast<S>(normal[d]);
+        const D deltay = narrow<S>(pB[k] - in.p[k]);
+        D grad = deltay * gscale;
+        const D pStatIn = static_cast<S>(-gTimesDist) * cst<S>(a.rho[k]);
+        const D qq = div_p2(pStatIn, static_cast<S>(absDistSq));
+        grad = grad + cst<S>(dvIn[ax]) * qq;
+        const bool upIsBoundary = grad.v * normal > S(0);
+        const D mob = upIsBoundary ? cst<S>(mobB[k]) : narrow<S>(in.lam[k]);
+        D v = Ks * grad;
+        v = v * (-mob);
+        const D vol = v * normal;
         // density by pressure comparison: both candidates are the same constant here
         double dens = a.rho[k];
         B[k] = widen(cst<S>(0.0) + vol * cst<S>(dens));
     }
 }
+
+// the material-law values of the free-flow boundary states next to cell i (per face)
+__device__ __forceinline__ void bc_aux_cell(const LinArgs &a, int i, double *aux);
 
 __device__ __forceinline__ long long boundary_face(const LinArgs &a, const int ijk[3], int dir) {
     const Grid &g = a.g;
@@ -260,6 +291,25 @@ __device__ __forceinline__ long long boundary_face(const LinArgs &a, const int i
     else if (ax == 1) loc = ijk[0] + (long long)g.nx * ijk[2];
     else loc = ijk[0] + (long long)g.nx * ijk[1];
     return a.side_off[dir] + loc;
+}
+
+__device__ __forceinline__ void bc_aux_cell(const LinArgs &a, int i, double *aux) {
+    const Grid &g = a.g;
+    int ijk[3];
+    cell_coords(g, i, ijk);
+    for (int dir = 0; dir < 2 * g.dim; ++dir) {
+        if (has_neighbour(g, ijk, dir)) continue;
+        const long long bf = boundary_face(a, ijk, dir);
+        if (a.bc_type[bf] != 1) continue;
+        const MatReg &r = region_of(a, i);
+        const double pe = a.pe ? a.pe[i] : 0.0;
+        const double SwB = a.bc_val[2 * bf + 1];
+        double krB[2];
+        relperm(r, SwB, krB[0], krB[1]);
+        aux[3 * bf] = pcnw(r, pe, SwB);
+        aux[3 * bf + 1] = krB[0] / a.mu[0];
+        aux[3 * bf + 2] = krB[1] / a.mu[1];
+    }
 }
 
 __device__ __forceinline__ void store_block(double *J, int slot, int N, int cell, const Ev r[2]) {
@@ -330,7 +380,7 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
         for (int k = 0; k < 2; ++k) {
             F[k] = F[k] * alpha;
             res0[k] = res0[k] + F[k];
-            rj[k] = (ev(0.0) - F[k]) / g.V;
+            rj[k] = div_p2(ev(0.0) - F[k], g.V);
         }
         store_block(a.J, slot_of_dir(g, dir ^ 1), g.N, nb, rj);
     }
@@ -381,7 +431,7 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
         int t = a.bc_type[bf];
         Ev Bv[2] = {ev(0.0), ev(0.0)};
         if (t == 1) {
-            freeflow_flux<DIM, S>(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf], a.bc_val[2 * bf + 1], Bv);
+            freeflow_flux<DIM, S>(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf], a.bc_val[2 * bf + 1], bf, Bv);
         } else if (t == 2) {
             Bv[0] = ev(a.bc_val[2 * bf]);
             Bv[1] = ev(a.bc_val[2 * bf + 1]);
@@ -390,7 +440,7 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
         for (int k = 0; k < 2; ++k) res0[k] = res0[k] + Bv[k] * mult;
     }
     // volumetric residual (fvbaselocalresidual.hh:170-186)
-    for (int k = 0; k < 2; ++k) res0[k] = res0[k] / g.V;
+    for (int k = 0; k < 2; ++k) res0[k] = div_p2(res0[k], g.V);
     reinterpret_cast<double2 *>(a.R)[i] = make_double2(res0[0].v, res0[1].v);
     store_block(a.J, g.diag, g.N, i, res0);
 }

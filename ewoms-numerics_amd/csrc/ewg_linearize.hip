@@ -23,6 +23,12 @@ __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
     linearize_cell<DIM, S>(a, i);
 }
 
+__global__ void __launch_bounds__(256) k_bc_aux(LinArgs a, double *aux) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.g.N) return;
+    bc_aux_cell(a, i, aux);
+}
+
 __global__ void __launch_bounds__(256) k_storage(LinArgs a, const double *x, double *M) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.g.N) return;
@@ -48,6 +54,10 @@ void launch_linearize(const LinArgs &a, hipStream_t s) {
         else
             hipLaunchKernelGGL((k_linearize<2, double>), dim3(blocks), dim3(256), 0, s, a);
     }
+}
+
+void launch_bc_aux(const LinArgs &a, double *aux, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_aux, dim3((a.g.N + 255) / 256), dim3(256), 0, s, a, aux);
 }
 
 void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s) {
