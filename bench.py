@@ -276,7 +276,9 @@ def main():
             from ewg import cnn
             ws = cnn.warm_stats(ctx)
             forms[form]["warm_start"] = {"accepted": ws["accepted"], "rejected": ws["rejected"],
-                                         "predict_s": ws["predict_s"], "network": net_info}
+                                         "predict_s": ws["predict_s"],
+                                         "last_region_cells": net.last_inference_cells(),
+                                         "network": net_info}
         print(f"# {form} kernels {ks}", file=sys.stderr, flush=True)
         if form == "fp64":
             ms = ctx.time_kernel(EWG_KERNEL_COPY, args.copy_reps)

@@ -389,7 +389,7 @@ int newton(ewg_ctx *c, double dt) {
 int newton_step(ewg_ctx *c, double dt) {
     if (!(c->cnn && c->prm.warm_start)) return newton(c, dt);
     double t0 = now_s();
-    int rc = cnn::predict_guess(c->cnn, c->x1, c->K, c->phi, dt, c->x0, c->stream);
+    int rc = cnn::predict_guess(c->cnn, c->x1, c->K, c->phi, dt, c->x0, c->stream, c->src_box);
     if (rc != EWG_OK) return set_err(c, rc, "warm start: prediction failed");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->warm_predict_s += now_s() - t0;
@@ -717,6 +717,33 @@ int ewg_set_boundary(ewg_ctx *c, const int32_t *type, const double *values) {
     HIP_TRY(c, hipMemcpy(c->bc_type, type, c->B * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->bc_val, values, 2 * c->B * sizeof(double), hipMemcpyHostToDevice));
     c->bc_set = true;
+    // the cells next to boundaries that inject the non-wetting phase (a rate, or a
+    // free-flow state with S_n > 0), for the warm start's active region (side s of the
+    // boundary faces: x-, x+, y-, y+, z-, z+, each in the order of the other two axes)
+    {
+        int bx[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
+        const Grid &g = c->g;
+        for (int s = 0; s < 2 * g.dim; ++s) {
+            const int ax = s >> 1;
+            for (long long bf = c->side_off[s]; bf < c->side_off[s + 1]; ++bf) {
+                const bool inject = (type[bf] == 2 && values[2 * bf + 1] != 0.0) ||
+                                    (type[bf] == 1 && 1.0 - values[2 * bf + 1] > 1e-8);
+                if (!inject) continue;
+                const long long loc = bf - c->side_off[s];
+                int ijk[3];
+                const int hi[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
+                if (ax == 0) { ijk[1] = (int)(loc % g.ny); ijk[2] = (int)(loc / g.ny); }
+                else if (ax == 1) { ijk[0] = (int)(loc % g.nx); ijk[2] = (int)(loc / g.nx); }
+                else { ijk[0] = (int)(loc % g.nx); ijk[1] = (int)(loc / g.nx); }
+                ijk[ax] = (s & 1) ? hi[ax] : 0;
+                for (int d = 0; d < 3; ++d) {
+                    bx[2 * d] = std::min(bx[2 * d], ijk[d]);
+                    bx[2 * d + 1] = std::max(bx[2 * d + 1], ijk[d]);
+                }
+            }
+        }
+        for (int k = 0; k < 6; ++k) c->src_box[k] = bx[k];
+    }
     if (c->static_set) {
         launch_bc_aux(lin_args(c, 1.0, 0), c->bc_aux, c->stream);
         HIP_TRY(c, hipStreamSynchronize(c->stream));

@@ -41,6 +41,11 @@ constexpr unsigned kFar = 0x80000000u;  // offset that reads as zero (out of ran
 constexpr int kTapsPad = 28;
 constexpr int kWgParts = 128;           // cell partitions of the weight gradient
 constexpr int kConvPF = 4;              // convolution operand prefetch distance (K steps)
+// inference on the active region: cells with S_n above kActiveEps, grown by kActiveMargin
+// cells (the five 3x3x3 layers see 5 cells: the active cells and 3 more layers around them
+// get the whole-grid prediction bit for bit)
+constexpr double kActiveEps = 1e-8;
+constexpr int kActiveMargin = 8;
 
 struct Layer {
     int ci, co, cip, cipb;  // real in/out channels, K channels forward / backward data
@@ -451,6 +456,82 @@ __global__ void k_guess(const double *xold, const float *y, int n, ewg_cnn_norm 
     xg[2 * c + 1] = s;
 }
 
+// Inference on the active region: the bounding box (x0, x1, y0, y1, z0, z1) of the cells
+// with a non-wetting saturation above eps, merged into box by min / max atomics (one set
+// per block that has such a cell)
+__global__ void k_box_init(int *box) {
+    if (threadIdx.x < 6) box[threadIdx.x] = (threadIdx.x & 1) ? -1 : 0x7fffffff;
+}
+
+__global__ void __launch_bounds__(256)
+k_active_box(const double *x, int nx, int ny, int n, double eps, int *box) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int v[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
+    if (c < n && 1.0 - x[2 * (size_t)c + 1] > eps) {
+        const int xi = c % nx, r = c / nx, yi = r % ny, zi = r / ny;
+        v[0] = v[1] = xi;
+        v[2] = v[3] = yi;
+        v[4] = v[5] = zi;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        for (int o = 32; o > 0; o >>= 1) {
+            const int u = __shfl_xor(v[k], o);
+            v[k] = (k & 1) ? max(v[k], u) : min(v[k], u);
+        }
+    __shared__ int red[4][6];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; ++k) red[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            for (int k = 0; k < 6; ++k) v[k] = (k & 1) ? max(v[k], red[w][k]) : min(v[k], red[w][k]);
+        if (v[1] >= 0)
+            for (int k = 0; k < 6; ++k) {
+                if (k & 1) atomicMax(box + k, v[k]);
+                else atomicMin(box + k, v[k]);
+            }
+    }
+}
+
+// features of the cells of the crop [x0, x0 + cx) x [y0, y0 + cy) x [z0, ...) of the grid
+__global__ void k_features_crop(const double *x, const double *K, const double *phi, int nx,
+                                int ny, int x0, int y0, int z0, int cx, int cy, int n, double dt,
+                                ewg_cnn_norm nm, unsigned short *feat) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = q % cx, r = q / cx, j = r % cy, k = r / cy;
+    const size_t c = (size_t)(x0 + i) + (size_t)nx * ((size_t)(y0 + j) + (size_t)ny * (z0 + k));
+    const double p = x[2 * c], sw = x[2 * c + 1];
+    const float f0 = static_cast<float>((p - nm.p_ref) / nm.p_scale);
+    const float f1 = static_cast<float>(1.0 - sw);
+    const float f2 = static_cast<float>((log(K[c]) - nm.lnk_ref) / nm.lnk_scale);
+    const float f3 = static_cast<float>((phi[c] - nm.phi_ref) / nm.phi_scale);
+    const float f4 = static_cast<float>(dt / nm.dt_scale);
+    *reinterpret_cast<u32x4 *>(feat + (size_t)q * 8) =
+        u32x4{f2bf(f0) | (unsigned)f2bf(f1) << 16, f2bf(f2) | (unsigned)f2bf(f3) << 16,
+              f2bf(f4), 0u};
+}
+
+// guess at the crop's cells (k_guess's arithmetic); the others keep the old state
+__global__ void k_guess_crop(const double *xold, const float *y, int nx, int ny, int x0, int y0,
+                             int z0, int cx, int cy, int n, ewg_cnn_norm nm, double *xg) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = q % cx, r = q / cx, j = r % cy, k = r / cy;
+    const size_t c = (size_t)(x0 + i) + (size_t)nx * ((size_t)(y0 + j) + (size_t)ny * (z0 + k));
+    const float2 yv = reinterpret_cast<const float2 *>(y)[q];
+    double dp = (double)yv.x * nm.dp_scale, dsn = (double)yv.y * nm.ds_scale;
+    dp = fabs(dp) >= nm.dp_min ? dp : 0.0;
+    dsn = fabs(dsn) >= nm.ds_min ? dsn : 0.0;
+    const double p = xold[2 * c] + dp;
+    double s = xold[2 * c + 1] - dsn;
+    s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+    xg[2 * c] = p;
+    xg[2 * c + 1] = s;
+}
+
 template <class T>
 hipError_t dalloc(T **p, size_t count) {
     return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
@@ -474,6 +555,8 @@ struct ewg_cnn {
     float *y = nullptr, *g[2] = {};
     unsigned short *gz = nullptr, *gzT = nullptr, *actT = nullptr, *feat = nullptr;
     unsigned short *bfeat = nullptr;  // gathered batch (training)
+    int *box_d = nullptr, *box_h = nullptr;  // active-region box (device, pinned host)
+    int64_t last_crop_cells = 0;
     float *btarget = nullptr;
     float *part = nullptr;
     double *lpart = nullptr;
@@ -511,8 +594,10 @@ void free_net(ewg_cnn *n) {
         ptrs.push_back(n->h[l]);
         ptrs.push_back(n->u[l]);
     }
+    ptrs.push_back(n->box_d);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (n->box_h) (void)hipHostFree(n->box_h);
     if (n->ev0) (void)hipEventDestroy(n->ev0);
     if (n->ev1) (void)hipEventDestroy(n->ev1);
     if (n->stream) (void)hipStreamDestroy(n->stream);
@@ -550,8 +635,13 @@ ConvArgs conv_args(const ewg_cnn *n, int ncells) {
 
 // forward pass over `ncells` cells of feat; keeps h[0..3] (and u[1..3] when training)
 void forward(ewg_cnn *n, const unsigned short *feat, int ncells, bool keep_u, hipStream_t s,
-             bool timed) {
+             bool timed, int gx = 0, int gy = 0, int gz = 0) {
     ConvArgs a = conv_args(n, ncells);
+    if (gx > 0) {  // samples of another grid (a crop of the network's grid)
+        a.nx = gx;
+        a.ny = gy;
+        a.nz = gz;
+    }
     if (timed) (void)hipEventRecord(n->ev0, s);
     a.in = feat;
     a.w = n->wf[0];
@@ -691,15 +781,59 @@ namespace ewg {
 namespace cnn {
 
 int predict_guess(ewg_cnn *n, const double *xold, const double *K, const double *phi,
-                  double dt, double *xguess, hipStream_t s) {
+                  double dt, double *xguess, hipStream_t s, const int *src_box) {
     const int nc = static_cast<int>(n->ncell);
-    hipLaunchKernelGGL(k_features, dim3((nc + 255) / 256), dim3(256), 0, s, xold, K, phi, nc,
-                       dt, n->norm, n->feat);
-    forward(n, n->feat, nc, false, s, false);
-    hipLaunchKernelGGL(k_guess, dim3((nc + 255) / 256), dim3(256), 0, s, xold, n->y, nc,
-                       n->norm, xguess);
+    const int nx = n->nx, ny = n->ny, nz = n->nz;
+    // the active region: cells with non-wetting phase, and the cells next to rate
+    // boundaries that inject it
+    hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, s, n->box_d);
+    hipLaunchKernelGGL(k_active_box, dim3((nc + 255) / 256), dim3(256), 0, s, xold, nx, ny, nc,
+                       kActiveEps, n->box_d);
+    if (hipMemcpyAsync(n->box_h, n->box_d, 6 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return EWG_DEVICE_ERROR;
+    int b[6];
+    for (int k = 0; k < 6; ++k) b[k] = n->box_h[k];
+    if (src_box && src_box[1] >= src_box[0])
+        for (int k = 0; k < 6; ++k) b[k] = (k & 1) ? std::max(b[k], src_box[k]) : std::min(b[k], src_box[k]);
+    if (b[1] < b[0]) {  // nothing active: the prediction would be discarded everywhere
+        n->last_crop_cells = 0;
+        return hipMemcpyAsync(xguess, xold, 2 * n->ncell * sizeof(double), hipMemcpyDeviceToDevice,
+                              s) == hipSuccess ? EWG_OK : EWG_DEVICE_ERROR;
+    }
+    // grow by the margin, clip; x extent a multiple of 16 (nx is one)
+    const int M = kActiveMargin;
+    int x0 = std::max(0, b[0] - M), x1 = std::min(nx - 1, b[1] + M);
+    const int y0 = std::max(0, b[2] - M), y1 = std::min(ny - 1, b[3] + M);
+    const int z0 = std::max(0, b[4] - M), z1 = std::min(nz - 1, b[5] + M);
+    x0 = (x0 / 16) * 16;
+    int cx = ((x1 - x0 + 1 + 15) / 16) * 16;
+    if (x0 + cx > nx) x0 = nx - cx;
+    const int cy = y1 - y0 + 1, cz = z1 - z0 + 1;
+    const long long ncrop = (long long)cx * cy * cz;
+    if (2 * ncrop > nc) {  // most of the grid: the whole grid
+        n->last_crop_cells = nc;
+        hipLaunchKernelGGL(k_features, dim3((nc + 255) / 256), dim3(256), 0, s, xold, K, phi, nc,
+                           dt, n->norm, n->feat);
+        forward(n, n->feat, nc, false, s, false);
+        hipLaunchKernelGGL(k_guess, dim3((nc + 255) / 256), dim3(256), 0, s, xold, n->y, nc,
+                           n->norm, xguess);
+        return hipGetLastError() == hipSuccess ? EWG_OK : EWG_DEVICE_ERROR;
+    }
+    const int nq = static_cast<int>(ncrop);
+    n->last_crop_cells = nq;
+    hipLaunchKernelGGL(k_features_crop, dim3((nq + 255) / 256), dim3(256), 0, s, xold, K, phi, nx,
+                       ny, x0, y0, z0, cx, cy, nq, dt, n->norm, n->feat);
+    forward(n, n->feat, nq, false, s, false, cx, cy, cz);
+    if (hipMemcpyAsync(xguess, xold, 2 * n->ncell * sizeof(double), hipMemcpyDeviceToDevice, s) !=
+        hipSuccess)
+        return EWG_DEVICE_ERROR;
+    hipLaunchKernelGGL(k_guess_crop, dim3((nq + 255) / 256), dim3(256), 0, s, xold, n->y, nx, ny,
+                       x0, y0, z0, cx, cy, nq, n->norm, xguess);
     return hipGetLastError() == hipSuccess ? EWG_OK : EWG_DEVICE_ERROR;
 }
+
+int64_t last_inference_cells(const ewg_cnn *n) { return n ? n->last_crop_cells : 0; }
 
 bool fits_grid(const ewg_cnn *n, int nx, int ny, int nz) {
     return n && n->nx == nx && n->ny == ny && n->nz == nz && n->max_batch >= 1;
@@ -709,6 +843,8 @@ bool fits_grid(const ewg_cnn *n, int nx, int ny, int nz) {
 }  // namespace ewg
 
 extern "C" {
+
+int64_t ewg_cnn_last_inference_cells(const ewg_cnn *net) { return last_inference_cells(net); }
 
 int ewg_cnn_create(int32_t nx, int32_t ny, int32_t nz, int32_t max_batch, ewg_cnn **out) {
     if (!out) return EWG_INVALID_ARGUMENT;
@@ -754,6 +890,9 @@ int ewg_cnn_create(int32_t nx, int32_t ny, int32_t nz, int32_t max_batch, ewg_cn
     CNN_ALLOC(n->gzT, V * 32);
     CNN_ALLOC(n->actT, V * 3 * 32);
     CNN_ALLOC(n->feat, ncell * 8);
+    CNN_ALLOC(n->box_d, 6);
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&n->box_h), 6 * sizeof(int))) != hipSuccess)
+        return fail(e);
     CNN_ALLOC(n->part, (size_t)kWgParts * 27 * 1024);
     CNN_ALLOC(n->lpart, (V + 255) / 256);
     CNN_ALLOC(n->bfeat, V * 8);

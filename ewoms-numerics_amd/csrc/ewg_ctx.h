@@ -39,6 +39,7 @@ struct ewg_ctx {
     double *bc_val = nullptr;
     double *bc_aux = nullptr;  // material-law values of the free-flow boundary states
     bool bc_set = false;
+    int src_box[6] = {0, -1, 0, -1, 0, -1};  // cells next to injecting rate boundaries
     // intensive quantities of the linearization's first pass
     double *iqv = nullptr, *iqd = nullptr;
     // preconditioner

@@ -52,6 +52,7 @@ def _lib():
             "ewg_cnn_train_step": (C.c_int, [_vp, _vp, _vp, C.c_int64, I, C.c_int32,
                                              C.c_float, D]),
             "ewg_cnn_last_forward_ms": (C.c_int, [_vp, D]),
+            "ewg_cnn_last_inference_cells": (C.c_int64, [_vp]),
             "ewg_device_alloc": (C.c_int, [C.c_int64, C.POINTER(_vp)]),
             "ewg_device_free": (C.c_int, [_vp]),
             "ewg_device_upload": (C.c_int, [_vp, _vp, C.c_int64]),
@@ -226,6 +227,10 @@ class WarmStartNet:
         ms = C.c_double(0)
         self._check(self.lib.ewg_cnn_last_forward_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def last_inference_cells(self):
+        """cells of the last warm-start prediction's active region"""
+        return int(self.lib.ewg_cnn_last_inference_cells(self.h))
 
     @staticmethod
     def _idx(idx):

@@ -101,8 +101,17 @@ int ewg_device_download(void *dst, const void *src, int64_t bytes);
  * solution(1) + the network's prediction, and with storage of the previous time level
  * from solution(1) (first-iteration storage recycling presumes the plain guess); a solve
  * that fails from the prediction is redone from the plain guess. The network must have
- * been created for the context's grid with max_batch >= 1. The context does not own it. */
+ * been created for the context's grid with max_batch >= 1. The context does not own it.
+ *
+ * The prediction is evaluated on the active region only: the bounding box of the cells
+ * with non-wetting saturation above 1e-8 and of the cells next to boundaries that inject
+ * it, grown by 8 cells (the five 3x3x3 layers reach 5 cells, so the box's cells and 3
+ * layers around them get the whole-grid prediction bit for bit); elsewhere the guess is
+ * solution(1). A region of more than half the grid is evaluated on the whole grid. */
 int ewg_set_warm_start(ewg_ctx *ctx, ewg_cnn *net);
+
+/* Cells the last warm-start prediction was evaluated on (0: no active cell). */
+int64_t ewg_cnn_last_inference_cells(const ewg_cnn *net);
 
 /* Steps that started from the prediction and converged (accepted), steps redone from the
  * plain guess (rejected), the time of the predictions (s, host clock around the
