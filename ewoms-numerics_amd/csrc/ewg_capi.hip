@@ -594,7 +594,8 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
             if ((e = hipMemset(c->wl64.L, 0, 3 * E * 4 * sizeof(double))) != hipSuccess ||
                 (e = hipMemset(c->wl64.U, 0, 3 * E * 4 * sizeof(double))) != hipSuccess ||
                 (e = hipMemset(c->wl64.D, 0, E * 4 * sizeof(double))) != hipSuccess ||
-                (e = hipMemset(c->wl64.w, 0, E * 2 * sizeof(double))) != hipSuccess)
+                (e = hipMemset(c->wl64.w, 0, E * 2 * sizeof(double))) != hipSuccess ||
+                (e = hipMemset(c->wl_af, 0, 7 * E * 4 * sizeof(double))) != hipSuccess)
                 return fail("wl64", e);
         }
     }

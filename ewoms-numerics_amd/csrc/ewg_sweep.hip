@@ -839,7 +839,7 @@ k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t 
 //   wave 1 and wave 2 move the tile edges' Dinv between tiles as in the sweeps (four
 //          64-bit words per value).
 // The compute wave reads its operands from a copy of the Jacobian's blocks in the
-// forward layout (k_wl_pack_factor, a bandwidth-bound pass before the launch), so that
+// forward layout (k_wl_pack_factor, a pass over the cells before the launch), so that
 // its loads are contiguous like the sweeps' (the natural-order blocks of a skewed tile
 // column touch a separate cache line per lane and load). The same pass copies the upper
 // blocks A_i,+x/+y/+z into the backward layout.
@@ -1229,52 +1229,78 @@ __global__ void __launch_bounds__(192) k_factor_ws_batch(const FwsArgs<T> *args,
     factor_ws_body<T, PD>(a);
 }
 
-// The factorization's operands in the forward layout (entry p: the cell of that tile,
-// local step and lane; zeros where there is no cell or no neighbour) and the upper
-// blocks in the backward layout (entry p of the reflected grid), one thread per entry.
+// The factorization's operands in the forward layout and the upper blocks in the
+// backward layout, one thread per cell (see pack_cell_of for the order): each cell's
+// entries are written where its tile column reaches it (32 bytes at a time). Entries
+// without a cell are not written (zero from allocation).
 template <class T>
-__device__ __forceinline__ void pack_factor_entry(const FwsArgs<T> &a, unsigned p) {
-    if (p >= a.entries) return;
+__device__ __forceinline__ void pack_factor_cell(const FwsArgs<T> &a, int c) {
+    if (c >= a.N) return;
     const int nsteps = a.nsteps, ntx = a.sync.ntx;
-    const int lane = p & 63, q = p >> 6;
-    const int step = q % nsteps, tileId = q / nsteps;
-    const int lx = lane & 7, ly = lane >> 3;
-    const int x = (tileId % ntx) * 8 + lx, y = (tileId / ntx) * 8 + ly, z = step - lx - ly;
-    const bool ok = x < a.nx && y < a.ny && z >= 0 && z < a.nz;
+    const int x = c % a.nx, r = c / a.nx, y = r % a.ny, z = r / a.ny;
     const size_t N = (size_t)a.N, E4 = a.entries * 4;
     const double *A = a.A;
     auto blk = [&](bool has, size_t slot, long long cell, double *o) {
-        for (int e = 0; e < 4; ++e) o[e] = has ? A[(slot * N + (size_t)cell) * 4 + e] : 0.0;
+        if (has) {
+            const double2 *src = reinterpret_cast<const double2 *>(A + (slot * N + (size_t)cell) * 4);
+            const double2 u = src[0], w = src[1];
+            o[0] = u.x; o[1] = u.y; o[2] = w.x; o[3] = w.y;
+        } else {
+            o[0] = o[1] = o[2] = o[3] = 0.0;
+        }
     };
+    // forward entry: tile (x / 8, y / 8), local step z + lx + ly, lane lx + 8 ly
+    const int lx = x & 7, ly = y & 7;
+    const size_t ef = ((size_t)((y >> 3) * ntx + (x >> 3)) * nsteps + z + lx + ly) * 64 + lx + 8 * ly;
     auto put = [&](double *dst, const double *v) {
-        *reinterpret_cast<double2 *>(dst + (size_t)p * 4) = make_double2(v[0], v[1]);
-        *reinterpret_cast<double2 *>(dst + (size_t)p * 4 + 2) = make_double2(v[2], v[3]);
+        *reinterpret_cast<double2 *>(dst + ef * 4) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(dst + ef * 4 + 2) = make_double2(v[2], v[3]);
     };
-    const long long c = ok ? x + (long long)a.nx * (y + (long long)a.ny * z) : 0;
     const long long str[3] = {(long long)a.nxny, (long long)a.nx, 1};
     const bool has[3] = {z > 0, y > 0, x > 0};
     double v[4];
-    blk(ok, a.diag, c, v);
+    blk(true, a.diag, c, v);
     put(a.Af, v);
     for (int s = 0; s < 3; ++s) {
-        blk(ok, s, c, v);
+        blk(true, s, c, v);
         put(a.Af + (1 + s) * E4, v);
         // the lower neighbour's block towards this cell: its +z, +y, +x slot
-        blk(ok && has[s], a.diag + 3 - s, c - str[s], v);
+        blk(has[s], a.diag + 3 - s, c - str[s], v);
         put(a.Af + (4 + s) * E4, v);
     }
-    // backward layout: the reflected cell's upper blocks (+x, +y, +z), stored as T
-    const long long cb = ok ? (a.nx - 1 - x) + (long long)a.nx * ((a.ny - 1 - y) + (long long)a.ny * (a.nz - 1 - z)) : 0;
+    // backward entry of the cell: the same in the reflected grid
+    const int xb = a.nx - 1 - x, yb = a.ny - 1 - y, zb = a.nz - 1 - z;
+    const int lxb = xb & 7, lyb = yb & 7;
+    const size_t eb = ((size_t)((yb >> 3) * ntx + (xb >> 3)) * nsteps + zb + lxb + lyb) * 64 + lxb + 8 * lyb;
     for (int s = 0; s < 3; ++s) {
-        blk(ok, a.diag + 1 + s, cb, v);
-        T *dst = a.U + s * E4 + (size_t)p * 4;
-        for (int e = 0; e < 4; ++e) dst[e] = static_cast<T>(v[e]);
+        blk(true, a.diag + 1 + s, c, v);
+        T *dst = a.U + s * E4 + eb * 4;
+        if constexpr (sizeof(T) == 8) {
+            *reinterpret_cast<double2 *>(dst) = make_double2(v[0], v[1]);
+            *reinterpret_cast<double2 *>(dst + 2) = make_double2(v[2], v[3]);
+        } else {
+            *reinterpret_cast<float4 *>(dst) =
+                make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
+                            static_cast<float>(v[2]), static_cast<float>(v[3]));
+        }
     }
+}
+
+// block b: 4 z-planes of one tile column (b = tile * ceil(nz / 4) + z-chunk), thread t:
+// column (t % 8, (t / 8) % 8) at z-plane t / 64, so that a wave reads one plane of the
+// tile (eight 8-cell runs) and a block's writes fill rows of its tile's entries together
+template <class T>
+__device__ __forceinline__ int pack_cell_of(const FwsArgs<T> &a, int b, int t) {
+    const int nzc = (a.nz + 3) / 4;
+    const int tile = b / nzc, zc = b % nzc;
+    const int x = (tile % a.sync.ntx) * 8 + (t & 7), y = (tile / a.sync.ntx) * 8 + ((t >> 3) & 7);
+    const int z = 4 * zc + (t >> 6);
+    return x < a.nx && y < a.ny && z < a.nz ? x + a.nx * (y + a.ny * z) : a.N;
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_wl_pack_factor(FwsArgs<T> a) {
-    pack_factor_entry<T>(a, blockIdx.x * 256u + threadIdx.x);
+    pack_factor_cell<T>(a, pack_cell_of(a, blockIdx.x, threadIdx.x));
 }
 
 // several realizations: block -> realization by pmem, its first block at poff
@@ -1284,7 +1310,7 @@ k_wl_pack_factor_batch(const FwsArgs<T> *args, const int *pmem, const int *poff)
     const int m = pmem[blockIdx.x];
     const FwsArgs<T> &a = args[m];
     if (a.active && *a.active == 0) return;
-    pack_factor_entry<T>(a, (blockIdx.x - poff[m]) * 256u + threadIdx.x);
+    pack_factor_cell<T>(a, pack_cell_of(a, (int)(blockIdx.x - poff[m]), threadIdx.x));
 }
 }  // namespace
 
@@ -1417,7 +1443,7 @@ template <class T>
 static void wl_factor(const Grid &g, const WfSync &sync, const double *A, double *Af,
                       const WlPrecond<T> &wl, int *err, hipStream_t s) {
     const FwsArgs<T> f = fws_args<T>(g, sync, A, Af, wl.L, wl.U, wl.D, err);
-    hipLaunchKernelGGL(k_wl_pack_factor<T>, dim3((unsigned)((f.entries + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(k_wl_pack_factor<T>, dim3((unsigned)(sync.ntiles * ((g.nz + 3) / 4))), dim3(256), 0,
                        s, f);
     hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
     hipLaunchKernelGGL((k_factor_ws<T, kFactorPD>), dim3(sync.ntiles), dim3(192), 0, s, f);
@@ -1478,7 +1504,7 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
                                 const_cast<T *>(static_cast<const T *>(m[i].D)), m[i].err);
             fa[i].active = m[i].active;
             poff[i] = (int)pmem.size();
-            pmem.insert(pmem.end(), (size_t)((entries + 255) / 256), (int)i);
+            pmem.insert(pmem.end(), (size_t)m[i].sync.ntiles * ((g.nz + 3) / 4), (int)i);
         }
         tmem.insert(tmem.end(), m[i].sync.ntiles, (int)i);
         tk[i] = m[i].sync.ticket;
