@@ -1239,49 +1239,48 @@ __device__ __forceinline__ void pack_factor_cell(const FwsArgs<T> &a, int c) {
     const int nsteps = a.nsteps, ntx = a.sync.ntx;
     const int x = c % a.nx, r = c / a.nx, y = r % a.ny, z = r / a.ny;
     const size_t N = (size_t)a.N, E4 = a.entries * 4;
-    const double *A = a.A;
-    auto blk = [&](bool has, size_t slot, long long cell, double *o) {
-        if (has) {
-            const double2 *src = reinterpret_cast<const double2 *>(A + (slot * N + (size_t)cell) * 4);
-            const double2 u = src[0], w = src[1];
-            o[0] = u.x; o[1] = u.y; o[2] = w.x; o[3] = w.y;
-        } else {
-            o[0] = o[1] = o[2] = o[3] = 0.0;
-        }
+    const double2 *__restrict__ A = reinterpret_cast<const double2 *>(a.A);
+    // all ten blocks first (the loads overlap), then the stores
+    const long long str[3] = {(long long)a.nxny, (long long)a.nx, 1};
+    const bool has[3] = {z > 0, y > 0, x > 0};
+    double2 v[10][2];
+    auto blk = [&](bool h, size_t slot, long long cell, double2 *o) {
+        const size_t k = (slot * N + (size_t)(h ? cell : 0)) * 2;
+        const double2 u = A[k], w = A[k + 1];
+        o[0] = h ? u : make_double2(0.0, 0.0);
+        o[1] = h ? w : make_double2(0.0, 0.0);
     };
+    blk(true, a.diag, c, v[0]);
+    for (int s = 0; s < 3; ++s) {
+        blk(true, s, c, v[1 + s]);
+        // the lower neighbour's block towards this cell: its +z, +y, +x slot
+        blk(has[s], a.diag + 3 - s, c - str[s], v[4 + s]);
+        blk(true, a.diag + 1 + s, c, v[7 + s]);
+    }
     // forward entry: tile (x / 8, y / 8), local step z + lx + ly, lane lx + 8 ly
     const int lx = x & 7, ly = y & 7;
     const size_t ef = ((size_t)((y >> 3) * ntx + (x >> 3)) * nsteps + z + lx + ly) * 64 + lx + 8 * ly;
-    auto put = [&](double *dst, const double *v) {
-        *reinterpret_cast<double2 *>(dst + ef * 4) = make_double2(v[0], v[1]);
-        *reinterpret_cast<double2 *>(dst + ef * 4 + 2) = make_double2(v[2], v[3]);
-    };
-    const long long str[3] = {(long long)a.nxny, (long long)a.nx, 1};
-    const bool has[3] = {z > 0, y > 0, x > 0};
-    double v[4];
-    blk(true, a.diag, c, v);
-    put(a.Af, v);
-    for (int s = 0; s < 3; ++s) {
-        blk(true, s, c, v);
-        put(a.Af + (1 + s) * E4, v);
-        // the lower neighbour's block towards this cell: its +z, +y, +x slot
-        blk(has[s], a.diag + 3 - s, c - str[s], v);
-        put(a.Af + (4 + s) * E4, v);
+    double *__restrict__ Af = a.Af;
+    for (int k = 0; k < 7; ++k) {
+        double2 *dst = reinterpret_cast<double2 *>(Af + k * E4 + ef * 4);
+        dst[0] = v[k][0];
+        dst[1] = v[k][1];
     }
     // backward entry of the cell: the same in the reflected grid
     const int xb = a.nx - 1 - x, yb = a.ny - 1 - y, zb = a.nz - 1 - z;
     const int lxb = xb & 7, lyb = yb & 7;
     const size_t eb = ((size_t)((yb >> 3) * ntx + (xb >> 3)) * nsteps + zb + lxb + lyb) * 64 + lxb + 8 * lyb;
+    T *__restrict__ U = a.U;
     for (int s = 0; s < 3; ++s) {
-        blk(true, a.diag + 1 + s, c, v);
-        T *dst = a.U + s * E4 + eb * 4;
+        T *dst = U + s * E4 + eb * 4;
+        const double2 p0 = v[7 + s][0], p1 = v[7 + s][1];
         if constexpr (sizeof(T) == 8) {
-            *reinterpret_cast<double2 *>(dst) = make_double2(v[0], v[1]);
-            *reinterpret_cast<double2 *>(dst + 2) = make_double2(v[2], v[3]);
+            reinterpret_cast<double2 *>(dst)[0] = p0;
+            reinterpret_cast<double2 *>(dst)[1] = p1;
         } else {
             *reinterpret_cast<float4 *>(dst) =
-                make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
-                            static_cast<float>(v[2]), static_cast<float>(v[3]));
+                make_float4(static_cast<float>(p0.x), static_cast<float>(p0.y),
+                            static_cast<float>(p1.x), static_cast<float>(p1.y));
         }
     }
 }
