@@ -71,6 +71,39 @@ def test_gradients_match_cpu_reference():
         assert e16 < 1e-2 and e64 < 5e-2, name
 
 
+def test_crop_inference_matches_whole_grid_inside():
+    """The warm start evaluates the network on a box of the grid (the active region grown by
+    8 cells): with zero padding at the box faces, every cell at least 5 cells (the reach of
+    the five 3x3x3 layers) inside a face that is not the grid's own gets the whole-grid
+    output bit for bit, since each output's accumulation does not depend on the other
+    cells of its tile."""
+    from ewg.cnn import DeviceArray, WarmStartNet, bf16_bits
+    nx, ny, nz = 64, 40, 24
+    x0, x1, y0, y1, z0, z1 = 16, 48, 0, 30, 6, 24  # the box touches the grid at y- and z+
+    rng = np.random.default_rng(21)
+    params = (R.init_params(seed=21, scale=2.0) + rng.normal(0.0, 0.05, R.NUM_PARAMS))
+    params = params.astype(np.float32)
+    feat = np.zeros((nz, ny, nx, 8))
+    feat[..., :5] = rng.normal(size=(nz, ny, nx, 5))
+    full = WarmStartNet(nx, ny, nz, max_batch=1, params=params)
+    y_full = full.forward(DeviceArray.from_numpy(bf16_bits(feat).reshape(1, -1)), 1)
+    y_full = y_full.numpy().reshape(nz, ny, nx, 2)
+    fc = np.ascontiguousarray(feat[z0:z1, y0:y1, x0:x1])
+    crop = WarmStartNet(x1 - x0, y1 - y0, z1 - z0, max_batch=1, params=params)
+    y_crop = crop.forward(DeviceArray.from_numpy(bf16_bits(fc).reshape(1, -1)), 1)
+    y_crop = y_crop.numpy().reshape(z1 - z0, y1 - y0, x1 - x0, 2)
+    # inside: 5 cells from the box's interior faces (none at y- and z+, the grid's faces)
+    zi, yi, xi = slice(5, z1 - z0), slice(0, y1 - y0 - 5), slice(5, x1 - x0 - 5)
+    box_full = y_full[z0:z1, y0:y1, x0:x1]
+    inner_crop, inner_full = y_crop[zi, yi, xi], box_full[zi, yi, xi]
+    print(f"crop {x1 - x0}x{y1 - y0}x{z1 - z0} of {nx}x{ny}x{nz}: inner {inner_crop.shape[:3]}, "
+          f"max |diff| {np.abs(inner_crop - inner_full).max():.3e}; whole box "
+          f"{np.abs(y_crop - box_full).max():.3e}", flush=True)
+    assert np.array_equal(inner_crop, inner_full)
+    # the zero padding does change the outputs next to the box's interior faces
+    assert not np.array_equal(y_crop, box_full)
+
+
 def test_training_fits_a_teacher_network():
     """Adam with the library's kernels drives the loss on a fixed batch down: targets are
     the FP64 outputs of another network of the same shape."""
