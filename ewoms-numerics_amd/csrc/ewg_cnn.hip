@@ -40,6 +40,8 @@ using u32x2 = __attribute__((ext_vector_type(2))) unsigned;
 constexpr unsigned kFar = 0x80000000u;  // offset that reads as zero (out of range)
 constexpr int kTapsPad = 28;
 constexpr int kWgParts = 128;           // cell partitions of the weight gradient
+constexpr int kWgRing = 4;              // weight-gradient chunks in flight per wavefront
+constexpr int kBiasParts = 32;          // cell partitions of the bias gradient
 constexpr int kConvPF = 4;              // convolution operand prefetch distance (K steps)
 // inference on the active region: cells with S_n above kActiveEps, grown by kActiveMargin
 // cells (the five 3x3x3 layers see 5 cells: the active cells and 3 more layers around them
@@ -264,19 +266,36 @@ __global__ void __launch_bounds__(64) k_wgrad(WgArgs a) {
     const int nchunk = a.ncells / 16;
     const int roff = dy * nx + dz * nxny;
     f32x16 acc[3] = {{}, {}, {}};
-    for (int c = p; c < nchunk; c += gridDim.x) {
+    // operands of chunk c (zeros past the last chunk or outside the grid)
+    auto load = [&](int c, bf16x8 &A, bf16x8 *B) {
+        const bool in = c < nchunk;
         const int v0 = 16 * c;  // the chunk lies in one x-row (nx is a multiple of 16)
         const int r = v0 / nx, y = r % ny, z = (r / ny) % nz;
-        const bool rowok = (unsigned)(y + dy) < (unsigned)ny && (unsigned)(z + dz) < (unsigned)nz;
-        const unsigned ga = i < a.cout ? (unsigned)((size_t)i * nc + v0 + 8 * g) * 2u : kFar;
-        const bf16x8 A = ld8(rg, ga);
+        const bool rowok = in && (unsigned)(y + dy) < (unsigned)ny && (unsigned)(z + dz) < (unsigned)nz;
+        const unsigned ga = in && i < a.cout ? (unsigned)((size_t)i * nc + v0 + 8 * g) * 2u : kFar;
+        A = ld8(rg, ga);
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             const unsigned bo = rowok && i < a.cin
                                     ? (unsigned)(((size_t)s * a.cin + i) * nc + v0 + 8 * g + roff) * 2u
                                     : kFar;
-            const bf16x8 B = ld8(ra, bo);
-            acc[s] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[s], 0, 0, 0);
+            B[s] = ld8(ra, bo);
+        }
+    };
+    // the block's chunks p, p + G, p + 2G, ... in that order, kWgRing of them in flight
+    const int G = gridDim.x;
+    bf16x8 Ar[kWgRing], Br[kWgRing][3];
+#pragma unroll
+    for (int k = 0; k < kWgRing; ++k) load(p + k * G, Ar[k], Br[k]);
+    for (int c = p; c < nchunk; c += kWgRing * G) {
+#pragma unroll
+        for (int k = 0; k < kWgRing; ++k) {
+            if (c + k * G < nchunk) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s)
+                    acc[s] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ar[k], Br[k][s], acc[s], 0, 0, 0);
+            }
+            load(c + (k + kWgRing) * G, Ar[k], Br[k]);
         }
     }
     // D[out][in]: lane holds in = i, out = 8q + 4g + r in element 4q + r
@@ -292,22 +311,28 @@ __global__ void __launch_bounds__(64) k_wgrad(WgArgs a) {
     }
 }
 
-// grad[layer W][out][in][tap] = sum over partitions (fixed order)
+// grad[layer W][out][in][tap] = sum over partitions (fixed order); one thread per entry of
+// a partition's [tap][out][in] block, so that the loads of a wave are contiguous
 __global__ void k_wreduce(const float *part, int nparts, int co, int ci, float *gw) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= co * ci * 27) return;
-    const int tap = e % 27, inn = (e / 27) % ci, out = e / (27 * ci);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 27 * 1024) return;
+    const int inn = q & 31, out = (q >> 5) & 31, tap = q >> 10;
+    if (inn >= ci || out >= co) return;
     float s = 0.0f;
-    for (int p = 0; p < nparts; ++p) s += part[((size_t)p * 27 + tap) * 1024 + out * 32 + inn];
-    gw[e] = s;
+#pragma unroll 8
+    for (int p = 0; p < nparts; ++p) s += part[(size_t)p * 27 * 1024 + q];
+    gw[((size_t)out * ci + inn) * 27 + tap] = s;
 }
 
-// db[out] = sum over cells of gzT[out][cell]; one block per output channel
-__global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *gzT, int ncells, float *db) {
-    const int o = blockIdx.x;
+// db[out] = sum over cells of gzT[out][cell]: block (out, b) sums partition b of the
+// cells into part[out][b], k_bias_final adds the partitions in order
+__global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *gzT, int ncells, float *part) {
+    const int o = blockIdx.x, b = blockIdx.y;
+    const int len = (ncells + kBiasParts - 1) / kBiasParts;
+    const int c0 = b * len, c1 = min(ncells, c0 + len);
     const unsigned short *row = gzT + (size_t)o * ncells;
     float s = 0.0f;
-    for (int c = threadIdx.x; c < ncells; c += 256) s += bf2f(row[c]);
+    for (int c = c0 + threadIdx.x; c < c1; c += 256) s += bf2f(row[c]);
     __shared__ float red[256];
     red[threadIdx.x] = s;
     __syncthreads();
@@ -315,7 +340,15 @@ __global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *gzT, in
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[o] = red[0];
+    if (threadIdx.x == 0) part[o * kBiasParts + b] = red[0];
+}
+
+__global__ void k_bias_final(const float *part, int co, float *db) {
+    const int o = threadIdx.x;
+    if (o >= co) return;
+    float s = 0.0f;
+    for (int b = 0; b < kBiasParts; ++b) s += part[o * kBiasParts + b];
+    db[o] = s;
 }
 
 // loss 1/(2 n) sum_c (y - t)^2 per block; gradient g = (y - t) / n as BF16 [cell][8]
@@ -556,6 +589,7 @@ struct ewg_cnn {
     unsigned short *gz = nullptr, *gzT = nullptr, *actT = nullptr, *feat = nullptr;
     unsigned short *bfeat = nullptr;  // gathered batch (training)
     int *box_d = nullptr, *box_h = nullptr;  // active-region box (device, pinned host)
+    float *bpart = nullptr;                   // bias-gradient partitions [32][kBiasParts]
     int64_t last_crop_cells = 0;
     float *btarget = nullptr;
     float *part = nullptr;
@@ -595,6 +629,7 @@ void free_net(ewg_cnn *n) {
         ptrs.push_back(n->u[l]);
     }
     ptrs.push_back(n->box_d);
+    ptrs.push_back(n->bpart);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (n->box_h) (void)hipHostFree(n->box_h);
@@ -690,11 +725,11 @@ void layer_grad(ewg_cnn *n, int l, const unsigned short *act, int ncells, hipStr
     w.part = n->part;
     const int nparts = std::max(1, std::min(kWgParts, ncells / 16));
     hipLaunchKernelGGL(k_wgrad, dim3(nparts, 9), dim3(64), 0, s, w);
-    const int ne = L.co * L.ci * 27;
-    hipLaunchKernelGGL(k_wreduce, dim3((ne + 255) / 256), dim3(256), 0, s, n->part, nparts,
+    hipLaunchKernelGGL(k_wreduce, dim3(27 * 1024 / 256), dim3(256), 0, s, n->part, nparts,
                        L.co, L.ci, n->grad + L.woff);
-    hipLaunchKernelGGL(k_bias_grad, dim3(L.co), dim3(256), 0, s, n->gzT, ncells,
-                       n->grad + L.boff);
+    hipLaunchKernelGGL(k_bias_grad, dim3(L.co, kBiasParts), dim3(256), 0, s, n->gzT, ncells,
+                       n->bpart);
+    hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(64), 0, s, n->bpart, L.co, n->grad + L.boff);
 }
 
 // forward + backward: loss into *loss, gradient into n->grad
@@ -891,6 +926,7 @@ int ewg_cnn_create(int32_t nx, int32_t ny, int32_t nz, int32_t max_batch, ewg_cn
     CNN_ALLOC(n->actT, V * 3 * 32);
     CNN_ALLOC(n->feat, ncell * 8);
     CNN_ALLOC(n->box_d, 6);
+    CNN_ALLOC(n->bpart, 32 * kBiasParts);
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&n->box_h), 6 * sizeof(int))) != hipSuccess)
         return fail(e);
     CNN_ALLOC(n->part, (size_t)kWgParts * 27 * 1024);
