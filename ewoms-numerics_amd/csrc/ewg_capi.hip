@@ -181,12 +181,14 @@ void precond_factor(ewg_ctx *c) {
     if (fp32) launch_round_factors(c->g, c->L64, c->D64, c->J, c->L32, c->D32, c->U32, c->stream);
 }
 
-void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status) {
+// tickets_clear: the kernel before on the stream has cleared the layout sweeps' tickets
+void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status,
+                   bool tickets_clear) {
     if (use_wavefront_layout(c)) {
         if (c->prm.precond_fp32)
-            wl_apply32(c->g, c->wf, c->wl32, d, v, status, c->stream);
+            wl_apply32(c->g, c->wf, c->wl32, d, v, status, c->stream, tickets_clear);
         else
-            wl_apply64(c->g, c->wf, c->wl64, d, v, status, c->stream);
+            wl_apply64(c->g, c->wf, c->wl64, d, v, status, c->stream, tickets_clear);
         return;
     }
     if (use_wavefront(c)) {
@@ -235,6 +237,9 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     if (*c->err_host) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
     if (c->ks_host->status == KS_CONVERGED) return EWG_OK;
     const int *st = &c->ks->status;
+    // the vector update before each preconditioner apply clears the layout sweeps'
+    // tickets (one kernel launch fewer per sweep)
+    unsigned *tk = use_wavefront_layout(c) ? c->wf.ticket : nullptr;
     // Iterations are queued speculatively and the state is read back after 1, 2, 4, then
     // every 8 of them: every kernel of an iteration returns at once when the device-side
     // status has left KS_RUNNING, so the iterations queued past convergence are no-ops
@@ -246,9 +251,9 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
             // the first iteration of a batch runs for certain (the state read back before
             // it was KS_RUNNING): its first apply and SpMV are the sampled launches
             launch_ks_beta(c->ks, c->part2, s);
-            launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s);
+            launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s, tk);
             if (q == 0) sample_begin(c, EWG_KERNEL_ILU_APPLY);
-            precond_apply(c, c->kp, c->ky, st);
+            precond_apply(c, c->kp, c->ky, st, tk != nullptr);
             if (q == 0) {
                 sample_end(c, EWG_KERNEL_ILU_APPLY);
                 sample_begin(c, EWG_KERNEL_SPMV);
@@ -256,9 +261,9 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
             launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
             if (q == 0) sample_end(c, EWG_KERNEL_SPMV);
             launch_ks_alpha(c->ks, c->part0, s);
-            launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s);
+            launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s, tk);
             launch_ks_crit1(c->ks, c->part1, c->partAny, s);
-            precond_apply(c, c->kr, c->kz, st);
+            precond_apply(c, c->kr, c->kz, st, tk != nullptr);
             launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
             launch_ks_omega(c->ks, c->part0, c->part1, s);
             launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
@@ -557,7 +562,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
             w.ntx = (g.nx + T - 1) / T;
             w.nty = (g.ny + T - 1) / T;
             w.ntiles = w.ntx * w.nty;
-            ALLOC(ticket, 1);
+            ALLOC(ticket, 2);
             ALLOC(dorder, w.ntiles);
             w.edge_words = (size_t)w.ntiles * T * g.nz * kWfMaxWords;
             ALLOC(ex, w.edge_words);
@@ -944,7 +949,7 @@ int ewg_time_kernel(ewg_ctx *c, int which, int reps, double *mean_ms) {
             case EWG_KERNEL_SPMV:
                 launch_spmv(c->g, c->J, c->x0, c->ky, nullptr, nullptr, nullptr, nullptr, nullptr, s);
                 break;
-            case EWG_KERNEL_ILU_APPLY: precond_apply(c, c->R, c->ky, nullptr); break;
+            case EWG_KERNEL_ILU_APPLY: precond_apply(c, c->R, c->ky, nullptr, false); break;
             case EWG_KERNEL_ILU_FACTOR: precond_factor(c); break;
             case EWG_KERNEL_COPY: launch_copy(nb_vals, c->J, c->copy_dst, s); break;
             default:

@@ -101,11 +101,14 @@ void launch_bicg_init(int n, const double *b, double *r, double *x, double *p, d
 void launch_ks_init(KrylovState *ks, const double *partMax, const double *partDot,
                     double rel_tol, double abs_tol, double max_resid, hipStream_t s);
 void launch_ks_beta(KrylovState *ks, const double *partRho, hipStream_t s);
+// tickets (optional): two tile tickets the kernel clears for the preconditioner apply
+// that follows it
 void launch_p_update(int n, const KrylovState *ks, const double *r, const double *v, double *p,
-                     hipStream_t s);
+                     hipStream_t s, unsigned *tickets = nullptr);
 void launch_ks_alpha(KrylovState *ks, const double *partDen, hipStream_t s);
 void launch_hs_update(int n, const KrylovState *ks, const double *y, const double *v, double *x,
-                      double *r, double *partMax, int *anyNonzero, hipStream_t s);
+                      double *r, double *partMax, int *anyNonzero, hipStream_t s,
+                      unsigned *tickets = nullptr);
 void launch_ks_crit1(KrylovState *ks, const double *partMax, const int *anyNonzero, hipStream_t s);
 void launch_ks_omega(KrylovState *ks, const double *partTT, const double *partTS, hipStream_t s);
 void launch_xr_update(int n, const KrylovState *ks, const double *z, const double *t, double *x,

@@ -194,7 +194,13 @@ __global__ void __launch_bounds__(kRedThreads) k_ks_beta(KrylovState *ks, const 
 }
 
 __global__ void __launch_bounds__(kRedThreads)
-k_p_update(int n, const KrylovState *ks, const double *r, const double *v, double *p) {
+k_p_update(int n, const KrylovState *ks, const double *r, const double *v, double *p,
+           unsigned *tickets) {
+    // the tile tickets of the preconditioner apply that follows (its two sweeps)
+    if (tickets && blockIdx.x == 0 && threadIdx.x == 0) {
+        tickets[0] = 0u;
+        tickets[1] = 0u;
+    }
     if (ks->status != KS_RUNNING) return;
     const double omega = ks->omega, beta = ks->beta;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -225,7 +231,12 @@ __global__ void __launch_bounds__(kRedThreads) k_ks_alpha(KrylovState *ks, const
 
 __global__ void __launch_bounds__(kRedThreads)
 k_hs_update(int n, const KrylovState *ks, const double *y, const double *v, double *x, double *r,
-            double *partMax, int *anyNonzero) {
+            double *partMax, int *anyNonzero, unsigned *tickets) {
+    // the tile tickets of the preconditioner apply that follows (its two sweeps)
+    if (tickets && blockIdx.x == 0 && threadIdx.x == 0) {
+        tickets[0] = 0u;
+        tickets[1] = 0u;
+    }
     if (ks->status != KS_RUNNING) return;
     const double alpha = ks->alpha;
     double m = 0.0;
@@ -432,16 +443,18 @@ void launch_ks_beta(KrylovState *ks, const double *partRho, hipStream_t s) {
     hipLaunchKernelGGL(k_ks_beta, dim3(1), dim3(kRedThreads), 0, s, ks, partRho);
 }
 void launch_p_update(int n, const KrylovState *ks, const double *r, const double *v, double *p,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_p_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, r, v, p);
+                     hipStream_t s, unsigned *tickets) {
+    hipLaunchKernelGGL(k_p_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, r, v, p,
+                       tickets);
 }
 void launch_ks_alpha(KrylovState *ks, const double *partDen, hipStream_t s) {
     hipLaunchKernelGGL(k_ks_alpha, dim3(1), dim3(kRedThreads), 0, s, ks, partDen);
 }
 void launch_hs_update(int n, const KrylovState *ks, const double *y, const double *v, double *x,
-                      double *r, double *partMax, int *anyNonzero, hipStream_t s) {
+                      double *r, double *partMax, int *anyNonzero, hipStream_t s,
+                      unsigned *tickets) {
     hipLaunchKernelGGL(k_hs_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, ks, y, v, x, r,
-                       partMax, anyNonzero);
+                       partMax, anyNonzero, tickets);
 }
 void launch_ks_crit1(KrylovState *ks, const double *partMax, const int *anyNonzero, hipStream_t s) {
     hipLaunchKernelGGL(k_ks_crit1, dim3(1), dim3(kRedThreads), 0, s, ks, partMax, anyNonzero);

@@ -787,6 +787,10 @@ __global__ void k_reset_tickets(unsigned *const *tickets, int n) {
 }
 
 __global__ void k_sweep_reset_ticket(unsigned *ticket) { *ticket = 0u; }
+__global__ void k_reset_ticket_pair(unsigned *ticket) {
+    ticket[0] = 0u;
+    ticket[1] = 0u;
+}
 
 // Copy the factors into the wavefront layout: one thread per layout entry. FWD: L (3
 // lower slots, forward order); otherwise U (3 upper slots; UT is double for the
@@ -1394,7 +1398,7 @@ void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double 
 
 template <class T>
 static void wl_apply(const Grid &g, const WfSync &sync, const WlPrecond<T> &wl, const double *d,
-                     double *v, const int *status, hipStream_t s) {
+                     double *v, const int *status, hipStream_t s, bool tickets_clear) {
     const size_t entries = wl_entries(g, sync);
     WsArgs<T> f{};
     f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N;
@@ -1404,24 +1408,24 @@ static void wl_apply(const Grid &g, const WfSync &sync, const WlPrecond<T> &wl, 
     f.d = d;
     f.blk[0] = wl.L; f.blk[1] = wl.L + entries * 4; f.blk[2] = wl.L + 2 * entries * 4;
     f.Dinv = nullptr; f.w = wl.w; f.out = nullptr; f.status = status; f.sync = sync;
-    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
+    if (!tickets_clear) hipLaunchKernelGGL(k_reset_ticket_pair, dim3(1), dim3(1), 0, s, sync.ticket);
     hipLaunchKernelGGL((k_sweep_ws<T, true, kSweepPD>), dim3(sync.ntiles), dim3(192), 0, s, f);
     WsArgs<T> b = f;
     b.d = nullptr;
     b.blk[0] = wl.U; b.blk[1] = wl.U + entries * 4; b.blk[2] = wl.U + 2 * entries * 4;
     b.Dinv = wl.D;
     b.out = v;
-    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
+    b.sync.ticket = sync.ticket + 1;
     hipLaunchKernelGGL((k_sweep_ws<T, false, kSweepPD>), dim3(sync.ntiles), dim3(192), 0, s, b);
 }
 
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
-                double *v, const int *status, hipStream_t s) {
-    wl_apply<double>(g, sync, wl, d, v, status, s);
+                double *v, const int *status, hipStream_t s, bool tickets_clear) {
+    wl_apply<double>(g, sync, wl, d, v, status, s, tickets_clear);
 }
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
-                double *v, const int *status, hipStream_t s) {
-    wl_apply<float>(g, sync, wl, d, v, status, s);
+                double *v, const int *status, hipStream_t s, bool tickets_clear) {
+    wl_apply<float>(g, sync, wl, d, v, status, s, tickets_clear);
 }
 
 template <class T>

@@ -25,7 +25,7 @@ constexpr long long kWfMaxCells = 1LL << 25;
 // synchronisation state: the ticket counter is reset by a one-thread kernel in front of
 // every launch; the edge buffers hold the sentinel between launches
 struct WfSync {
-    unsigned *ticket;
+    unsigned *ticket;          // two counters (the warp-specialized apply uses both)
     const int2 *order;         // [ntiles] tiles (tx, ty) in anti-diagonal order
     unsigned long long *ex;    // [ntiles][TY][nz][kWfMaxWords] last-column values
     unsigned long long *ey;    // [ntiles][TX][nz][kWfMaxWords] last-row values
@@ -81,10 +81,13 @@ void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
                  const WlPrecond<double> &wl, int *err, hipStream_t s);
 void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
                  const WlPrecond<float> &wl, int *err, hipStream_t s);
+// The forward sweep takes tickets from sync.ticket[0], the backward sweep from
+// sync.ticket[1]; tickets_clear: the caller's previous kernel on the stream has set both
+// to zero (otherwise a small kernel does).
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
-                double *v, const int *status, hipStream_t s);
+                double *v, const int *status, hipStream_t s, bool tickets_clear = false);
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
-                double *v, const int *status, hipStream_t s);
+                double *v, const int *status, hipStream_t s, bool tickets_clear = false);
 
 // Several realizations' preconditioner applications v_i = (L_i U_i)^-1 d_i in one pair
 // of sweep launches (each realization with its own layout, tickets and edge buffers; its
