@@ -392,25 +392,24 @@ __global__ void __launch_bounds__(TX *TY) k_sweep(SweepArgs<T> a) {
 }
 
 // ----------------------------------------------------------------------------------
-// Warp-specialized sweep: one 8x8 tile per workgroup of three wavefronts.
+// Warp-specialized sweep: one 8x8 tile per workgroup of two wavefronts.
 //   wave 0 computes the tile's columns as k_sweep does, but reads the values of the
 //          upstream tiles from an LDS ring instead of global memory, so that its vector
-//          memory counter only ever holds operand prefetches issued PD steps earlier and
-//          its own stores (a late cross-tile load would make every wait on it also wait
-//          for all older loads and stores);
+//          memory counter only holds operand prefetches issued PD steps earlier and its
+//          own stores (a late cross-tile load would make every wait on it also wait for
+//          all older loads and stores). It stores its last column and row straight to
+//          the edge buffers with coherent stores: each is issued PD steps before the
+//          wait on any load issued after it, which covers its completion.
 //   wave 1 polls the upstream tiles' edge words (16 edge lanes x 4 z-values in flight),
 //          copies each value into its ring slot once the compute wave has moved past the
 //          value the slot held, publishes how many values of each edge lane are in place,
 //          and recycles the global words to the sentinel.
-//   wave 2 publishes the tile's own last column and row: the compute wave leaves those
-//          values in an LDS outbox and counts its steps, and wave 2 stores them to the
-//          edge buffers (coherent stores to words other tiles are polling are slow to
-//          complete, and would hold up every later wait of the compute wave).
 // The compute wave checks each step against counts it read one step earlier (so the
 // check does not expose the LDS latency) and announces its progress once per PD steps.
+// (A third wave that took the edge values from an LDS outbox and stored them made every
+// tile hand-over 1-2 us slower: the outbox round trip and that wave's polling.)
 // ----------------------------------------------------------------------------------
 constexpr int kRing = 16;  // ring slots per edge lane (z modulo kRing)
-constexpr int kOut = 32;   // outbox slots per downstream edge lane
 constexpr unsigned kLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
 
 __device__ __forceinline__ int lds_ld(const int *p) {
@@ -426,13 +425,8 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     static_assert(PD < kRing, "the edge wave must be able to run ahead of the compute wave");
     __shared__ P2<T> xch[2][NT];
     __shared__ P2<T> ering[2][8][kRing];  // [x edge | y edge][edge lane][z % kRing]
-    __shared__ P2<T> outbox[2][8][kOut];  // values for the downstream tiles, by z % kOut
     __shared__ int filled[16];            // values of each edge lane in place (z < filled)
     __shared__ int progress;              // compute steps completed (per PD steps)
-    __shared__ int stepdone;              // compute steps completed (every step)
-    __shared__ int pubstep;               // outputs of steps below it are published
-    __shared__ int dummy_i[NT];
-    __shared__ P2<T> dummy_v[NT];
     __shared__ unsigned s_tk;
     if (a.status && *a.status != KS_RUNNING) return;
     const int tid = threadIdx.x;
@@ -446,68 +440,19 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     const int tileId = ty * a.sync.ntx + tx;
     // edge lanes without an upstream tile read zeros for good (the matching blocks are
     // zero as well) and count as complete
-    for (int i = tid; i < 2 * 8 * kRing; i += 192) (&ering[0][0][0])[i] = P2<T>{T(0), T(0)};
-    for (int i = tid; i < 2 * NT; i += 192) (&xch[0][0])[i] = P2<T>{T(0), T(0)};
+    for (int i = tid; i < 2 * 8 * kRing; i += 2 * NT) (&ering[0][0][0])[i] = P2<T>{T(0), T(0)};
+    for (int i = tid; i < 2 * NT; i += 2 * NT) (&xch[0][0])[i] = P2<T>{T(0), T(0)};
     if (tid < 16) {
         const int dir = tid >> 3, e = tid & 7;
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + e < a.ny) : (ty > 0 && tx * TX + e < a.nx);
         filled[tid] = valid ? 0 : nz;
     }
-    if (tid == 0) {
-        progress = 0;
-        stepdone = 0;
-        pubstep = 0;
-    }
+    if (tid == 0) progress = 0;
     __syncthreads();
     constexpr unsigned zw = kWfMaxWords * 8;  // bytes per z of one edge lane
     const __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
     const __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
 
-    if (tid >= 2 * NT) {
-        // ----------------------------------------------------------- publisher wave
-        // lanes 0..7 forward the tile's last column (x edge rows), 8..15 its last row
-        const int l = tid - 2 * NT, e = l & 15, dir = e >> 3, idx = e & 7;
-        const bool valid = l < 16 && (dir == 0 ? (tx + 1 < a.sync.ntx && ty * TY + idx < a.ny)
-                                               : (ty + 1 < a.sync.nty && tx * TX + idx < a.nx));
-        const unsigned gout = !valid ? 0u : (unsigned)((((size_t)tileId * 8 + idx) * nz) * zw);
-        const P2<T> *box = &outbox[dir][idx][0];
-        int zp = valid ? 0 : nz;  // next z to publish; value z comes from step z + 7 + idx
-        unsigned idle = 0;
-        int shown = 0;
-        while (true) {
-            // all outputs of steps below this lane's next step are out
-            int ps = zp < nz ? zp + 7 + idx : 0x3fffffff;
-            ps = min(ps, __shfl_xor(ps, 1));
-            ps = min(ps, __shfl_xor(ps, 2));
-            ps = min(ps, __shfl_xor(ps, 4));
-            ps = min(ps, __shfl_xor(ps, 8));
-            if (l == 0 && ps > shown) {
-                lds_st(&pubstep, ps);
-                shown = ps;
-            }
-            if (__builtin_amdgcn_ballot_w64(zp < nz) == 0ull) break;
-            const int done = lds_ld(&stepdone);
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            const bool ready = zp < nz && zp + 7 + idx < done;
-            const P2<T> v = box[zp & (kOut - 1)];
-            ste<T>(rex, ready && dir == 0 ? gout + (unsigned)zp * zw : kFar, val_ew(v));
-            ste<T>(rey, ready && dir == 1 ? gout + (unsigned)zp * zw : kFar, val_ew(v));
-            zp += ready ? 1 : 0;
-            if (__builtin_amdgcn_ballot_w64(ready) != 0ull) {
-                idle = 0;
-            } else {
-                if (++idle >= kSpinLimit) {
-                    atomicExch(a.sync.err, 1);
-                    break;
-                }
-                if ((idle & 63) == 0 &&
-                    __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    break;
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        return;
-    }
     if (tid >= NT) {
         // ---------------------------------------------------------------- edge wave
         const int l = tid - NT, e = l & 15, j = l >> 4;
@@ -646,12 +591,10 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
 
     P2<T> prev = P2<T>{T(0), T(0)};
     const bool hasL = lx > 0, hasD = ly > 0;
-    // outputs for the downstream tiles go to the outbox (the publisher wave stores them);
-    // other lanes write their own dummy slots, and one lane announces each finished step
-    P2<T> *const oX = prodX ? &outbox[0][ly][0] : &dummy_v[tid];
-    P2<T> *const oY = prodY ? &outbox[1][lx][0] : &dummy_v[tid];
-    const int oMaskX = prodX ? kOut - 1 : 0, oMaskY = prodY ? kOut - 1 : 0;
-    int *const sdone = tid == 0 ? &stepdone : &dummy_i[tid];
+    // this tile's words in the edge buffers: its last column (edge lane ly) and last row
+    // (edge lane lx), z-value words consecutive
+    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nz) * zw);
+    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nz) * zw);
     // neighbour sources: the in-tile lane of the previous step, or the ring slot of this
     // lane's z (first column / row of the tile)
     const P2<T> *const eX = &ering[0][ly][0];
@@ -681,20 +624,6 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
         // executes them in that order)
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         if (tid == 0) lds_st(&progress, tau0);
-        // the outbox slots this block writes must have been published (z - kOut ago);
-        // the publisher only waits for this wave, so this rarely spins
-        {
-            bool full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
-            if (__builtin_amdgcn_ballot_w64(full) != 0ull) {
-                unsigned spins = 0;
-                do {
-                    __builtin_amdgcn_s_sleep(1);
-                    full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
-                } while (__builtin_amdgcn_ballot_w64(full) != 0ull && ++spins < kSpinLimit);
-                if (full) atomicExch(a.sync.err, 1);
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        }
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
             const int tau = tau0 + k;
@@ -739,9 +668,9 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 s1 += c.di[2] * r0; s1 += c.di[3] * r1;
                 res = P2<T>{s0, s1};
             }
-            // hand the downstream tiles' values to the publisher, then the result vector
-            oX[z & oMaskX] = res;
-            oY[z & oMaskY] = res;
+            // the downstream tiles' values, then the result vector
+            ste<T>(rex, prodX && zin ? gX + (unsigned)z * zw : kFar, val_ew(res));
+            ste<T>(rey, prodY && zin ? gY + (unsigned)z * zw : kFar, val_ew(res));
             if (FWD) {
                 // the last block's steps past nsteps would land in the next tile's rows
                 stp<T>(rw, tau < nsteps ? ow : kFar, res);
@@ -753,8 +682,6 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
             }
             prev = res;
             xch[wbuf][tid] = res;
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            lds_st(sdone, tau + 1);
             __builtin_amdgcn_wave_barrier();
             // counts and neighbour values for the next step
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -769,14 +696,14 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
 }
 
 template <class T, bool FWD, int PD>
-__global__ void __launch_bounds__(192) k_sweep_ws(WsArgs<T> a) {
+__global__ void __launch_bounds__(128) k_sweep_ws(WsArgs<T> a) {
     sweep_ws_body<T, FWD, PD>(a);
 }
 
 // the same sweep for several realizations in one launch: block -> realization by tmem,
 // each block then takes a tile ticket of its realization as the single launch does
 template <class T, bool FWD, int PD>
-__global__ void __launch_bounds__(192) k_sweep_ws_batch(const WsArgs<T> *args, const int *tmem) {
+__global__ void __launch_bounds__(128) k_sweep_ws_batch(const WsArgs<T> *args, const int *tmem) {
     const WsArgs<T> a = args[tmem[blockIdx.x]];
     sweep_ws_body<T, FWD, PD>(a);
 }
@@ -839,9 +766,10 @@ k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t 
 //   wave 0 computes Dinv_i = inv(A_ii - sum_j (A_ij Dinv_j) A_ji) over the -z, -y, -x
 //          neighbours j (Dinv_j from its own register, the LDS exchange or the edge
 //          ring), stores L_ij = A_ij Dinv_j in the forward layout (zeros where there is
-//          no cell or no neighbour) and Dinv_i at the cell's backward-layout entry;
-//   wave 1 and wave 2 move the tile edges' Dinv between tiles as in the sweeps (four
-//          64-bit words per value).
+//          no cell or no neighbour) and Dinv_i at the cell's backward-layout entry,
+//          and stores its last column's and row's Dinv to the edge buffers;
+//   wave 1 polls the upstream tiles' edge words into the LDS ring as in the sweeps
+//          (four 64-bit words per value).
 // The compute wave reads its operands from a copy of the Jacobian's blocks in the
 // forward layout (k_wl_pack_factor, a pass over the cells before the launch), so that
 // its loads are contiguous like the sweeps' (the natural-order blocks of a skewed tile
@@ -915,13 +843,8 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     static_assert(PD < kRing, "the edge wave must be able to run ahead of the compute wave");
     __shared__ B4 xch[2][NT];
     __shared__ B4 ering[2][8][kRing];
-    __shared__ B4 outbox[2][8][kOut];
     __shared__ int filled[16];
     __shared__ int progress;
-    __shared__ int stepdone;
-    __shared__ int pubstep;
-    __shared__ int dummy_i[NT];
-    __shared__ B4 dummy_v[NT];
     __shared__ unsigned s_tk;
     if (a.active && *a.active == 0) return;
     const int tid = threadIdx.x;
@@ -934,18 +857,14 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     const int nz = a.nz, nsteps = a.nsteps;
     const int ntx = a.sync.ntx;
     const int tileId = ty * ntx + tx;
-    for (int i = tid; i < 2 * 8 * kRing; i += 192) (&ering[0][0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
-    for (int i = tid; i < 2 * NT; i += 192) (&xch[0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
+    for (int i = tid; i < 2 * 8 * kRing; i += 2 * NT) (&ering[0][0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
+    for (int i = tid; i < 2 * NT; i += 2 * NT) (&xch[0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
     if (tid < 16) {
         const int dir = tid >> 3, e = tid & 7;
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + e < a.ny) : (ty > 0 && tx * TX + e < a.nx);
         filled[tid] = valid ? 0 : nz;
     }
-    if (tid == 0) {
-        progress = 0;
-        stepdone = 0;
-        pubstep = 0;
-    }
+    if (tid == 0) progress = 0;
     __syncthreads();
     constexpr unsigned zw = kWfMaxWords * 8;
     static_assert(kWfMaxWords == 4, "a Dinv block takes four edge words");
@@ -968,51 +887,6 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
         eb0 = (tb * nsteps + nz - 1 + lx + ly + lxb + lyb) * 64 + lxb + 8 * lyb;
     }
 
-    if (wave == 2) {
-        // ----------------------------------------------------------- publisher wave
-        const int e = l & 15, dir = e >> 3, idx = e & 7;
-        const bool valid = l < 16 && (dir == 0 ? (tx + 1 < ntx && ty * TY + idx < a.ny)
-                                               : (ty + 1 < a.sync.nty && tx * TX + idx < a.nx));
-        const unsigned gout = !valid ? 0u : (unsigned)((((size_t)tileId * 8 + idx) * nz) * zw);
-        const B4 *box = &outbox[dir][idx][0];
-        int zp = valid ? 0 : nz;
-        unsigned idle = 0;
-        int shown = 0;
-        while (true) {
-            int ps = zp < nz ? zp + 7 + idx : 0x3fffffff;
-            ps = min(ps, __shfl_xor(ps, 1));
-            ps = min(ps, __shfl_xor(ps, 2));
-            ps = min(ps, __shfl_xor(ps, 4));
-            ps = min(ps, __shfl_xor(ps, 8));
-            if (l == 0 && ps > shown) {
-                lds_st(&pubstep, ps);
-                shown = ps;
-            }
-            if (__builtin_amdgcn_ballot_w64(zp < nz) == 0ull) break;
-            const int done = lds_ld(&stepdone);
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            const bool ready = zp < nz && zp + 7 + idx < done;
-            const B4 v = box[zp & (kOut - 1)];
-            unsigned long long w[4];
-            __builtin_memcpy(w, v.a, sizeof(w));
-            ste4(rex, ready && dir == 0 ? gout + (unsigned)zp * zw : kFar, w);
-            ste4(rey, ready && dir == 1 ? gout + (unsigned)zp * zw : kFar, w);
-            zp += ready ? 1 : 0;
-            if (__builtin_amdgcn_ballot_w64(ready) != 0ull) {
-                idle = 0;
-            } else {
-                if (++idle >= kSpinLimit) {
-                    atomicExch(a.sync.err, 1);
-                    break;
-                }
-                if ((idle & 63) == 0 &&
-                    __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    break;
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        return;
-    }
     if (wave == 1) {
         // ---------------------------------------------------------------- edge wave
         const int e = l & 15, j = l >> 4;
@@ -1114,10 +988,9 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
 
     B4 prev = B4{{0.0, 0.0, 0.0, 0.0}};
     const bool hasL = lx > 0, hasD = ly > 0;
-    B4 *const oX = prodX ? &outbox[0][ly][0] : &dummy_v[l];
-    B4 *const oY = prodY ? &outbox[1][lx][0] : &dummy_v[l];
-    const int oMaskX = prodX ? kOut - 1 : 0, oMaskY = prodY ? kOut - 1 : 0;
-    int *const sdone = l == 0 ? &stepdone : &dummy_i[l];
+    // this tile's words in the edge buffers (last column: edge lane ly; last row: lx)
+    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nz) * zw);
+    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nz) * zw);
     const B4 *const eX = &ering[0][ly][0];
     const B4 *const eY = &ering[1][lx][0];
     const int *const fX = &filled[ly];
@@ -1137,18 +1010,6 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     for (int tau0 = 0; tau0 < nsteps; tau0 += PD) {
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         if (l == 0) lds_st(&progress, tau0);
-        {
-            bool full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
-            if (__builtin_amdgcn_ballot_w64(full) != 0ull) {
-                unsigned spins = 0;
-                do {
-                    __builtin_amdgcn_s_sleep(1);
-                    full = lds_ld(&pubstep) <= tau0 + PD - 1 - kOut;
-                } while (__builtin_amdgcn_ballot_w64(full) != 0ull && ++spins < kSpinLimit);
-                if (full) atomicExch(a.sync.err, 1);
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        }
 #pragma unroll
         for (int k = 0; k < PD; ++k) {
             const int tau = tau0 + k;
@@ -1199,8 +1060,13 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
             res.a[1] = -D[1] * det_1;
             res.a[2] = -D[2] * det_1;
             res.a[3] = D[0] * det_1;
-            oX[z & oMaskX] = res;
-            oY[z & oMaskY] = res;
+            {
+                // the downstream tiles' edge words (coherent stores)
+                unsigned long long w[4];
+                __builtin_memcpy(w, res.a, sizeof(w));
+                ste4(rex, prodX && zin ? gX + (unsigned)z * zw : kFar, w);
+                ste4(rey, prodY && zin ? gY + (unsigned)z * zw : kFar, w);
+            }
             const unsigned oL = tau < nsteps ? (wl0 + (unsigned)tau * 64u) * bB : kFar;
             stb<T>(rL0, oL, Lv[0]);
             stb<T>(rL1, oL, Lv[1]);
@@ -1208,8 +1074,6 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
             stb<T>(rD, active ? (unsigned)(eb0 - tau * 64) * bB : kFar, res.a);
             prev = res;
             xch[wbuf][l] = res;
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            lds_st(sdone, tau + 1);
             __builtin_amdgcn_wave_barrier();
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             cntX = lds_ld(fX);
@@ -1223,12 +1087,12 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
 }
 
 template <class T, int PD>
-__global__ void __launch_bounds__(192) k_factor_ws(FwsArgs<T> a) {
+__global__ void __launch_bounds__(128) k_factor_ws(FwsArgs<T> a) {
     factor_ws_body<T, PD>(a);
 }
 
 template <class T, int PD>
-__global__ void __launch_bounds__(192) k_factor_ws_batch(const FwsArgs<T> *args, const int *tmem) {
+__global__ void __launch_bounds__(128) k_factor_ws_batch(const FwsArgs<T> *args, const int *tmem) {
     const FwsArgs<T> a = args[tmem[blockIdx.x]];
     factor_ws_body<T, PD>(a);
 }
@@ -1409,14 +1273,14 @@ static void wl_apply(const Grid &g, const WfSync &sync, const WlPrecond<T> &wl, 
     f.blk[0] = wl.L; f.blk[1] = wl.L + entries * 4; f.blk[2] = wl.L + 2 * entries * 4;
     f.Dinv = nullptr; f.w = wl.w; f.out = nullptr; f.status = status; f.sync = sync;
     if (!tickets_clear) hipLaunchKernelGGL(k_reset_ticket_pair, dim3(1), dim3(1), 0, s, sync.ticket);
-    hipLaunchKernelGGL((k_sweep_ws<T, true, kSweepPD>), dim3(sync.ntiles), dim3(192), 0, s, f);
+    hipLaunchKernelGGL((k_sweep_ws<T, true, kSweepPD>), dim3(sync.ntiles), dim3(128), 0, s, f);
     WsArgs<T> b = f;
     b.d = nullptr;
     b.blk[0] = wl.U; b.blk[1] = wl.U + entries * 4; b.blk[2] = wl.U + 2 * entries * 4;
     b.Dinv = wl.D;
     b.out = v;
     b.sync.ticket = sync.ticket + 1;
-    hipLaunchKernelGGL((k_sweep_ws<T, false, kSweepPD>), dim3(sync.ntiles), dim3(192), 0, s, b);
+    hipLaunchKernelGGL((k_sweep_ws<T, false, kSweepPD>), dim3(sync.ntiles), dim3(128), 0, s, b);
 }
 
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
@@ -1449,7 +1313,7 @@ static void wl_factor(const Grid &g, const WfSync &sync, const double *A, double
     hipLaunchKernelGGL(k_wl_pack_factor<T>, dim3((unsigned)(sync.ntiles * ((g.nz + 3) / 4))), dim3(256), 0,
                        s, f);
     hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
-    hipLaunchKernelGGL((k_factor_ws<T, kFactorPD>), dim3(sync.ntiles), dim3(192), 0, s, f);
+    hipLaunchKernelGGL((k_factor_ws<T, kFactorPD>), dim3(sync.ntiles), dim3(128), 0, s, f);
 }
 
 void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
@@ -1555,17 +1419,17 @@ void wl_batch_apply(void *h, hipStream_t s) {
     const dim3 rg((b->n + 255) / 256), rb(256);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_sweep_ws_batch<float, true, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, true, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
                            static_cast<const WsArgs<float> *>(b->fwd), b->tmem);
     else
-        hipLaunchKernelGGL((k_sweep_ws_batch<double, true, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, true, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
                            static_cast<const WsArgs<double> *>(b->fwd), b->tmem);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_sweep_ws_batch<float, false, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, false, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
                            static_cast<const WsArgs<float> *>(b->bwd), b->tmem);
     else
-        hipLaunchKernelGGL((k_sweep_ws_batch<double, false, kSweepPD>), dim3(b->ntiles), dim3(192), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, false, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
                            static_cast<const WsArgs<double> *>(b->bwd), b->tmem);
 }
 
@@ -1580,10 +1444,10 @@ void wl_batch_factor(void *h, hipStream_t s) {
                            static_cast<const FwsArgs<double> *>(b->fac), b->pmem, b->poff);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_factor_ws_batch<float, kFactorPD>), dim3(b->ntiles), dim3(192), 0, s,
+        hipLaunchKernelGGL((k_factor_ws_batch<float, kFactorPD>), dim3(b->ntiles), dim3(128), 0, s,
                            static_cast<const FwsArgs<float> *>(b->fac), b->tmem);
     else
-        hipLaunchKernelGGL((k_factor_ws_batch<double, kFactorPD>), dim3(b->ntiles), dim3(192), 0, s,
+        hipLaunchKernelGGL((k_factor_ws_batch<double, kFactorPD>), dim3(b->ntiles), dim3(128), 0, s,
                            static_cast<const FwsArgs<double> *>(b->fac), b->tmem);
 }
 
