@@ -229,13 +229,22 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     launch_bicg_init(n, b, c->kr, x, c->kp, c->kv, c->part0, c->part1, s);
     launch_ks_init(c->ks, c->part0, c->part1, c->prm.linear_tolerance, absTol,
                    c->prm.linear_max_error, s);
-    HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost, s));
+    // read back together with the first batch of iterations (one host wait fewer): the
+    // factorization's error flag and the initial status (the batch's kernels return at
+    // once when the initial residual already meets the criterion)
     HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
-    sample_collect(c);
+    HIP_TRY(c, hipMemcpyAsync(c->err_host + 1, &c->ks->status, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
     *iters = 0;
-    if (*c->err_host) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
-    if (c->ks_host->status == KS_CONVERGED) return EWG_OK;
+    if (c->prm.linear_max_iterations <= 0) {
+        HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
+                                  s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        sample_collect(c);
+        if (c->err_host[0]) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
+        if (c->err_host[1] == KS_CONVERGED) return EWG_OK;
+        return set_err(c, EWG_NOT_CONVERGED, "linear solver did not converge");
+    }
     const int *st = &c->ks->status;
     // the vector update before each preconditioner apply clears the layout sweeps'
     // tickets (one kernel launch fewer per sweep)
@@ -248,18 +257,19 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     for (int it = 0; it < c->prm.linear_max_iterations;) {
         const int nb = std::min(batch, c->prm.linear_max_iterations - it);
         for (int q = 0; q < nb; ++q) {
-            // the first iteration of a batch runs for certain (the state read back before
-            // it was KS_RUNNING): its first apply and SpMV are the sampled launches
+            // the first iteration of a later batch runs for certain (the state read back
+            // before it was KS_RUNNING): its first apply and SpMV are the sampled launches
+            const bool sampled = q == 0 && it > 0;
             launch_ks_beta(c->ks, c->part2, s);
             launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s, tk);
-            if (q == 0) sample_begin(c, EWG_KERNEL_ILU_APPLY);
+            if (sampled) sample_begin(c, EWG_KERNEL_ILU_APPLY);
             precond_apply(c, c->kp, c->ky, st, tk != nullptr);
-            if (q == 0) {
+            if (sampled) {
                 sample_end(c, EWG_KERNEL_ILU_APPLY);
                 sample_begin(c, EWG_KERNEL_SPMV);
             }
             launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
-            if (q == 0) sample_end(c, EWG_KERNEL_SPMV);
+            if (sampled) sample_end(c, EWG_KERNEL_SPMV);
             launch_ks_alpha(c->ks, c->part0, s);
             launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s, tk);
             launch_ks_crit1(c->ks, c->part1, c->partAny, s);
@@ -269,23 +279,33 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
             launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
             launch_ks_crit2(c->ks, c->partAny, s);
         }
+        const bool first = it == 0;
         it += nb;
         HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
                                   s));
+        HIP_TRY(c, hipMemcpyAsync(c->err_host + 2, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost,
+                                  s));
         HIP_TRY(c, hipStreamSynchronize(s));
         sample_collect(c);
+        if (first && c->err_host[0]) {
+            // the iterations after the failed factorization ran for nothing; a wavefront
+            // wait that ran out may have left words in the edge buffers
+            if (c->wf.ex) wf_init_edges(c->wf, s);
+            HIP_TRY(c, hipStreamSynchronize(s));
+            return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
+        }
+        if (first && c->err_host[1] == KS_CONVERGED) return EWG_OK;
+        if (c->err_host[2]) {
+            // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
+            if (c->wf.ex) wf_init_edges(c->wf, s);
+            HIP_TRY(c, hipStreamSynchronize(s));
+            *iters = c->ks_host->iterations;
+            return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
+        }
         if (c->ks_host->status != KS_RUNNING) break;
         batch = std::min(batch * 2, 8);
     }
     *iters = c->ks_host->iterations;
-    HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
-    if (*c->err_host) {
-        // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
-        if (c->wf.ex) wf_init_edges(c->wf, s);
-        HIP_TRY(c, hipStreamSynchronize(s));
-        return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
-    }
     switch (c->ks_host->status) {
         case KS_CONVERGED: return EWG_OK;
         case KS_BREAKDOWN: return set_err(c, EWG_NUMERICAL_ISSUE, "Breakdown of the BiCGStab solver");
@@ -337,6 +357,15 @@ int newton(ewg_ctx *c, double dt) {
     c->num_iterations = 0;
     const size_t vbytes = sizeof(double) * 2 * c->g.N;
     int rc;
+    // The finiteness check of an update is read back with the next host wait (the next
+    // residual norm, or the end of the loop) instead of a wait of its own.
+    bool update_pending = false;
+    auto update_failed = [&]() {
+        update_pending = false;
+        if (std::isfinite(c->scal_host[1])) return false;
+        c->num_iterations = 2 * c->prm.newton_target_iterations;
+        return true;
+    };
     while (proceed(c)) {
         c->last_error = c->error;  // beginIteration_
         HIP_TRY(c, hipMemcpyAsync(c->xcur, c->x0, vbytes, hipMemcpyDeviceToDevice, c->stream));
@@ -346,6 +375,8 @@ int newton(ewg_ctx *c, double dt) {
         c->last_error = c->error;  // preSolve_
         double e;
         if ((rc = device_max_abs(c, c->R, &e)) != EWG_OK) return rc;
+        if (update_pending && update_failed())
+            return set_err(c, EWG_NUMERICAL_ISSUE, "Non-finite update!");
         double t1 = now_s();
         c->stats.time_linearize_s += t1 - t0;
         c->error = e;
@@ -371,15 +402,15 @@ int newton(ewg_ctx *c, double dt) {
         }
         launch_newton_update(2 * c->g.N, c->xcur, c->kx, c->x0, c->part0, c->stream);
         launch_reduce_sum(c->part0, c->dscalar, c->stream);
-        HIP_TRY(c, hipMemcpyAsync(c->scal_host, c->dscalar, sizeof(double),
+        HIP_TRY(c, hipMemcpyAsync(c->scal_host + 1, c->dscalar, sizeof(double),
                                   hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        update_pending = true;
         c->stats.time_update_s += now_s() - t2;
-        if (!std::isfinite(c->scal_host[0])) {
-            c->num_iterations = 2 * c->prm.newton_target_iterations;
-            return set_err(c, EWG_NUMERICAL_ISSUE, "Non-finite update!");
-        }
         ++c->num_iterations;
+    }
+    if (update_pending) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (update_failed()) return set_err(c, EWG_NUMERICAL_ISSUE, "Non-finite update!");
     }
     if (!(c->error <= c->prm.newton_tolerance)) {
         c->num_iterations = 2 * c->prm.newton_target_iterations;
@@ -609,7 +640,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
         return fail("ks_host", e);
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->scal_host), 8 * sizeof(double))) != hipSuccess)
         return fail("scal_host", e);
-    if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->err_host), sizeof(int))) != hipSuccess)
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->err_host), 4 * sizeof(int))) != hipSuccess)
         return fail("err_host", e);
     // blocks of absent neighbours stay zero for the whole life of the context
     if ((e = hipMemset(c->J, 0, nb * sizeof(double))) != hipSuccess) return fail("J", e);
