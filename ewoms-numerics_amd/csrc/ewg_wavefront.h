@@ -73,7 +73,6 @@ void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double 
                const double *A, const WlPrecond<double> &wl, hipStream_t s);
 void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
                const double *A, const WlPrecond<float> &wl, hipStream_t s);
-// v = (LU)^-1 d with one compute, one edge and one publisher wavefront per tile
 // the factorization straight into the layout (FP64 arithmetic; FP32 rounds the stored
 // factors and upper blocks); entries without a cell keep what they hold (zeros). Af:
 // scratch for the operands in the forward layout, 7 x wl_entries x 4 doubles
@@ -81,9 +80,10 @@ void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
                  const WlPrecond<double> &wl, int *err, hipStream_t s);
 void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
                  const WlPrecond<float> &wl, int *err, hipStream_t s);
-// The forward sweep takes tickets from sync.ticket[0], the backward sweep from
-// sync.ticket[1]; tickets_clear: the caller's previous kernel on the stream has set both
-// to zero (otherwise a small kernel does).
+// v = (LU)^-1 d with one compute and one edge wavefront per tile. The forward sweep takes
+// tickets from sync.ticket[0], the backward sweep from sync.ticket[1]; tickets_clear:
+// the caller's previous kernel on the stream has set both to zero (otherwise a small
+// kernel does).
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
                 double *v, const int *status, hipStream_t s, bool tickets_clear = false);
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
