@@ -181,7 +181,8 @@ void precond_factor(ewg_ctx *c) {
     if (fp32) launch_round_factors(c->g, c->L64, c->D64, c->J, c->L32, c->D32, c->U32, c->stream);
 }
 
-// tickets_clear: the kernel before on the stream has cleared the layout sweeps' tickets
+// tickets_clear: a kernel queued on the stream since the last apply has cleared the
+// layout sweeps' tickets
 void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status,
                    bool tickets_clear) {
     if (use_wavefront_layout(c)) {

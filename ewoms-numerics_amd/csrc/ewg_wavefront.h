@@ -82,8 +82,8 @@ void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
                  const WlPrecond<float> &wl, int *err, hipStream_t s);
 // v = (LU)^-1 d with one compute and one edge wavefront per tile. The forward sweep takes
 // tickets from sync.ticket[0], the backward sweep from sync.ticket[1]; tickets_clear:
-// the caller's previous kernel on the stream has set both to zero (otherwise a small
-// kernel does).
+// a kernel queued on the stream after the last sweep that took tickets has set both to
+// zero (otherwise a small kernel in front of the forward sweep does).
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
                 double *v, const int *status, hipStream_t s, bool tickets_clear = false);
 void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, const double *d,
