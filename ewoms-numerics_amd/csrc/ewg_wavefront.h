@@ -22,8 +22,10 @@ constexpr int kWfMaxWords = 4;  // 64-bit words per exchanged value (FP64 2x2 bl
 // cell and array, plus a skew margin and the out-of-range base 2^31) must not wrap
 constexpr long long kWfMaxCells = 1LL << 25;
 
-// synchronisation state: the ticket counter is reset by a one-thread kernel in front of
-// every launch; the edge buffers hold the sentinel between launches
+// synchronisation state: each launch takes tile tickets from a counter that starts at
+// zero (reset by a one-thread kernel in front of the launch, or by the BiCGStab vector
+// update queued before a preconditioner apply); the edge buffers hold the sentinel
+// between launches
 struct WfSync {
     unsigned *ticket;          // two counters (the warp-specialized apply uses both)
     const int2 *order;         // [ntiles] tiles (tx, ty) in anti-diagonal order
