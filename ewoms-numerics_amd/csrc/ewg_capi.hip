@@ -59,6 +59,8 @@ void free_all(ewg_ctx *c) {
         if (sm.e0) (void)hipEventDestroy(sm.e0);
         if (sm.e1) (void)hipEventDestroy(sm.e1);
     }
+    if (c->warm_e0) (void)hipEventDestroy(c->warm_e0);
+    if (c->warm_e1) (void)hipEventDestroy(c->warm_e1);
     if (c->ks_host) (void)hipHostFree(c->ks_host);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
     if (c->err_host) (void)hipHostFree(c->err_host);
@@ -423,13 +425,33 @@ int newton(ewg_ctx *c, double dt) {
 // One Newton solve of a time step. With a warm start, the solve starts from
 // solution(1) + the network's prediction; if it fails to converge from there, it is redone
 // from the plain guess solution(1).
+// Duration of the last prediction (device clock between the events around it), 0 when
+// none is pending. Its events have completed once a Newton solve after it has returned.
+double warm_pending_s(const ewg_ctx *c) {
+    if (!c->warm_pending) return 0.0;
+    float ms = 0.0f;
+    if (hipEventSynchronize(c->warm_e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, c->warm_e0, c->warm_e1) != hipSuccess)
+        return 0.0;
+    return 1e-3 * ms;
+}
+
 int newton_step(ewg_ctx *c, double dt) {
     if (!(c->cnn && c->prm.warm_start)) return newton(c, dt);
-    double t0 = now_s();
+    // the prediction is timed by events on the stream, so that the solve's launches
+    // queue behind it without a wait (the host-timed first linearization then also
+    // covers the end of the prediction)
+    c->warm_predict_s += warm_pending_s(c);
+    c->warm_pending = false;
+    if (!c->warm_e0) {
+        HIP_TRY(c, hipEventCreate(&c->warm_e0));
+        HIP_TRY(c, hipEventCreate(&c->warm_e1));
+    }
+    HIP_TRY(c, hipEventRecord(c->warm_e0, c->stream));
     int rc = cnn::predict_guess(c->cnn, c->x1, c->K, c->phi, dt, c->x0, c->stream, c->src_box);
     if (rc != EWG_OK) return set_err(c, rc, "warm start: prediction failed");
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    c->warm_predict_s += now_s() - t0;
+    HIP_TRY(c, hipEventRecord(c->warm_e1, c->stream));
+    c->warm_pending = true;
     c->no_recycle = true;
     rc = newton(c, dt);
     c->no_recycle = false;
@@ -944,6 +966,7 @@ int ewg_reset_stats(ewg_ctx *c) {
     }
     c->warm_accepted = c->warm_rejected = 0;
     c->warm_predict_s = 0.0;
+    c->warm_pending = false;
     return EWG_OK;
 }
 
@@ -1033,7 +1056,7 @@ int ewg_warm_start_stats(const ewg_ctx *c, int64_t *accepted, int64_t *rejected,
     if (last_initial_error) *last_initial_error = c->first_error;
     if (accepted) *accepted = c->warm_accepted;
     if (rejected) *rejected = c->warm_rejected;
-    if (predict_s) *predict_s = c->warm_predict_s;
+    if (predict_s) *predict_s = c->warm_predict_s + warm_pending_s(c);
     return EWG_OK;
 }
 

@@ -88,7 +88,10 @@ struct ewg_ctx {
     ewg_cnn *cnn = nullptr;
     bool no_recycle = false;   // storage of the previous level from solution(1)
     int64_t warm_accepted = 0, warm_rejected = 0;
-    double warm_predict_s = 0.0;
+    double warm_predict_s = 0.0;  // predictions whose events have been read
+    // events around the last prediction on the stream; pending: not yet in warm_predict_s
+    hipEvent_t warm_e0 = nullptr, warm_e1 = nullptr;
+    bool warm_pending = false;
     double first_error = 0.0;  // Newton error at the first linearization of the last solve
     std::string err;
 };

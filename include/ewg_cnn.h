@@ -114,9 +114,9 @@ int ewg_set_warm_start(ewg_ctx *ctx, ewg_cnn *net);
 int64_t ewg_cnn_last_inference_cells(const ewg_cnn *net);
 
 /* Steps that started from the prediction and converged (accepted), steps redone from the
- * plain guess (rejected), the time of the predictions (s, host clock around the
- * synchronised launches), and the Newton error (max-norm of the residual) at the first
- * linearization of the last Newton solve, warm or plain. */
+ * plain guess (rejected), the time of the predictions (s, device clock between events
+ * recorded around each one on the context's stream), and the Newton error (max-norm of
+ * the residual) at the first linearization of the last Newton solve, warm or plain. */
 int ewg_warm_start_stats(const ewg_ctx *ctx, int64_t *accepted, int64_t *rejected,
                          double *predict_s, double *last_initial_error);
 
