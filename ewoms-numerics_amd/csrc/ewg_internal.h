@@ -18,6 +18,14 @@
 
 namespace ewg {
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8), each XCD with
+// its own L2. Renumbered so that XCD k takes the k-th eighth of the logical blocks in
+// order, neighbouring blocks, which share cache lines, run on one XCD (with nb not a
+// multiple of 8: no renumbering).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
 struct Grid {
     int nx, ny, nz, dim;
     int N, nxny;

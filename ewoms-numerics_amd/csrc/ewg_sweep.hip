@@ -1165,17 +1165,11 @@ __device__ __forceinline__ int pack_cell_of(const FwsArgs<T> &a, int b, int t) {
     return x < a.nx && y < a.ny && z < a.nz ? x + a.nx * (y + a.ny * z) : a.N;
 }
 
-// Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8). Renumbered so
-// that XCD k takes the k-th eighth of the chunks in order, the z-chunks of a tile column,
-// which write parts of the same rows of entries, run together on one XCD and their
-// partial cache lines meet in its L2 (with nb not a multiple of 8: no renumbering).
-__device__ __forceinline__ int xcd_chunk(int b, int nb) {
-    return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
-}
-
+// the z-chunks of a tile column write parts of the same rows of entries: renumbered for
+// the XCDs, they run together on one XCD and their partial cache lines meet in its L2
 template <class T>
 __global__ void __launch_bounds__(256) k_wl_pack_factor(FwsArgs<T> a) {
-    pack_factor_cell<T>(a, pack_cell_of(a, xcd_chunk(blockIdx.x, gridDim.x), threadIdx.x));
+    pack_factor_cell<T>(a, pack_cell_of(a, xcd_block(blockIdx.x, gridDim.x), threadIdx.x));
 }
 
 // several realizations: block -> realization by pmem, its first block at poff
