@@ -1172,14 +1172,16 @@ __global__ void __launch_bounds__(256) k_wl_pack_factor(FwsArgs<T> a) {
     pack_factor_cell<T>(a, pack_cell_of(a, xcd_block(blockIdx.x, gridDim.x), threadIdx.x));
 }
 
-// several realizations: block -> realization by pmem, its first block at poff
+// several realizations: block -> realization by pmem, its first block at poff (blocks
+// renumbered for the XCDs over the whole grid, as in the single launch)
 template <class T>
 __global__ void __launch_bounds__(256)
 k_wl_pack_factor_batch(const FwsArgs<T> *args, const int *pmem, const int *poff) {
-    const int m = pmem[blockIdx.x];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int m = pmem[b];
     const FwsArgs<T> &a = args[m];
     if (a.active && *a.active == 0) return;
-    pack_factor_cell<T>(a, pack_cell_of(a, (int)(blockIdx.x - poff[m]), threadIdx.x));
+    pack_factor_cell<T>(a, pack_cell_of(a, b - poff[m], threadIdx.x));
 }
 }  // namespace
 
