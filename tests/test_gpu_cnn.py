@@ -148,6 +148,11 @@ def test_zero_network_reproduces_the_plain_newton_path():
     assert np.array_equal(plain.solution(0), warm.solution(0))
     st = cnn.warm_stats(warm)
     assert st["accepted"] == 4 and st["rejected"] == 0
+    # the four predictions are timed by events around them on the stream; a reset clears
+    # the total, including the last prediction's pending time
+    assert 0.0 < st["predict_s"] < 10.0
+    warm.reset_stats()
+    assert cnn.warm_stats(warm)["predict_s"] == 0.0
 
 
 def test_rejected_prediction_falls_back_to_the_plain_guess():
