@@ -229,6 +229,7 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     sample_begin(c, EWG_KERNEL_ILU_FACTOR);
     precond_factor(c);
     sample_end(c, EWG_KERNEL_ILU_FACTOR);
+    c->have_precond = true;
     launch_bicg_init(n, b, c->kr, x, c->kp, c->kv, c->part0, c->part1, s);
     launch_ks_init(c->ks, c->part0, c->part1, c->prm.linear_tolerance, absTol,
                    c->prm.linear_max_error, s);
@@ -464,6 +465,54 @@ int newton_step(ewg_ctx *c, double dt) {
     HIP_TRY(c, hipMemcpyAsync(c->x0, c->x1, 2 * (size_t)c->g.N * sizeof(double),
                               hipMemcpyDeviceToDevice, c->stream));
     return newton(c, dt);
+}
+
+// Entry of cell (x, y, z) in the wavefront layout of the 8x8 tiles (the inverse of the
+// mapping the packing and factorization kernels write: lane lx + 8 ly of tile
+// tx + ntx ty reaches z at local step z + lx + ly). The backward layout holds cell
+// (x, y, z) at the forward entry of the mirrored cell.
+size_t wl_entry(const Grid &g, const WfSync &w, int x, int y, int z) {
+    const int lx = x & 7, ly = y & 7;
+    const size_t tile = (size_t)(x >> 3) + (size_t)w.ntx * (y >> 3);
+    return (tile * (size_t)(g.nz + 14) + (size_t)(z + lx + ly)) * 64 + (size_t)(lx + 8 * ly);
+}
+
+// The factors the sweeps read, widened to double, as natural-order slot arrays:
+// L[dim][N][4] (lower slots in ascending column order), Dinv[N][4], U[dim][N][4] (upper
+// slots in ascending column order).
+template <class T>
+int gather_layout(ewg_ctx *c, const WlPrecond<T> &wl, std::vector<double> &L,
+                  std::vector<double> &D, std::vector<double> &U) {
+    const Grid &g = c->g;
+    const size_t E = wl_entries(g, c->wf), N = g.N;
+    std::vector<T> hL(3 * E * 4), hU(3 * E * 4), hD(E * 4);
+    HIP_TRY(c, hipMemcpy(hL.data(), wl.L, hL.size() * sizeof(T), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(hU.data(), wl.U, hU.size() * sizeof(T), hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(hD.data(), wl.D, hD.size() * sizeof(T), hipMemcpyDeviceToHost));
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.ny; ++y)
+            for (int x = 0; x < g.nx; ++x) {
+                const size_t i = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+                const size_t pf = wl_entry(g, c->wf, x, y, z);
+                const size_t pb = wl_entry(g, c->wf, g.nx - 1 - x, g.ny - 1 - y, g.nz - 1 - z);
+                for (int e = 0; e < 4; ++e) {
+                    for (int s = 0; s < 3; ++s) {
+                        L[(s * N + i) * 4 + e] = (double)hL[(s * E + pf) * 4 + e];
+                        U[(s * N + i) * 4 + e] = (double)hU[(s * E + pb) * 4 + e];
+                    }
+                    D[i * 4 + e] = (double)hD[pb * 4 + e];
+                }
+            }
+    return EWG_OK;
+}
+
+template <class T>
+int gather_natural(ewg_ctx *c, const T *dL, const T *dD, std::vector<double> &v, size_t off,
+                   size_t count) {
+    std::vector<T> h(count);
+    HIP_TRY(c, hipMemcpy(h.data(), dL ? dL : dD, count * sizeof(T), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < count; ++k) v[off + k] = (double)h[k];
+    return EWG_OK;
 }
 
 double suggest_dt(const ewg_ctx *c, double oldDt) {  // newtonmethod.hh:527-544
@@ -888,6 +937,81 @@ int ewg_solve(ewg_ctx *c, double *dx, int32_t *iterations) {
         HIP_TRY(c, hipMemcpy(dx, c->kx, 2 * (size_t)c->g.N * sizeof(double), hipMemcpyDeviceToHost));
     }
     return rc;
+}
+
+int ewg_get_preconditioner(ewg_ctx *c, double *lu) {
+    if (!c || !lu) return EWG_INVALID_ARGUMENT;
+    if (!c->have_system) return set_err(c, EWG_INVALID_STATE, "no linearization");
+    hipStream_t s = c->stream;
+    HIP_TRY(c, hipMemsetAsync(c->ilu_err, 0, sizeof(int), s));
+    precond_factor(c);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    c->have_precond = false;
+    if (c->err_host[0]) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
+    c->have_precond = true;
+    const Grid &g = c->g;
+    const size_t N = g.N, nd = (size_t)g.dim;
+    std::vector<double> L(nd * N * 4), D(N * 4), U(nd * N * 4);
+    int rc = EWG_OK;
+    const bool fp32 = c->prm.precond_fp32;
+    if (use_wavefront_layout(c)) {
+        rc = fp32 ? gather_layout(c, c->wl32, L, D, U) : gather_layout(c, c->wl64, L, D, U);
+    } else if (fp32) {
+        if ((rc = gather_natural<float>(c, c->L32, nullptr, L, 0, L.size())) == EWG_OK &&
+            (rc = gather_natural<float>(c, nullptr, c->D32, D, 0, D.size())) == EWG_OK)
+            rc = gather_natural<float>(c, c->U32, nullptr, U, 0, U.size());
+    } else {
+        if ((rc = gather_natural<double>(c, c->L64, nullptr, L, 0, L.size())) == EWG_OK &&
+            (rc = gather_natural<double>(c, nullptr, c->D64, D, 0, D.size())) == EWG_OK)
+            rc = gather_natural<double>(c, c->J + (size_t)(g.diag + 1) * N * 4, nullptr, U, 0,
+                                        U.size());
+    }
+    if (rc != EWG_OK) return rc;
+    for (size_t b = 0; b < c->bcrs_src.size(); ++b) {
+        const size_t slot = (size_t)(c->bcrs_src[b] / (int64_t)N);
+        const size_t i = (size_t)(c->bcrs_src[b] % (int64_t)N);
+        const double *src = slot < (size_t)g.diag ? &L[(slot * N + i) * 4]
+                            : slot == (size_t)g.diag ? &D[i * 4]
+                                                     : &U[((slot - g.diag - 1) * N + i) * 4];
+        std::memcpy(lu + 4 * b, src, 4 * sizeof(double));
+    }
+    return EWG_OK;
+}
+
+int ewg_apply_preconditioner(ewg_ctx *c, const double *d, double *v) {
+    if (!c || !d || !v) return EWG_INVALID_ARGUMENT;
+    if (!c->have_precond) return set_err(c, EWG_INVALID_STATE, "no preconditioner");
+    hipStream_t s = c->stream;
+    const size_t bytes = 2 * (size_t)c->g.N * sizeof(double);
+    HIP_TRY(c, hipMemcpyAsync(c->kp, d, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemsetAsync(c->ilu_err, 0, sizeof(int), s));
+    precond_apply(c, c->kp, c->ky, nullptr, false);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (c->err_host[0]) {
+        // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
+        if (c->wf.ex) wf_init_edges(c->wf, s);
+        HIP_TRY(c, hipStreamSynchronize(s));
+        return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
+    }
+    HIP_TRY(c, hipMemcpy(v, c->ky, bytes, hipMemcpyDeviceToHost));
+    return EWG_OK;
+}
+
+int ewg_apply_operator(ewg_ctx *c, const double *x, double *y) {
+    if (!c || !x || !y) return EWG_INVALID_ARGUMENT;
+    if (!c->have_system) return set_err(c, EWG_INVALID_STATE, "no linearization");
+    hipStream_t s = c->stream;
+    const size_t bytes = 2 * (size_t)c->g.N * sizeof(double);
+    HIP_TRY(c, hipMemcpyAsync(c->kp, x, bytes, hipMemcpyHostToDevice, s));
+    launch_spmv(c->g, c->J, c->kp, c->ky, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(s));
+    HIP_TRY(c, hipMemcpy(y, c->ky, bytes, hipMemcpyDeviceToHost));
+    return EWG_OK;
 }
 
 int ewg_newton(ewg_ctx *c, double dt, int32_t *iterations, double *final_error) {

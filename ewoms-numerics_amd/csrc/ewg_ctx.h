@@ -83,6 +83,8 @@ struct ewg_ctx {
     int last_linear_iterations = 0;
     bool static_set = false;
     bool have_system = false;
+    bool have_precond = false;   // factors of the current Jacobian exist (a solve or
+                                 // ewg_get_preconditioner made them)
     ewg_stats stats{};
     // learned warm start (params.warm_start with an attached network)
     ewg_cnn *cnn = nullptr;

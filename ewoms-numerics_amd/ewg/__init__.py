@@ -38,6 +38,13 @@ def load():
         lib.ewg_kernel_samples.argtypes = [vp, C.c_int, C.POINTER(C.c_int64),
                                            C.POINTER(C.c_double)]
         lib.ewg_device_info.restype = C.c_int
+        _D = C.POINTER(C.c_double)
+        for name, args in (("get_preconditioner", [vp, _D]),
+                           ("apply_preconditioner", [vp, _D, _D]),
+                           ("apply_operator", [vp, _D, _D])):
+            fn = getattr(lib, "ewg_" + name)
+            fn.restype = C.c_int
+            fn.argtypes = args
         lib.ewg_device_info.argtypes = [C.c_char_p, C.c_int]
         _LIB = lib
     return _LIB
@@ -56,6 +63,29 @@ class GpuContext(Context):
 
     def reset_stats(self):
         self._check(self.lib.ewg_reset_stats(self.h))
+
+    def preconditioner(self):
+        """ILU(0) factors of the current Jacobian in BCRS order (ewg_get_preconditioner)."""
+        import numpy as np
+        lu = np.empty(4 * self.nnzb)
+        self._check(self.lib.ewg_get_preconditioner(self.h, lu.ctypes.data_as(C.POINTER(C.c_double))))
+        return lu
+
+    def apply_preconditioner(self, d):
+        import numpy as np
+        d = np.ascontiguousarray(d, dtype=np.float64).ravel()
+        v = np.empty(2 * self.N)
+        P = C.POINTER(C.c_double)
+        self._check(self.lib.ewg_apply_preconditioner(self.h, d.ctypes.data_as(P), v.ctypes.data_as(P)))
+        return v
+
+    def apply_operator(self, x):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.empty(2 * self.N)
+        P = C.POINTER(C.c_double)
+        self._check(self.lib.ewg_apply_operator(self.h, x.ctypes.data_as(P), y.ctypes.data_as(P)))
+        return y
 
     def kernel_samples(self, which: int):
         """(count, total_ms) of the event-timed launches of kernel `which` (needs

@@ -197,6 +197,24 @@ int ewg_set_system(ewg_ctx *ctx, const double *resid, const double *jac);
  * breakdown. dx (2N doubles) is an optional host copy; iterations may be NULL. */
 int ewg_solve(ewg_ctx *ctx, double *dx, int32_t *iterations);
 
+/* PreconditionerWrapperILU::prepare (ewoms/numerics/linear/istlpreconditionerwrappers.hh:177-183,
+ * which builds Dune::SeqILU<.., 0> on the solver's matrix): factorizes the Jacobian of the
+ * last linearization (or ewg_set_system) exactly as ewg_solve does (FP64, or FP32 factors
+ * with params.precond_fp32) and returns the factors in the BCRS order of ewg_get_pattern:
+ * nnzb*4 doubles, L_ij = A_ij inv(D_j) at the lower blocks, inv(D_i) at the diagonal and
+ * the upper blocks U_ij = A_ij as the sweeps read them (FP32 factors widened exactly). */
+int ewg_get_preconditioner(ewg_ctx *ctx, double *lu);
+
+/* SeqILU::apply as the BiCGStab solver calls it through its preconditioner
+ * (ewoms/numerics/linear/bicgstabsolver.hh:221, :276): v = (LU)^-1 d with the factors of the last factorization
+ * (a solve or ewg_get_preconditioner); d and v are 2N doubles, AoS. */
+int ewg_apply_preconditioner(ewg_ctx *ctx, const double *d, double *v);
+
+/* OverlappingOperator::apply (ewoms/numerics/linear/overlappingoperator.hh:59-63),
+ * i.e. BCRSMatrix::mv: y = J x with the Jacobian of the last linearization; x and y are
+ * 2N doubles, AoS. */
+int ewg_apply_operator(ewg_ctx *ctx, const double *x, double *y);
+
 /* NewtonMethod::apply (nonlinear/newtonmethod.hh:308-517) for one time step of size
  * dt on the device, starting from solution(0). Returns 0 when converged,
  * EWG_NOT_CONVERGED / EWG_NUMERICAL_ISSUE on failure (solution(0) then holds the

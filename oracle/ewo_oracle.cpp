@@ -1155,6 +1155,17 @@ int ewo_ilu0_apply(const ewo_ctx *c, const double *LU, const double *d, double *
     return EWG_OK;
 }
 
+// The FP32 preconditioner's arithmetic: factors and right-hand side rounded to float,
+// both sweeps in float, the result widened (the form libewg runs with precond_fp32).
+int ewo_ilu0_apply32(const ewo_ctx *c, const float *LU, const double *d, double *v) {
+    if (!c || !LU || !d || !v) return EWG_INVALID_ARGUMENT;
+    std::vector<float> df(2 * c->N), vf(2 * c->N);
+    for (int64_t k = 0; k < 2 * c->N; ++k) df[k] = static_cast<float>(d[k]);
+    ilu0_apply(*c, LU, df.data(), vf.data());
+    for (int64_t k = 0; k < 2 * c->N; ++k) v[k] = static_cast<double>(vf[k]);
+    return EWG_OK;
+}
+
 int ewo_bicgstab(ewo_ctx *c, const double *A, const double *b, double *x, int32_t *iterations) {
     if (!c || !A || !b || !x) return EWG_INVALID_ARGUMENT;
     std::vector<double> bv(b, b + 2 * c->N), xv;

@@ -46,6 +46,8 @@ int ewo_storage(const ewo_ctx *ctx, const double *x, double *M);
 int ewo_spmv(const ewo_ctx *ctx, const double *A, const double *x, double *y);
 int ewo_ilu0_factor(const ewo_ctx *ctx, const double *A, double *LU);
 int ewo_ilu0_apply(const ewo_ctx *ctx, const double *LU, const double *d, double *v);
+/* the same apply in float: LU and d rounded to float, the result widened */
+int ewo_ilu0_apply32(const ewo_ctx *ctx, const float *LU, const double *d, double *v);
 int ewo_bicgstab(ewo_ctx *ctx, const double *A, const double *b, double *x,
                  int32_t *iterations);
 

@@ -45,6 +45,8 @@ def load():
             fn = getattr(lib, "ewo_" + name)
             fn.restype = C.c_int
             fn.argtypes = args
+        lib.ewo_ilu0_apply32.restype = C.c_int
+        lib.ewo_ilu0_apply32.argtypes = [vp, C.POINTER(C.c_float), _D, _D]
         lib.ewo_step_log.restype = C.c_int
         lib.ewo_step_log.argtypes = [vp, _I, _D, C.c_int32]
         lib.ewo_newton_history.restype = C.c_int
@@ -82,6 +84,13 @@ class OracleContext(Context):
     def ilu0_apply(self, LU, d):
         v = np.empty(2 * self.N)
         self._check(self.lib.ewo_ilu0_apply(self.h, _p(LU), _p(np.ascontiguousarray(d)), _p(v)))
+        return v
+
+    def ilu0_apply32(self, LU32, d):
+        LU32 = np.ascontiguousarray(LU32, dtype=np.float32)
+        v = np.empty(2 * self.N)
+        self._check(self.lib.ewo_ilu0_apply32(self.h, LU32.ctypes.data_as(C.POINTER(C.c_float)),
+                                              _p(np.ascontiguousarray(d)), _p(v)))
         return v
 
     def bicgstab(self, A, b):
