@@ -25,8 +25,13 @@ W warm-up steps run first, the state is reset to the initial condition, and then
 K steps from t = 0 are timed with the state resident in HBM (default K = 40, the whole
 configured run). value = cells*K/seconds.
 
-The CPU baseline is the C++ oracle (single thread, the reference's default
-ThreadsPerProcess = 1) timed on a bounded sample of the same workload on this host.
+The CPU baseline is the C++ oracle timed on a bounded sample of the same workload on
+this host (the first --cpu-sample-steps steps from t = 0): single thread, the reference's
+default ThreadsPerProcess = 1, and with the linearization's element loop over every usable
+core; the headline form's GPU rate over the same first steps is reported beside it. The
+CPU model and core counts are recorded.
+
+--kernel NAME times one kernel alone (for profiler and counter runs).
 """
 from __future__ import annotations
 
@@ -43,6 +48,7 @@ for _p in (ROOT, os.path.join(ROOT, "ewoms-numerics_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+EWG_COPY_BYTES = 1 << 30   # include/ewg_capi.h: the copy probe moves this many bytes each way
 
 
 def algorithmic_bytes(P, nnzb, fp32_precond=False):
@@ -115,6 +121,11 @@ FORM_DETAIL = {
                      "update), judged by the unchanged Newton test with fallback; FP64 state, "
                      "residual, Newton norms, Krylov recurrences and update",
 }
+# the arithmetic each form computes in: FP64 state, residual, Newton norms, Krylov
+# recurrences and update everywhere; FP32 ILU(0) factors and sweeps, FP32 face fluxes and
+# the BF16 network (FP32 accumulation) where the form uses them
+DTYPE = {"fp64": "f64", "fp32_precond": "f64+f32", "fp32_flux": "f64+f32",
+         "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16"}
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 CNN_FLOP_PER_CELL = 2 * 88992  # multiply-adds of the five 3x3x3 convolutions, x2
 
@@ -220,9 +231,28 @@ def warm_start_network(P, args):
     return net, info
 
 
-def cpu_baseline(P, sample_steps):
+def host_cpu():
+    """CPU model (from /proc/cpuinfo) and the cores this process may use."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return model, usable, os.cpu_count() or usable
+
+
+def _oracle_steps(P, sample_steps, threads):
     from oracle.pyoracle import OracleContext
     o = OracleContext(P)
+    o.set_threads(threads)
     t0 = time.perf_counter()
     for _ in range(sample_steps):
         rc, its, err = o.newton(P.dt)
@@ -231,12 +261,27 @@ def cpu_baseline(P, sample_steps):
         o.advance_time_level()
     el = time.perf_counter() - t0
     s = o.stats()
-    return {"value": P.N * sample_steps / el, "unit": "cell-timesteps/s", "cores": 1,
-            "kind": "port",
-            "sample": (f"first {sample_steps} time step(s) of the same 128x128x64 case "
-                       f"({s['newton_iterations']} linearizations, "
-                       f"{s['linear_iterations']} BiCGStab iterations) in {el:.1f} s"),
-            "cpu": platform.processor() or platform.machine()}
+    o.close()
+    return P.N * sample_steps / el, el, s
+
+
+def cpu_baseline(P, sample_steps, gpu_same_sample=None):
+    """The oracle on this host over the first `sample_steps` steps of the same case:
+    (i) one thread, the reference default ThreadsPerProcess = 1; (ii) the element loop of
+    the linearization over every usable core (ThreadsPerProcess = nproc; the linear
+    solver stays serial, as in the reference)."""
+    model, usable, total = host_cpu()
+    v1, el1, s = _oracle_steps(P, sample_steps, 1)
+    vn, eln, _ = _oracle_steps(P, sample_steps, usable)
+    out = {"value": v1, "unit": "cell-timesteps/s", "cores": 1, "kind": "port",
+           "sample": (f"first {sample_steps} time step(s) of the same 128x128x64 case "
+                      f"({s['newton_iterations']} linearizations, "
+                      f"{s['linear_iterations']} BiCGStab iterations) in {el1:.1f} s"),
+           "cpu_model": model, "nproc": total, "usable_cores": usable,
+           "threaded_linearization": {"value": vn, "cores": usable, "seconds": round(eln, 2)}}
+    if gpu_same_sample is not None:
+        out["gpu_same_sample"] = gpu_same_sample
+    return out
 
 
 def main():
@@ -256,11 +301,17 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-reps", type=int, default=10)
+    ap.add_argument("--kernel", choices=["linearize", "spmv", "ilu_apply", "ilu_factor", "copy"],
+                    help="time one kernel only (the first form of --forms) and exit")
+    ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.kernel:
+        kernel_mode(args)
+        return
 
     import ewg
     from ewg import problems
@@ -287,7 +338,12 @@ def main():
             from ewg import cnn
             cnn.attach(ctx, net)
         n, el, st = timed_run(ctx, P, args.warmup, args.steps)
+        # the same form over the CPU baseline's sample (the first steps from t = 0)
+        ctx.set_solution(P.x0, 0)
+        ctx.set_solution(P.x0, 1)
+        nf, elf, _ = timed_run(ctx, P, 0, args.cpu_sample_steps)
         forms[form] = {"value": P.N * n / el, "steps": n, "seconds": el,
+                       "first_steps": {"steps": nf, "seconds": elf, "value": P.N * nf / elf},
                        "newton_iterations": st["newton_iterations"],
                        "linear_iterations": st["linear_iterations"],
                        "linear_solves": st["linear_solves"],
@@ -320,8 +376,9 @@ def main():
                                          "network": net_info}
         print(f"# {form} kernels {ks}", file=sys.stderr, flush=True)
         if form == "fp64":
+            ctx.time_kernel(EWG_KERNEL_COPY, 1)
             ms = ctx.time_kernel(EWG_KERNEL_COPY, args.copy_reps)
-            copy_bytes = 2 * 8 * 4 * ctx.nnzb
+            copy_bytes = 2 * EWG_COPY_BYTES
             copy_gbs = copy_bytes / (ms * 1e-3) / 1e9
             print(f"# copy {copy_bytes} bytes {ms:.4f} ms -> {copy_gbs:.0f} GB/s",
                   file=sys.stderr, flush=True)
@@ -365,7 +422,8 @@ def main():
                     "traffic": pmc_traffic(dom_name if prec == "fp64" else f"{dom_name}_fp32")}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(P, args.cpu_sample_steps)
+        cpu = cpu_baseline(P, args.cpu_sample_steps,
+                           {"form": head_form, **head["first_steps"]})
     line = {
         "metric": "cell-timesteps/s to Newton tol 1e-6 (learned+FP32-precond vs plain FP64); HBM GB/s",
         "value": head["value"],
@@ -377,7 +435,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": DTYPE[head_form],
         "data": "synthetic (seeded log-normal permeability, spectral synthesis)",
         "config": {"workload": f"3D heterogeneous two-phase DNAPL injection {args.nx}x{args.ny}x{args.nz}"
                                f" cells, dt {P.dt:g} s, Newton tol 1e-6, BiCGStab+ILU0",
@@ -394,7 +452,40 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
+    # one compact closing line on stderr (the tail a log keeps)
+    summ = [f"{f} {forms[f]['value'] / 1e6:.1f}M" for f in forms]
+    dev = forms.get(head_form, {}).get("vs_fp64")
+    print(f"# summary: {', '.join(summ)} cell-timesteps/s; {head_form} / fp64 = "
+          f"{line['speedup_vs_plain_fp64']}"
+          + (f"; max |dS| vs fp64 {max(dev['max_dS']):.2e} (one step "
+             f"{max(dev['max_dS_one_step']):.2e})" if dev else "")
+          + (f"; roofline {roofline['kernel']} {roofline['frac']:.3f}" if roofline else "")
+          + (f"; copy {line['copy_GBs']} GB/s" if line["copy_GBs"] else ""),
+          file=sys.stderr, flush=True)
+
+
+def kernel_mode(args):
+    """One kernel of the 128x128x64 case launched `reps` times at the state after two time
+    steps of the chosen form (for profiler and counter runs)."""
+    import ewg
+    from ewg import problems
+    from ewg.capi import (EWG_KERNEL_COPY, EWG_KERNEL_ILU_APPLY, EWG_KERNEL_ILU_FACTOR,
+                          EWG_KERNEL_LINEARIZE, EWG_KERNEL_SPMV)
+    kid = {"linearize": EWG_KERNEL_LINEARIZE, "spmv": EWG_KERNEL_SPMV,
+           "ilu_apply": EWG_KERNEL_ILU_APPLY, "ilu_factor": EWG_KERNEL_ILU_FACTOR,
+           "copy": EWG_KERNEL_COPY}[args.kernel]
+    form = args.forms.split(",")[0]
+    P = problems.heterogeneous_3d(args.nx, args.ny, args.nz)
+    ctx = ewg.GpuContext(P, form_params(form, max_time_step_size=P.dt))
+    ctx.run(0.0, 2 * P.dt, P.dt)
+    ctx.linearize(P.dt, 1, want=False)
+    ctx.time_kernel(kid, 1)
+    ms = ctx.time_kernel(kid, args.reps)
+    nbytes = (2 * EWG_COPY_BYTES if args.kernel == "copy" else
+              algorithmic_bytes(P, ctx.nnzb, bool(FORMS[form].get("precond_fp32")))[args.kernel])
+    print(json.dumps({"kernel": args.kernel, "form": form, "reps": args.reps, "mean_ms": ms,
+                      "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
 if __name__ == "__main__":

@@ -68,7 +68,7 @@ struct ewg_ctx {
         int64_t count = 0;
         double total_ms = 0.0;
     } samples[4];
-    double *copy_dst = nullptr;  // destination of the bandwidth-reference copy
+    double *copy_src = nullptr, *copy_dst = nullptr;  // the bandwidth-reference copy's buffers
     KrylovState *ks = nullptr;
     // pinned host mirrors
     KrylovState *ks_host = nullptr;

@@ -131,7 +131,7 @@ void launch_newton_update(int n, const double *cur, const double *dx, double *x,
                           hipStream_t s);
 void launch_reduce_sum(const double *part, double *out, hipStream_t s);
 
-// dst = src (n doubles, n a multiple of 4)
+// dst = src (n doubles, n even)
 void launch_copy(size_t n, const double *src, double *dst, hipStream_t s);
 
 }  // namespace ewg

@@ -339,10 +339,21 @@ __global__ void __launch_bounds__(kRedThreads) k_ks_crit2(KrylovState *ks, const
 
 // Newton helpers
 // plain streaming copy, 32 bytes per lane and iteration: the bandwidth reference
-__global__ void __launch_bounds__(256) k_copy(size_t n4, const double4 *src, double4 *dst) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+// dst = src over n16 16-byte words: the bandwidth reference. Each thread keeps four
+// coalesced 16-byte loads in flight before it stores them.
+__global__ void __launch_bounds__(256)
+k_copy(size_t n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                    d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
 }
 
 __global__ void __launch_bounds__(kRedThreads) k_max_abs(int n, const double *x, double *part) {
@@ -478,9 +489,8 @@ void launch_reduce_max(const double *part, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kRedThreads), 0, s, part, out);
 }
 void launch_copy(size_t n, const double *src, double *dst, hipStream_t s) {
-    const size_t n4 = n / 4;
-    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, n4, reinterpret_cast<const double4 *>(src),
-                       reinterpret_cast<double4 *>(dst));
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, n / 2,
+                       reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst));
 }
 
 void launch_reduce_sum(const double *part, double *out, hipStream_t s) {

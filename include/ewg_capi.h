@@ -232,9 +232,11 @@ int ewg_reset_stats(ewg_ctx *ctx);
 /* Measurement hook (no counterpart in the reference): launches one of the hot
  * kernels `reps` times on the context's stream at the current state and returns the
  * mean duration per launch measured with HIP events on that stream. EWG_KERNEL_COPY is a
- * plain device-to-device copy of the Jacobian's values (the bandwidth reference). */
+ * plain device-to-device copy of EWG_COPY_BYTES between two buffers allocated on its
+ * first use (reads and writes EWG_COPY_BYTES each: the bandwidth reference). */
 enum { EWG_KERNEL_LINEARIZE = 0, EWG_KERNEL_SPMV = 1, EWG_KERNEL_ILU_APPLY = 2,
        EWG_KERNEL_ILU_FACTOR = 3, EWG_KERNEL_COPY = 5 };
+#define EWG_COPY_BYTES (1ull << 30)
 int ewg_time_kernel(ewg_ctx *ctx, int which, int reps, double *mean_ms);
 
 /* Launches timed with device events while params.kernel_events is set (no counterpart in

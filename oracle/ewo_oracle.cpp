@@ -23,6 +23,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -345,6 +346,7 @@ struct ewo_ctx {
     std::vector<double> newton_errors;  // error of every linearization of the last Newton
     std::string err;
     bool static_set = false;
+    int threads = 1;  // worker threads of the linearization (ThreadsPerProcess)
 };
 
 namespace {
@@ -549,105 +551,124 @@ void linearize_all(ewo_ctx &c, double dt, int newton_iter, double *R, double *J)
     // intensive-quantity cache (EnableIntensiveQuantityCache, fvbasediscretization.hh:650-681):
     // every DOF is evaluated once per linearization with both derivative slots seeded
     std::vector<IQ> iqs(static_cast<size_t>(c.N));
-    for (int64_t cell = 0; cell < c.N; ++cell) iqs[cell] = intensive(c, cell, x);
-    for (int64_t cell = 0; cell < c.N; ++cell) {
-        int ijk[3];
-        cell_coords(c, cell, ijk);
-        // stencil: DOF 0 = the element, then one DOF per interior intersection
-        int64_t dofs[7];
-        int dirs[6];
-        int nd = 1, ni = 0;
-        dofs[0] = cell;
-        for (int dir = 0; dir < nfaces; ++dir) {
-            int64_t nb = neighbour(c, ijk, dir);
-            if (nb >= 0) {
-                dirs[ni++] = dir;
-                dofs[nd++] = nb;
-            }
+    // Elements are independent: element `cell` writes R[cell], column `cell` of J and its
+    // own storage cache entry only, so a contiguous range of elements per worker thread
+    // gives the same bits as one thread (the reference dispatches the element loop over
+    // ThreadsPerProcess worker threads, fvbaselinearizer.hh:416-471).
+    auto over_cells = [&](auto &&body) {
+        const int T = std::max(1, c.threads);
+        if (T == 1) {
+            body(int64_t(0), c.N);
+            return;
         }
-        IQ iq[7];
-        for (int l = 0; l < nd; ++l) iq[l] = iqs[dofs[l]];
-        Ev res[7][2];
-        // fluxes (fvbaselocalresidual.hh:295-367)
-        for (int f = 0; f < ni; ++f) {
-            Ev F[2];
-            interior_flux(c, iq[0], iq[f + 1], cell, dofs[f + 1], ijk, dirs[f], F);
-            double alpha = 1.0;
-            alpha *= c.area[dirs[f] >> 1];
-            for (int k = 0; k < 2; ++k) {
-                F[k] = F[k] * alpha;
-                res[0][k] = res[0][k] + F[k];
-                res[f + 1][k] = res[f + 1][k] - F[k];
-            }
-        }
-        // storage (fvbaselocalresidual.hh:494-629)
-        {
-            Ev M[2];
-            storage_of(c, cell, iq[0], M);
-            double Mold[2];
-            if (c.prm.enable_storage_cache) {
-                if (newton_iter == 0) {
-                    if (c.prm.recycle_first_iteration_storage) {
-                        Mold[0] = M[0].v;
-                        Mold[1] = M[1].v;
-                    } else {
-                        IQ qo = intensive(c, cell, c.x1.data());
-                        Ev Mo[2];
-                        storage_of(c, cell, qo, Mo);
-                        Mold[0] = Mo[0].v;
-                        Mold[1] = Mo[1].v;
-                    }
-                    c.storage_old[2 * cell] = Mold[0];
-                    c.storage_old[2 * cell + 1] = Mold[1];
-                } else {
-                    Mold[0] = c.storage_old[2 * cell];
-                    Mold[1] = c.storage_old[2 * cell + 1];
+        std::vector<std::thread> pool;
+        for (int t = 0; t < T; ++t)
+            pool.emplace_back(body, c.N * t / T, c.N * (t + 1) / T);
+        for (auto &th : pool) th.join();
+    };
+    over_cells([&](int64_t c0, int64_t c1) {
+        for (int64_t cell = c0; cell < c1; ++cell) iqs[cell] = intensive(c, cell, x);
+    });
+    over_cells([&](int64_t c0, int64_t c1) {
+        for (int64_t cell = c0; cell < c1; ++cell) {
+            int ijk[3];
+            cell_coords(c, cell, ijk);
+            // stencil: DOF 0 = the element, then one DOF per interior intersection
+            int64_t dofs[7];
+            int dirs[6];
+            int nd = 1, ni = 0;
+            dofs[0] = cell;
+            for (int dir = 0; dir < nfaces; ++dir) {
+                int64_t nb = neighbour(c, ijk, dir);
+                if (nb >= 0) {
+                    dirs[ni++] = dir;
+                    dofs[nd++] = nb;
                 }
-            } else {
-                IQ qo = intensive(c, cell, c.x1.data());
-                Ev Mo[2];
-                storage_of(c, cell, qo, Mo);
-                Mold[0] = Mo[0].v;
-                Mold[1] = Mo[1].v;
             }
-            double scv = c.V * 1.0;
-            for (int k = 0; k < 2; ++k) {
-                Ev tmp = M[k] - Mold[k];
-                tmp = tmp * (scv / dt);
-                res[0][k] = res[0][k] + tmp;
+            IQ iq[7];
+            for (int l = 0; l < nd; ++l) iq[l] = iqs[dofs[l]];
+            Ev res[7][2];
+            // fluxes (fvbaselocalresidual.hh:295-367)
+            for (int f = 0; f < ni; ++f) {
+                Ev F[2];
+                interior_flux(c, iq[0], iq[f + 1], cell, dofs[f + 1], ijk, dirs[f], F);
+                double alpha = 1.0;
+                alpha *= c.area[dirs[f] >> 1];
+                for (int k = 0; k < 2; ++k) {
+                    F[k] = F[k] * alpha;
+                    res[0][k] = res[0][k] + F[k];
+                    res[f + 1][k] = res[f + 1][k] - F[k];
+                }
+            }
+            // storage (fvbaselocalresidual.hh:494-629)
+            {
+                Ev M[2];
+                storage_of(c, cell, iq[0], M);
+                double Mold[2];
+                if (c.prm.enable_storage_cache) {
+                    if (newton_iter == 0) {
+                        if (c.prm.recycle_first_iteration_storage) {
+                            Mold[0] = M[0].v;
+                            Mold[1] = M[1].v;
+                        } else {
+                            IQ qo = intensive(c, cell, c.x1.data());
+                            Ev Mo[2];
+                            storage_of(c, cell, qo, Mo);
+                            Mold[0] = Mo[0].v;
+                            Mold[1] = Mo[1].v;
+                        }
+                        c.storage_old[2 * cell] = Mold[0];
+                        c.storage_old[2 * cell + 1] = Mold[1];
+                    } else {
+                        Mold[0] = c.storage_old[2 * cell];
+                        Mold[1] = c.storage_old[2 * cell + 1];
+                    }
+                } else {
+                    IQ qo = intensive(c, cell, c.x1.data());
+                    Ev Mo[2];
+                    storage_of(c, cell, qo, Mo);
+                    Mold[0] = Mo[0].v;
+                    Mold[1] = Mo[1].v;
+                }
+                double scv = c.V * 1.0;
+                for (int k = 0; k < 2; ++k) {
+                    Ev tmp = M[k] - Mold[k];
+                    tmp = tmp * (scv / dt);
+                    res[0][k] = res[0][k] + tmp;
+                }
+            }
+            // boundary (fvbaselocalresidual.hh:425-487)
+            for (int dir = 0; dir < nfaces; ++dir) {
+                if (neighbour(c, ijk, dir) >= 0) continue;
+                int64_t bf = boundary_face_index(c, ijk, dir);
+                int t = c.bc_type[bf];
+                Ev Bv[2];
+                if (t == EWG_BC_FREE_FLOW) {
+                    freeflow_flux(c, iq[0], cell, ijk, dir, c.bc_val[2 * bf], c.bc_val[2 * bf + 1], Bv);
+                } else if (t == EWG_BC_MASS_RATE) {
+                    Bv[0] = Ev(c.bc_val[2 * bf]);
+                    Bv[1] = Ev(c.bc_val[2 * bf + 1]);
+                } else {
+                    Bv[0] = Ev(0.0);
+                    Bv[1] = Ev(0.0);
+                }
+                double mult = c.area[dir >> 1] * 1.0;
+                for (int k = 0; k < 2; ++k) res[0][k] = res[0][k] + Bv[k] * mult;
+            }
+            // volumetric residual (fvbaselocalresidual.hh:170-186)
+            for (int l = 0; l < nd; ++l)
+                for (int k = 0; k < 2; ++k) res[l][k] = res[l][k] / c.V;
+            // scatter (fvbaselinearizer.hh:495-508)
+            R[2 * cell] += res[0][0].v;
+            R[2 * cell + 1] += res[0][1].v;
+            for (int l = 0; l < nd; ++l) {
+                int64_t p = find_block(c, dofs[l], cell);
+                double *blk = J + 4 * p;
+                for (int k = 0; k < 2; ++k)
+                    for (int pv = 0; pv < 2; ++pv) blk[2 * k + pv] += res[l][k].d[pv];
             }
         }
-        // boundary (fvbaselocalresidual.hh:425-487)
-        for (int dir = 0; dir < nfaces; ++dir) {
-            if (neighbour(c, ijk, dir) >= 0) continue;
-            int64_t bf = boundary_face_index(c, ijk, dir);
-            int t = c.bc_type[bf];
-            Ev Bv[2];
-            if (t == EWG_BC_FREE_FLOW) {
-                freeflow_flux(c, iq[0], cell, ijk, dir, c.bc_val[2 * bf], c.bc_val[2 * bf + 1], Bv);
-            } else if (t == EWG_BC_MASS_RATE) {
-                Bv[0] = Ev(c.bc_val[2 * bf]);
-                Bv[1] = Ev(c.bc_val[2 * bf + 1]);
-            } else {
-                Bv[0] = Ev(0.0);
-                Bv[1] = Ev(0.0);
-            }
-            double mult = c.area[dir >> 1] * 1.0;
-            for (int k = 0; k < 2; ++k) res[0][k] = res[0][k] + Bv[k] * mult;
-        }
-        // volumetric residual (fvbaselocalresidual.hh:170-186)
-        for (int l = 0; l < nd; ++l)
-            for (int k = 0; k < 2; ++k) res[l][k] = res[l][k] / c.V;
-        // scatter (fvbaselinearizer.hh:495-508)
-        R[2 * cell] += res[0][0].v;
-        R[2 * cell + 1] += res[0][1].v;
-        for (int l = 0; l < nd; ++l) {
-            int64_t p = find_block(c, dofs[l], cell);
-            double *blk = J + 4 * p;
-            for (int k = 0; k < 2; ++k)
-                for (int pv = 0; pv < 2; ++pv) blk[2 * k + pv] += res[l][k].d[pv];
-        }
-    }
+    });
 }
 
 void build_pattern(ewo_ctx &c) {
@@ -997,6 +1018,12 @@ int fail(ewo_ctx *c, int code, const std::string &msg) {
 // C API
 // -------------------------------------------------------------------------------------
 extern "C" {
+
+int ewo_set_threads(ewo_ctx *c, int threads) {
+    if (!c || threads < 1) return EWG_INVALID_ARGUMENT;
+    c->threads = threads;
+    return EWG_OK;
+}
 
 double ewo_log(double x) { return d_log(x); }
 double ewo_exp(double x) { return d_exp(x); }

@@ -37,6 +37,9 @@ int64_t ewo_num_blocks(const ewo_ctx *ctx);
 int64_t ewo_num_boundary_faces(const ewo_ctx *ctx);
 int ewo_get_pattern(const ewo_ctx *ctx, int32_t *rowptr, int32_t *colidx);
 
+/* worker threads of the element loop (ThreadsPerProcess, fvbasediscretization.hh:198,
+ * default 1); the linear solver stays serial as in the reference */
+int ewo_set_threads(ewo_ctx *ctx, int threads);
 /* residual + Jacobian at solution(0); storage cache semantics as in ewg_linearize */
 int ewo_linearize(ewo_ctx *ctx, double dt, int newton_iter, double *resid, double *jac);
 /* storage M(x) [2N] (value only) of an arbitrary state */

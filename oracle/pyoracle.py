@@ -45,6 +45,8 @@ def load():
             fn = getattr(lib, "ewo_" + name)
             fn.restype = C.c_int
             fn.argtypes = args
+        lib.ewo_set_threads.restype = C.c_int
+        lib.ewo_set_threads.argtypes = [vp, C.c_int]
         lib.ewo_ilu0_apply32.restype = C.c_int
         lib.ewo_ilu0_apply32.argtypes = [vp, C.POINTER(C.c_float), _D, _D]
         lib.ewo_step_log.restype = C.c_int
@@ -85,6 +87,9 @@ class OracleContext(Context):
         v = np.empty(2 * self.N)
         self._check(self.lib.ewo_ilu0_apply(self.h, _p(LU), _p(np.ascontiguousarray(d)), _p(v)))
         return v
+
+    def set_threads(self, n):
+        self._check(self.lib.ewo_set_threads(self.h, int(n)))
 
     def ilu0_apply32(self, LU32, d):
         LU32 = np.ascontiguousarray(LU32, dtype=np.float32)

@@ -150,3 +150,18 @@ def test_neighbour_columns_are_minus_face_derivatives():
     expect = np.zeros(2 * P.N)
     expect[1::2] = phi * (P.rho[0] - P.rho[1]) / P.dt
     assert np.allclose(colsum, expect, rtol=1e-9, atol=1e-12 * np.abs(A).max())
+
+
+def test_threaded_linearization_is_bitwise_identical():
+    """The element loop over worker threads (ThreadsPerProcess) gives the single-thread
+    residual and Jacobian bit for bit: every element writes its own row entry and column."""
+    from ewg import problems
+    P = problems.heterogeneous_3d(9, 7, 5, seed=9, patch=2)
+    x = problems.perturbed_state(P, seed=10, dS=0.2)
+    out = []
+    for threads in (1, 3):
+        o = _oracle(P)
+        o.set_threads(threads)
+        out.append(_residual_at(o, P, x, P.dt, problems.perturbed_state(P, seed=11, dS=0.1)))
+        out.append(o.linearize(P.dt, 1)[1])
+    assert np.array_equal(out[0], out[2]) and np.array_equal(out[1], out[3])
