@@ -605,7 +605,10 @@ __global__ void __launch_bounds__(BT) k_b_init(const Member *mem, Maps mp, const
     }
 }
 
-__global__ void __launch_bounds__(BT) k_b_ks_init(const Member *mem, Maps mp, const int *act, const int *err, KrylovState *ks,
+// err: singular pivot of the member's factorization; serr: one of its tile-wavefront
+// waits ran out (either is a breakdown of the member's solve)
+__global__ void __launch_bounds__(BT) k_b_ks_init(const Member *mem, Maps mp, const int *act, const int *err,
+                                                  const int *serr, KrylovState *ks,
                                                   const double *partMax, const double *partDot) {
     const int id = blockIdx.x;
     if (!act[id]) {
@@ -631,7 +634,8 @@ __global__ void __launch_bounds__(BT) k_b_ks_init(const Member *mem, Maps mp, co
         k.rho_i = d;
         k.iterations = 0;
         const double red = e / fmax(1e-20, e);
-        k.status = err[id] ? KS_BREAK : ((red <= m.lin_rel_tol || e <= m.lin_abs_tol) ? KS_CONV : KS_RUN);
+        k.status = (err[id] || serr[id]) ? KS_BREAK
+                   : ((red <= m.lin_rel_tol || e <= m.lin_abs_tol) ? KS_CONV : KS_RUN);
     }
 }
 
@@ -734,10 +738,17 @@ __global__ void __launch_bounds__(BT) k_b_hs_update(const Member *mem, Maps mp, 
     }
 }
 
-__global__ void __launch_bounds__(BT) k_b_ks_crit1(Maps mp, KrylovState *ks, const double *partMax, const int *partAny) {
+// serr[id] set: a wait of the member's preconditioner sweeps ran out, its result is not
+// the preconditioner's (a breakdown of the member's solve)
+__global__ void __launch_bounds__(BT) k_b_ks_crit1(Maps mp, KrylovState *ks, const double *partMax, const int *partAny,
+                                                   const int *serr) {
     const int id = blockIdx.x;
     KrylovState &k = ks[id];
     if (k.status != KS_RUN) return;
+    if (serr[id]) {
+        if (threadIdx.x == 0) k.status = KS_BREAK;
+        return;
+    }
     const double mx = range_max(partMax, mp.coff[id], mp.coff[id + 1]);
     const int any = range_any(partAny, mp.coff[id], mp.coff[id + 1]);
     if (threadIdx.x == 0) b_criterion(k, mx, any == 0);
@@ -787,10 +798,15 @@ __global__ void __launch_bounds__(BT) k_b_xr_update(const Member *mem, Maps mp, 
     }
 }
 
-__global__ void __launch_bounds__(BT) k_b_ks_crit2(Maps mp, KrylovState *ks, const int *partAny) {
+__global__ void __launch_bounds__(BT) k_b_ks_crit2(Maps mp, KrylovState *ks, const int *partAny,
+                                                   const int *serr) {
     const int id = blockIdx.x;
     KrylovState &k = ks[id];
     if (k.status != KS_RUN) return;
+    if (serr[id]) {
+        if (threadIdx.x == 0) k.status = KS_BREAK;
+        return;
+    }
     const int any = range_any(partAny, mp.coff[id], mp.coff[id + 1]);
     if (threadIdx.x == 0) {
         // the residual checked here is still s (the object r is updated afterwards)
@@ -822,6 +838,12 @@ __global__ void __launch_bounds__(BT) k_b_update(const Member *mem, Maps mp, con
 __global__ void __launch_bounds__(BT) k_b_zero(int n, int *v) {
     const int i = blockIdx.x * BT + threadIdx.x;
     if (i < n) v[i] = 0;
+}
+
+// v[i] = 1 where mask[i] (the host's forced sweep failures)
+__global__ void __launch_bounds__(BT) k_b_set(int n, int *v, const int *mask) {
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i < n && mask[i]) v[i] = 1;
 }
 
 }  // namespace batch
@@ -858,7 +880,17 @@ struct ewg_ensemble {
     // members with the tiles' wavefront layout (params.ilu_schedule = 0, 3D, all of one
     // precision): the preconditioner applications run as batched tile-wavefront sweeps
     void *wl_py = nullptr, *wl_rz = nullptr;
-    std::vector<KrylovState> ks_host;
+    // per member: set when one of its tile-wavefront waits (sweeps or factorization) ran
+    // out; turned into a Krylov breakdown of that member
+    int *sweep_err = nullptr;
+    int *force_mask = nullptr;           // device copy of h_force (ewg_ensemble_force_sweep_failure)
+    std::vector<int> force_left;         // per member: solves still to start with the flag set
+    // pinned host staging of every host<->device transfer of the batched mode, n entries
+    // each; each is rewritten only after a stream synchronisation that follows its last
+    // transfer
+    int *h_act = nullptr, *h_it = nullptr, *h_sel = nullptr, *h_serr = nullptr, *h_force = nullptr;
+    double *h_dts = nullptr, *h_res = nullptr;
+    KrylovState *h_ks = nullptr;
     std::string err;
 };
 
@@ -954,7 +986,18 @@ int batched_setup(ewg_ensemble *e) {
     ENS_TRY(e, hipMemcpy(e->coff, coff.data(), coff.size() * sizeof(int), hipMemcpyHostToDevice));
     ENS_TRY(e, hipMemcpy(e->lmem, lmem.data(), lmem.size() * sizeof(int), hipMemcpyHostToDevice));
     ENS_TRY(e, hipMemcpy(e->loff, loff.data(), loff.size() * sizeof(int), hipMemcpyHostToDevice));
-    e->ks_host.resize(n);
+    ENS_TRY(e, dal(&e->sweep_err, n));
+    ENS_TRY(e, dal(&e->force_mask, n));
+    ENS_TRY(e, hipMemset(e->sweep_err, 0, n * sizeof(int)));
+    e->force_left.assign(n, 0);
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_act), n * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_it), n * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_sel), n * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_serr), n * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_force), n * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_dts), n * sizeof(double)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_res), n * sizeof(double)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_ks), n * sizeof(KrylovState)));
     // batched tile-wavefront sweeps when every member carries the layout
     bool wl = true;
     for (int m = 0; m < n; ++m) {
@@ -972,6 +1015,7 @@ int batched_setup(ewg_ensemble *e) {
             WlMember w{};
             w.g = c->g;
             w.sync = c->wf;
+            w.sync.err = e->sweep_err + m;  // the member's own flag, checked per solve
             if (fp32) {
                 w.L = c->wl32.L; w.U = c->wl32.U; w.D = c->wl32.D; w.w = c->wl32.w;
             } else {
@@ -1026,13 +1070,15 @@ int iteration_graph(ewg_ensemble *e) {
                        (int)V_Y, (int)V_V, (int)V_RES, -1, e->part0, static_cast<double *>(nullptr));
     hipLaunchKernelGGL(k_b_ks_alpha, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part0);
     hipLaunchKernelGGL(k_b_hs_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part1, e->partAny);
-    hipLaunchKernelGGL(k_b_ks_crit1, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part1, e->partAny);
+    hipLaunchKernelGGL(k_b_ks_crit1, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part1, e->partAny,
+                       e->sweep_err);
     launch_apply(e, V_R, V_Z);
     hipLaunchKernelGGL(k_b_spmv, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks,
                        (int)V_Z, (int)V_Y, (int)V_Y, (int)V_R, e->part0, e->part1);
     hipLaunchKernelGGL(k_b_ks_omega, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part0, e->part1);
     hipLaunchKernelGGL(k_b_xr_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2, e->partAny);
-    hipLaunchKernelGGL(k_b_ks_crit2, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->partAny);
+    hipLaunchKernelGGL(k_b_ks_crit2, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->partAny,
+                       e->sweep_err);
     hipError_t h = hipStreamEndCapture(e->stream, &gr);
     ENS_TRY(e, h);
     h = hipGraphInstantiate(&e->iter_graph, gr, nullptr, nullptr, 0);
@@ -1049,22 +1095,28 @@ bool proceed_h(int it, double err, double lerr, const Member &m) {
 }
 
 // one Newton solve (newtonmethod.hh:308-517) of dts[m] for every member with part[m],
-// from x0 = x1; rc[m] = 0 or the EWG_* code of the failure; err[m] = final error
+// from x0 = x1; rc[m] = 0 or the EWG_* code of the failure; err[m] = final error.
+// Every transfer goes through the ensemble's pinned staging buffers (h_*); each buffer is
+// rewritten only after the stream synchronisation that follows its previous transfer.
 int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vector<double> &dts,
                    std::vector<int> &rc, std::vector<double> &err) {
     const int n = static_cast<int>(e->members.size());
+    if ((int)part.size() != n || (int)dts.size() != n || (int)rc.size() != n ||
+        (int)err.size() != n || (int)e->force_left.size() != n || !e->h_act || !e->h_ks)
+        return set_err(e, EWG_INVALID_STATE, "batched Newton: per-member arrays of the wrong size");
     const Maps mp = maps(e);
     hipStream_t s = e->stream;
     enum { RUN = 0, DONE = 1, FAILED = 2, OUT = 3 };
     std::vector<int> state(n, OUT), it(n, 0), act(n, 0);
-    std::vector<double> lerr(n, 1e100), res(n, 0.0);
+    std::vector<double> lerr(n, 1e100);
     for (int m = 0; m < n; ++m) {
         rc[m] = 0;
         err[m] = 1e100;
         if (part[m]) state[m] = RUN;
     }
-    auto upload = [&](const std::vector<int> &v, int *d) -> int {
-        ENS_TRY(e, hipMemcpyAsync(d, v.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    auto upload = [&](const std::vector<int> &v, int *pinned, int *d) -> int {
+        std::copy(v.begin(), v.end(), pinned);
+        ENS_TRY(e, hipMemcpyAsync(d, pinned, n * sizeof(int), hipMemcpyHostToDevice, s));
         return EWG_OK;
     };
     auto fail = [&](int m, int code) {
@@ -1072,7 +1124,8 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
         rc[m] = code;
     };
     if (int r = iteration_graph(e)) return r;
-    ENS_TRY(e, hipMemcpyAsync(e->dts, dts.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    std::copy(dts.begin(), dts.end(), e->h_dts);
+    ENS_TRY(e, hipMemcpyAsync(e->dts, e->h_dts, n * sizeof(double), hipMemcpyHostToDevice, s));
     while (true) {
         bool any = false;
         for (int m = 0; m < n; ++m) {
@@ -1087,21 +1140,21 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
             lerr[m] = err[m];  // beginIteration_
         }
         if (!any) break;
-        if (int r = upload(act, e->act)) return r;
-        if (int r = upload(it, e->niter)) return r;
+        if (int r = upload(act, e->h_act, e->act)) return r;
+        if (int r = upload(it, e->h_it, e->niter)) return r;
         hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, (int)V_XCUR, (int)V_X0);
         hipLaunchKernelGGL(k_b_lin<0>, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->niter, e->dts);
         hipLaunchKernelGGL(k_b_lin<1>, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->niter, e->dts);
         hipLaunchKernelGGL(k_b_maxabs, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, (int)V_RES, e->part0);
         hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(BT), 0, s, mp, e->act, e->part0, 1, e->pres);
-        ENS_TRY(e, hipMemcpyAsync(res.data(), e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        ENS_TRY(e, hipMemcpyAsync(e->h_res, e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
         ENS_TRY(e, hipStreamSynchronize(s));
         bool solve = false;
         for (int m = 0; m < n; ++m) {
             if (!act[m]) continue;
             e->outcome[m].newton++;
             lerr[m] = err[m];  // preSolve_
-            err[m] = res[m];
+            err[m] = e->h_res[m];
             act[m] = 0;
             if (err[m] > e->host[m].newton_max_error) {
                 fail(m, EWG_NUMERICAL_ISSUE);
@@ -1113,8 +1166,9 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
             }
         }
         if (!solve) continue;
-        if (int r = upload(act, e->act)) return r;
+        if (int r = upload(act, e->h_act, e->act)) return r;
         hipLaunchKernelGGL(k_b_zero, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ilu_err);
+        hipLaunchKernelGGL(k_b_zero, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->sweep_err);
         if (e->wl_py) {
             // every member's factorization in one tile-wavefront launch, into its layout
             wl_batch_factor(e->wl_py, s);
@@ -1124,24 +1178,51 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
             hipLaunchKernelGGL(k_b_round, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act);
         }
         hipLaunchKernelGGL(k_b_init, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0, e->part1);
-        hipLaunchKernelGGL(k_b_ks_init, dim3(n), dim3(BT), 0, s, e->dmem, mp, e->act, e->ilu_err, e->ks, e->part0, e->part1);
+        hipLaunchKernelGGL(k_b_ks_init, dim3(n), dim3(BT), 0, s, e->dmem, mp, e->act, e->ilu_err, e->sweep_err,
+                           e->ks, e->part0, e->part1);
+        // members whose sweeps the host makes fail in this solve (a test of the failure
+        // path): their flag is set after the factorization, before the first sweep
+        bool forced = false;
+        for (int m = 0; m < n; ++m) {
+            e->h_force[m] = act[m] && e->force_left[m] > 0;
+            if (e->h_force[m]) {
+                --e->force_left[m];
+                forced = true;
+            }
+        }
+        if (forced) {
+            ENS_TRY(e, hipMemcpyAsync(e->force_mask, e->h_force, n * sizeof(int), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_b_set, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->sweep_err, e->force_mask);
+        }
         ENS_TRY(e, hipGetLastError());
         // iterations are replayed speculatively, the states read after 1, 2, 4, then every
         // 8 of them (members that left the running state skip the rest)
         int batch = 1;
         while (true) {
-            ENS_TRY(e, hipMemcpyAsync(e->ks_host.data(), e->ks, n * sizeof(KrylovState), hipMemcpyDeviceToHost, s));
+            ENS_TRY(e, hipMemcpyAsync(e->h_ks, e->ks, n * sizeof(KrylovState), hipMemcpyDeviceToHost, s));
+            ENS_TRY(e, hipMemcpyAsync(e->h_serr, e->sweep_err, n * sizeof(int), hipMemcpyDeviceToHost, s));
             ENS_TRY(e, hipStreamSynchronize(s));
             bool running = false;
-            for (int m = 0; m < n; ++m) running |= e->ks_host[m].status == KS_RUN;
+            for (int m = 0; m < n; ++m) running |= e->h_ks[m].status == KS_RUN;
             if (!running) break;
             for (int q = 0; q < batch; ++q) ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
             batch = std::min(batch * 2, 8);
         }
+        // a member whose wavefront wait ran out may have left unconsumed words in its
+        // tile edge buffers: back to the sentinel before its next sweep
+        if (e->wl_py) {
+            bool reset = false;
+            for (int m = 0; m < n; ++m)
+                if (e->h_serr[m]) {
+                    wf_init_edges(e->members[m]->wf, s);
+                    reset = true;
+                }
+            if (reset) ENS_TRY(e, hipStreamSynchronize(s));
+        }
         bool upd = false;
         for (int m = 0; m < n; ++m) {
             if (!act[m]) continue;
-            const KrylovState &k = e->ks_host[m];
+            const KrylovState &k = e->h_ks[m];
             e->outcome[m].linear += k.iterations;
             e->outcome[m].solves++;
             if (k.status == KS_CONV) {
@@ -1152,14 +1233,14 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
             fail(m, k.status == KS_BREAK ? EWG_NUMERICAL_ISSUE : EWG_NOT_CONVERGED);
         }
         if (!upd) continue;
-        if (int r = upload(act, e->act)) return r;
+        if (int r = upload(act, e->h_act, e->act)) return r;
         hipLaunchKernelGGL(k_b_update, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0);
         hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(BT), 0, s, mp, e->act, e->part0, 0, e->pres);
-        ENS_TRY(e, hipMemcpyAsync(res.data(), e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        ENS_TRY(e, hipMemcpyAsync(e->h_res, e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
         ENS_TRY(e, hipStreamSynchronize(s));
         for (int m = 0; m < n; ++m) {
             if (!act[m]) continue;
-            if (!std::isfinite(res[m])) fail(m, EWG_NUMERICAL_ISSUE);
+            if (!std::isfinite(e->h_res[m])) fail(m, EWG_NUMERICAL_ISSUE);
             else ++it[m];
         }
     }
@@ -1173,7 +1254,10 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
 // dst <- src for the members with sel[m] (act buffer reused)
 int batched_copy(ewg_ensemble *e, const std::vector<int> &sel, int dst, int src) {
     const int n = static_cast<int>(e->members.size());
-    ENS_TRY(e, hipMemcpyAsync(e->act, sel.data(), n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    if ((int)sel.size() != n || !e->h_sel)
+        return set_err(e, EWG_INVALID_STATE, "batched copy: selection of the wrong size");
+    std::copy(sel.begin(), sel.end(), e->h_sel);
+    ENS_TRY(e, hipMemcpyAsync(e->act, e->h_sel, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
     hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, maps(e), e->act, dst, src);
     ENS_TRY(e, hipGetLastError());
     ENS_TRY(e, hipStreamSynchronize(e->stream));
@@ -1318,9 +1402,14 @@ void ewg_ensemble_destroy(ewg_ensemble *e) {
     wl_batch_free(e->wl_py);
     wl_batch_free(e->wl_rz);
     void *bufs[] = {e->cmem, e->coff, e->lmem, e->loff, e->act, e->niter, e->ilu_err, e->ks,
-                    e->part0, e->part1, e->part2, e->pres, e->partAny, e->dts};
+                    e->part0, e->part1, e->part2, e->pres, e->partAny, e->dts, e->sweep_err,
+                    e->force_mask};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    void *pinned[] = {e->h_act, e->h_it, e->h_sel, e->h_serr, e->h_force, e->h_dts, e->h_res,
+                      e->h_ks};
+    for (void *b : pinned)
+        if (b) (void)hipHostFree(b);
     for (double *b : e->xs)
         if (b) (void)hipFree(b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1328,6 +1417,15 @@ void ewg_ensemble_destroy(ewg_ensemble *e) {
 }
 
 const char *ewg_ensemble_last_error(const ewg_ensemble *e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int ewg_ensemble_force_sweep_failure(ewg_ensemble *e, int32_t member, int32_t solves) {
+    if (!e || member < 0 || member >= (int32_t)e->members.size() || solves < 0)
+        return EWG_INVALID_ARGUMENT;
+    if ((int)e->force_left.size() != (int)e->members.size())
+        return set_err(e, EWG_INVALID_STATE, "ensemble not set up");
+    e->force_left[member] = solves;
+    return EWG_OK;
+}
 
 int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
     if (!e || !(dt > 0.0)) return EWG_INVALID_ARGUMENT;

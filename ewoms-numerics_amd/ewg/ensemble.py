@@ -60,6 +60,7 @@ def _lib():
             "ewg_ensemble_run": (C.c_int, [_vp, C.c_double, C.c_int32,
                                            C.POINTER(C.c_double)]),
             "ewg_ensemble_stats": (C.c_int, [_vp, C.c_int32, C.POINTER(MemberStats)]),
+            "ewg_ensemble_force_sweep_failure": (C.c_int, [_vp, C.c_int32, C.c_int32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -113,6 +114,11 @@ class Ensemble:
         return {"mode": MODES[t.mode], "threads": t.threads,
                 "candidates": [{"mode": MODES[t.cand_mode[k]], "threads": t.cand_threads[k],
                                 "ms": t.cand_ms[k]} for k in range(t.ncandidates)]}
+
+    def force_sweep_failure(self, member, solves=1):
+        """Test hook: the member's next `solves` batched solves start with its sweep error
+        flag set (ewg_ensemble_force_sweep_failure)."""
+        self._check(self.lib.ewg_ensemble_force_sweep_failure(self.h, member, solves))
 
     def set_mode(self, mode, threads=512):
         self._check(self.lib.ewg_ensemble_set_mode(self.h, mode, threads))

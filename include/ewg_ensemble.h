@@ -81,6 +81,13 @@ int ewg_ensemble_run(ewg_ensemble *ens, double dt, int32_t steps, double *second
 int ewg_ensemble_stats(const ewg_ensemble *ens, int32_t i,
                        ewg_ensemble_member_stats *out);
 
+/* Test hook of the failure path (no counterpart in the reference): the next `solves`
+ * linear solves of member i in the batched mode start their preconditioner sweeps with
+ * the member's wavefront-wait error flag already set, as if one of its waits had run
+ * out. The member's solve then breaks down (EWG_NUMERICAL_ISSUE, retried with a halved
+ * step), its tile edge buffers are reset, and the other members are unaffected. */
+int ewg_ensemble_force_sweep_failure(ewg_ensemble *ens, int32_t i, int32_t solves);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,3 +149,42 @@ def test_tuner_and_graph_replay(mode):
     for i in range(4):
         assert ea.stats(i)["steps"] == eb.stats(i)["steps"] == 3
         assert np.array_equal(a[i].solution(0), b[i].solution(0))
+
+
+def test_forced_sweep_failure_is_isolated_and_recovers():
+    """The host makes member 1's first batched solve start with its sweep error flag set
+    (as if a wavefront wait had run out; every wait stays bounded): that solve breaks
+    down, the member's step is retried as two half steps after its tile edge buffers are
+    reset, and it then matches a single context stepped the same way; the other members
+    match their single runs exactly as without the failure."""
+    import ewg
+    from ewg import ensemble
+    probs = _problems()
+    prm = ensemble.member_params()
+    ctxs = [ewg.GpuContext(P, prm) for P in probs]
+    ens = ensemble.Ensemble(ctxs)
+    ens.set_mode(ensemble.BATCHED, 512)
+    ens.force_sweep_failure(1, 1)
+    dt = probs[0].dt
+    ens.run(dt, 2)
+    st = [ens.stats(i) for i in range(4)]
+    print(st, flush=True)
+    ref = _single_runs([probs[0], probs[2], probs[3]], prm, 2)
+    for i, (its, _, x) in zip((0, 2, 3), ref):
+        assert st[i]["status"] == 0 and st[i]["steps"] == 2 and st[i]["step_divisions"] == 0
+        assert st[i]["newton_iterations"] == its
+        assert np.abs(ctxs[i].solution(0)[1::2] - x[1::2]).max() < 1e-7
+    # member 1: the failed attempt's linearization, then dt/2 twice, then dt
+    c = ewg.GpuContext(probs[1], prm)
+    its = 1
+    for h in (dt / 2, dt / 2, dt):
+        rc, it, _ = c.newton(h)
+        assert rc == 0
+        its += it
+        c.advance_time_level()
+    assert st[1]["status"] == 0 and st[1]["steps"] == 2 and st[1]["step_divisions"] == 1
+    assert st[1]["newton_iterations"] == its
+    dS = np.abs(ctxs[1].solution(0)[1::2] - c.solution(0)[1::2]).max()
+    print(f"member 1 vs single context with the halved first step: max |dS| {dS:.2e}",
+          flush=True)
+    assert dS < 1e-7
