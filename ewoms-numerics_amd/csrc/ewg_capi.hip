@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,52 @@ int set_err(ewg_ctx *c, int code, const std::string &msg) {
             return set_err(ctx, EWG_DEVICE_ERROR,                                         \
                            std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
+
+// ---------------------------------------------------------------------------------
+// Scratch of ewg_apply_preconditioner_noalloc: one array per device (each device that
+// loads the library holds its own copy until the process ends), split into fixed regions
+// so that the edge words always live in the edge region:
+//   [0, 256)                 the two tile tickets of the sweeps
+//   [256, 256 + 16 MiB)      both edge buffers of the tile sweeps (the sentinel between
+//                            calls; the sweeps restore it word by word)
+//   [.., .. + 32 MiB)        the forward sweep's intermediate vector
+// Host code never passes `scratch` itself to a kernel: it is a host-side stand-in there;
+// the device address comes from hipGetSymbolAddress once per device (noalloc_setup).
+// ---------------------------------------------------------------------------------
+constexpr size_t kScratchTickets = 256;
+constexpr size_t kScratchEdges = 16ull << 20;
+constexpr size_t kScratchVec = 32ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchTickets + kScratchEdges + kScratchVec];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex[kMaxDevices];   // one call at a time per device
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// the edge region as one pair of edge buffers (for the sentinel fill)
+WfSync scratch_edges(unsigned char *base) {
+    WfSync w{};
+    w.ex = reinterpret_cast<unsigned long long *>(base + kScratchTickets);
+    w.edge_words = kScratchEdges / 16;
+    w.ey = w.ex + w.edge_words;
+    return w;
+}
+
+// Once per device, with that device current: its scratch array's device address, the edge
+// region filled with the sentinel.
+int noalloc_setup(int dev, hipStream_t s) {
+    if (dev < 0 || dev >= kMaxDevices) return EWG_DEVICE_ERROR;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex[dev]);
+    if (g_scratch_ptr[dev]) return EWG_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return EWG_DEVICE_ERROR;
+    unsigned char *base = static_cast<unsigned char *>(p);
+    wf_init_edges(scratch_edges(base), s);
+    if (hipStreamSynchronize(s) != hipSuccess) return EWG_DEVICE_ERROR;
+    g_scratch_ptr[dev] = base;
+    return EWG_OK;
+}
+
+__global__ void k_flag_clear(int *flag) { *flag = 0; }
 
 template <class T>
 hipError_t dalloc(T **p, size_t count) {
@@ -622,8 +669,11 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
         return EWG_DEVICE_ERROR;
     };
     hipError_t e;
+    if ((e = hipGetDevice(&c->device)) != hipSuccess) return fail("device", e);
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return fail("stream", e);
+    if (noalloc_setup(c->device, c->stream) != EWG_OK)
+        return fail("no-allocation scratch", hipErrorInvalidSymbol);
     const size_t n2 = 2 * (size_t)N;
     const size_t nb = (size_t)g.nslots * N * 4;
 #define ALLOC(p, cnt) \
@@ -999,6 +1049,64 @@ int ewg_apply_preconditioner(ewg_ctx *c, const double *d, double *v) {
         return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
     }
     HIP_TRY(c, hipMemcpy(v, c->ky, bytes, hipMemcpyDeviceToHost));
+    return EWG_OK;
+}
+
+int ewg_apply_preconditioner_noalloc(ewg_ctx *c, const double *d, double *v) {
+    if (!c || !d || !v) return EWG_INVALID_ARGUMENT;
+    const Grid &g = c->g;
+    const bool fp32 = c->prm.precond_fp32;
+    const bool layout = use_wavefront_layout(c);
+    if (!layout && use_wavefront(c))
+        return set_err(c, EWG_INVALID_ARGUMENT,
+                       "the no-allocation apply runs ilu_schedule 0 (3D) or the hyperplane schedule");
+    const size_t vec_bytes = layout ? wl_entries(g, c->wf) * 2 * (fp32 ? sizeof(float) : sizeof(double))
+                                    : (fp32 ? 2 * (size_t)g.N * sizeof(float) : 0);
+    if ((layout && 2 * c->wf.edge_words * 8 > kScratchEdges) || vec_bytes > kScratchVec)
+        return set_err(c, EWG_INVALID_ARGUMENT,
+                       "grid too large for the no-allocation scratch (16 MiB of tile edges, "
+                       "32 MiB of intermediate)");
+    if (!c->have_precond) return set_err(c, EWG_INVALID_STATE, "no preconditioner");
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device])
+        return set_err(c, EWG_INVALID_STATE, "no-allocation scratch not set up on this device");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex[c->device]);
+    unsigned char *base = g_scratch_ptr[c->device];
+    hipStream_t s = c->stream;
+    if (layout) {
+        WfSync w = c->wf;
+        w.ticket = reinterpret_cast<unsigned *>(base);
+        w.ex = reinterpret_cast<unsigned long long *>(base + kScratchTickets);
+        w.ey = w.ex + c->wf.edge_words;
+        w.err = c->ilu_err;
+        hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, c->ilu_err);
+        if (fp32) {
+            WlPrecond<float> wl = c->wl32;
+            wl.w = reinterpret_cast<float *>(base + kScratchTickets + kScratchEdges);
+            wl_apply32(g, w, wl, d, v, nullptr, s, false);
+        } else {
+            WlPrecond<double> wl = c->wl64;
+            wl.w = reinterpret_cast<double *>(base + kScratchTickets + kScratchEdges);
+            wl_apply64(g, w, wl, d, v, nullptr, s, false);
+        }
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        if (c->err_host[0]) {
+            // a wait ran out: the edge region may hold unconsumed words
+            wf_init_edges(scratch_edges(base), s);
+            HIP_TRY(c, hipStreamSynchronize(s));
+            return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
+        }
+        return EWG_OK;
+    }
+    if (fp32) {
+        float *w = reinterpret_cast<float *>(base + kScratchTickets + kScratchEdges);
+        launch_ilu_apply32(g, c->L32, c->D32, c->U32, d, w, v, nullptr, s);
+    } else {
+        launch_ilu_apply64(g, c->L64, c->D64, c->J, d, v, nullptr, s);
+    }
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(s));
     return EWG_OK;
 }
 

@@ -29,6 +29,7 @@ struct ewg_ctx {
     long long B = 0;
     long long side_off[7] = {0};
     hipStream_t stream = nullptr;
+    int device = 0;  // the device current at ewg_create (every buffer lives there)
     // device state
     double *x0 = nullptr, *x1 = nullptr, *xcur = nullptr, *sto = nullptr;
     double *R = nullptr, *J = nullptr;

@@ -210,6 +210,19 @@ int ewg_get_preconditioner(ewg_ctx *ctx, double *lu);
  * (a solve or ewg_get_preconditioner); d and v are 2N doubles, AoS. */
 int ewg_apply_preconditioner(ewg_ctx *ctx, const double *d, double *v);
 
+/* No counterpart in the reference: ewg_apply_preconditioner for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). d and v are DEVICE pointers (2N doubles, AoS) on the context's device; the
+ * sweeps' scratch (tile tickets, tile edge buffers, the forward intermediate) is a
+ * fixed 48 MiB array per device set up by ewg_create, so the call allocates nothing.
+ * Grids whose tile edges exceed 16 MiB or whose intermediate exceeds 32 MiB are rejected
+ * with EWG_INVALID_ARGUMENT (128x128x64 fits; ilu_schedule 2 and 3 are rejected). The
+ * call runs on the context's stream and synchronizes it before returning: the caller
+ * orders its own writes of d before the call. Calls on one device share the scratch and
+ * take a per-device lock; work of other processes or unlocked code on that scratch
+ * would race. */
+int ewg_apply_preconditioner_noalloc(ewg_ctx *ctx, const double *d, double *v);
+
 /* OverlappingOperator::apply (ewoms/numerics/linear/overlappingoperator.hh:59-63),
  * i.e. BCRSMatrix::mv: y = J x with the Jacobian of the last linearization; x and y are
  * 2N doubles, AoS. */
