@@ -192,21 +192,17 @@ void sample_collect(ewg_ctx *c) {
     }
 }
 
-bool use_wavefront(const ewg_ctx *c) {
-    return c->g.dim == 3 && c->prm.ilu_schedule != 1 && c->g.N <= kWfMaxCells &&
-           c->wf.ticket != nullptr;
-}
-
-// schedule 0: the sweeps read the factors in the wavefront layout (its buffers exist
-// when the layout fits 32-bit byte offsets; otherwise the single-wavefront tile sweep)
+// ilu_schedule 0 on a 3D grid: the factors and sweeps in the tiles' wavefront layout
+// (its buffers exist when the layout fits 32-bit byte offsets); otherwise the hyperplane
+// launches (2D grids, ilu_schedule 1, grids beyond the layout), with the same results
 bool use_wavefront_layout(const ewg_ctx *c) {
-    return use_wavefront(c) && c->prm.ilu_schedule == 0 && c->wl64.L &&
-           (!c->prm.precond_fp32 || c->wl32.L);
+    return c->g.dim == 3 && c->prm.ilu_schedule == 0 && c->g.N <= kWfMaxCells &&
+           c->wf.ticket != nullptr && c->wl64.L && (!c->prm.precond_fp32 || c->wl32.L);
 }
 
 // The factorization always runs in FP64; the FP32 preconditioner stores the factors and
 // the upper blocks rounded to FP32 (one rounding of the exact-arithmetic-order FP64
-// factors, the same for every schedule).
+// factors, the same for both schedules).
 void precond_factor(ewg_ctx *c) {
     const bool fp32 = c->prm.precond_fp32;
     if (use_wavefront_layout(c)) {
@@ -217,17 +213,7 @@ void precond_factor(ewg_ctx *c) {
             wl_factor64(c->g, c->wf, c->J, c->wl_af, c->wl64, c->ilu_err, c->stream);
         return;
     }
-    if (use_wavefront(c))
-        wf_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->wf, c->stream);
-    else
-        launch_ilu_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->stream);
-    if (use_wavefront_layout(c)) {
-        if (fp32)
-            wl_pack32(c->g, c->wf, c->L64, c->D64, c->J, c->wl32, c->stream);
-        else
-            wl_pack64(c->g, c->wf, c->L64, c->D64, c->J, c->wl64, c->stream);
-        return;
-    }
+    launch_ilu_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->stream);
     if (fp32) launch_round_factors(c->g, c->L64, c->D64, c->J, c->L32, c->D32, c->U32, c->stream);
 }
 
@@ -240,23 +226,6 @@ void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status,
             wl_apply32(c->g, c->wf, c->wl32, d, v, status, c->stream, tickets_clear);
         else
             wl_apply64(c->g, c->wf, c->wl64, d, v, status, c->stream, tickets_clear);
-        return;
-    }
-    if (use_wavefront(c)) {
-        const int sched = c->prm.ilu_schedule;
-        if (sched == 2) {
-            if (c->prm.precond_fp32)
-                wf_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->wf, c->stream);
-            else
-                wf_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->wf, c->stream);
-            return;
-        }
-        const int variant = 8;  // single-wavefront tiles, natural layout
-        if (c->prm.precond_fp32)
-            sweep_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->wf, variant,
-                          c->stream);
-        else
-            sweep_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->wf, variant, c->stream);
         return;
     }
     if (c->prm.precond_fp32)
@@ -617,7 +586,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     if (material->nregions < 1 || !material->region_params) return EWG_INVALID_ARGUMENT;
     if (material->law != EWG_LAW_VAN_GENUCHTEN && material->law != EWG_LAW_BROOKS_COREY)
         return EWG_INVALID_ARGUMENT;
-    if (params->ilu_schedule < 0 || params->ilu_schedule > 3) return EWG_INVALID_ARGUMENT;
+    if (params->ilu_schedule < 0 || params->ilu_schedule > 1) return EWG_INVALID_ARGUMENT;
     if (params->linearize_fp32 != 0 && params->linearize_fp32 != 1) return EWG_INVALID_ARGUMENT;
     auto *c = new ewg_ctx();
     c->grid = *grid;
@@ -823,8 +792,8 @@ const char *ewg_last_error(const ewg_ctx *c) { return c ? c->err.c_str() : "null
 
 int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
     if (!c || !p) return EWG_INVALID_ARGUMENT;
-    if (p->ilu_schedule < 0 || p->ilu_schedule > 3)
-        return set_err(c, EWG_INVALID_ARGUMENT, "ilu_schedule must be 0..3");
+    if (p->ilu_schedule < 0 || p->ilu_schedule > 1)
+        return set_err(c, EWG_INVALID_ARGUMENT, "ilu_schedule must be 0 or 1");
     if (p->ilu_schedule != 1 && c->g.dim == 3 && c->g.N <= kWfMaxCells && !c->wf.ticket)
         return set_err(c, EWG_INVALID_ARGUMENT,
                        "a tile-wavefront schedule needs a context created with one");
@@ -1057,9 +1026,6 @@ int ewg_apply_preconditioner_noalloc(ewg_ctx *c, const double *d, double *v) {
     const Grid &g = c->g;
     const bool fp32 = c->prm.precond_fp32;
     const bool layout = use_wavefront_layout(c);
-    if (!layout && use_wavefront(c))
-        return set_err(c, EWG_INVALID_ARGUMENT,
-                       "the no-allocation apply runs ilu_schedule 0 (3D) or the hyperplane schedule");
     const size_t vec_bytes = layout ? wl_entries(g, c->wf) * 2 * (fp32 ? sizeof(float) : sizeof(double))
                                     : (fp32 ? 2 * (size_t)g.N * sizeof(float) : 0);
     if ((layout && 2 * c->wf.edge_words * 8 > kScratchEdges) || vec_bytes > kScratchVec)

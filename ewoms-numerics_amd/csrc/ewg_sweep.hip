@@ -1,8 +1,10 @@
 // ewg_sweep.hip -- the two ILU(0) triangular sweeps as persistent tile-wavefront launches,
 // tuned for instruction count (the sweeps run twice per BiCGStab half-iteration).
 //
-// Same schedule as k_wavefront in ewg_wavefront.hip (tile columns through z, skewed
-// local steps, tickets in anti-diagonal order, edge buffers with a sentinel), with:
+// Schedule: 8x8 tiles of z-columns, each tile's lanes skewed so that lane (lx, ly) does
+// z = step - lx - ly, tiles taken by ticket in anti-diagonal order (every producer is
+// resident before its consumers), cross-tile values through edge buffers with a
+// sentinel, with:
 //   - per-lane 32-bit byte offsets that advance by a constant per step; a lane whose
 //     cell is outside the grid (z < 0, z >= nz, or outside the plane) lands out of the
 //     buffer range, so the hardware drops its stores and zeroes its loads: no
@@ -134,22 +136,6 @@ __device__ __forceinline__ void mmv(const T *m, P2<T> v, T &r0, T &r1) {
     r1 -= m[3] * v.y;
 }
 
-template <class T>
-struct SweepArgs {
-    int nx, ny, nz, nxny, N;
-    // forward: rhs = d (FP64), blocks = L slots 0..2 (-z,-y,-x), w = result (T)
-    // backward: rhs = w (T, forward result), blocks = U slots (+x,+y,+z), Dinv;
-    //           result -> w (T) and out (FP64; may alias w when T is double)
-    const double *d;
-    const void *blk[3];
-    unsigned blk_bytes;     // bytes per block of the backward sweep's U arrays
-    const T *Dinv;
-    T *w;
-    double *out;
-    const int *status;
-    WfSync sync;
-};
-
 // arguments of the warp-specialized sweep, whose factors live in the wavefront layout:
 // entry ((tile * nsteps + step) * 64 + lane) holds the cell that lane of that tile
 // reaches at that local step (forward order for L and the forward result, backward
@@ -175,226 +161,11 @@ struct In {
     T di[4];
 };
 
-template <class T, bool FWD, int TX, int TY, int PD, int PE>
-__global__ void __launch_bounds__(TX *TY) k_sweep(SweepArgs<T> a) {
-    constexpr int NT = TX * TY;
-    constexpr int NWAVE = NT / 64;
-    static_assert(NT % 64 == 0, "whole wavefronts");
-    static_assert(PD % PE == 0 && PE <= PD, "edge ring must divide the operand ring");
-    __shared__ P2<T> xch[2][NT];
-    __shared__ unsigned s_tk;
-    if (a.status && *a.status != KS_RUNNING) return;
-    const int tid = threadIdx.x;
-    if (tid == 0) s_tk = atomicAdd(a.sync.ticket, 1u);
-    __syncthreads();
-    const unsigned tk = s_tk;
-    if (tk >= (unsigned)a.sync.ntiles) return;
-    const int2 tile = a.sync.order[tk];
-    const int tx = tile.x, ty = tile.y;
-    const int lx = tid % TX, ly = tid / TX;
-    const int xs = tx * TX + lx, ys = ty * TY + ly;
-    const bool inplane = xs < a.nx && ys < a.ny;
-    const int nz = a.nz;
-    const int tileId = ty * a.sync.ntx + tx;
-    const bool consX = inplane && lx == 0 && tx > 0;
-    const bool consY = inplane && ly == 0 && ty > 0;
-    const bool prodX = inplane && lx == TX - 1 && tx + 1 < a.sync.ntx;
-    const bool prodY = inplane && ly == TY - 1 && ty + 1 < a.sync.nty;
-
-    // cell of (xs, ys, z) = c0 + z*dz in sweep coordinates
-    int c0, dz;
-    if (FWD) {
-        c0 = xs + a.nx * ys;
-        dz = a.nxny;
-    } else {
-        c0 = (a.nx - 1 - xs) + a.nx * (a.ny - 1 - ys) + a.nxny * (nz - 1);
-        dz = -a.nxny;
-    }
-    // byte offsets at local step 0 (z = -lx-ly), advanced by a constant per step
-    const int cs0 = c0 - (lx + ly) * dz;  // cell of local step 0
-    const unsigned bB = FWD ? 4 * sizeof(T) : a.blk_bytes;  // block array stride per cell
-    const unsigned vB = 2 * sizeof(T);                     // result vector stride per cell
-    const unsigned diB = 4 * sizeof(T);
-    // current-step offsets (stores) and prefetch offsets (loads, PD steps ahead)
-    unsigned ovec = inplane ? (unsigned)(cs0 * (int)vB) : kFar;
-    unsigned oout = inplane ? (unsigned)(cs0 * 16) : kFar;
-    unsigned pblk = inplane ? (unsigned)(cs0 * (int)bB) : kFar;
-    unsigned pdi = inplane ? (unsigned)(cs0 * (int)diB) : kFar;
-    unsigned pvec = ovec, pd = oout;
-    // lanes outside the plane stay at kFar (their offsets never move into range)
-    const int dzl = inplane ? dz : 0;
-    const unsigned sblk = (unsigned)(dzl * (int)bB), sdi = (unsigned)(dzl * (int)diB),
-                   svec = (unsigned)(dzl * (int)vB), sd = (unsigned)(dzl * 16);
-
-    const size_t N = (size_t)a.N;
-    const __amdgpu_buffer_rsrc_t rb0 = rsrc(a.blk[0], N * bB), rb1 = rsrc(a.blk[1], N * bB),
-                                 rb2 = rsrc(a.blk[2], N * bB);
-    const __amdgpu_buffer_rsrc_t rdi = rsrc(a.Dinv, FWD ? 0 : N * diB);
-    const __amdgpu_buffer_rsrc_t rw = rsrc(a.w, N * vB);
-    const __amdgpu_buffer_rsrc_t rd = rsrc(a.d, FWD ? N * 16 : 0);
-    const __amdgpu_buffer_rsrc_t rout = rsrc(a.out, a.out ? N * 16 : 0);
-    const __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
-    const __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
-    constexpr unsigned zw = kWfMaxWords * 8;  // bytes per z of one edge lane
-    const unsigned exIn = consX ? (unsigned)((((size_t)(tileId - 1) * TY + ly) * nz) * zw) : 0u;
-    const unsigned eyIn = consY ? (unsigned)((((size_t)(tileId - a.sync.ntx) * TX + lx) * nz) * zw) : 0u;
-    const unsigned exOut = prodX ? (unsigned)((((size_t)tileId * TY + ly) * nz) * zw) : 0u;
-    const unsigned eyOut = prodY ? (unsigned)((((size_t)tileId * TX + lx) * nz) * zw) : 0u;
-    // edge-word load offsets of local step 0, advanced by zw per step: a consumer lane
-    // reads its column's word of z = step - lx - ly (steps outside 0..nz-1 read words
-    // that are never used), every other lane stays out of range
-    unsigned lex = consX ? exIn - (unsigned)(lx + ly) * zw : kFar;
-    unsigned ley = consY ? eyIn - (unsigned)(lx + ly) * zw : kFar;
-    const unsigned sex = consX ? zw : 0u, sey = consY ? zw : 0u;
-    const int nsteps = (TX - 1) + (TY - 1) + nz;
-
-    In<T, FWD> ring[PD];
-    EW<T> ex[PE], ey[PE];
-
-    // operands of the step the p* offsets point at; steps past the end read out of
-    // range and are never used
-    auto load = [&](In<T, FWD> &in) {
-        if (FWD) {
-            P2<double> dv = ldp<double>(rd, pd);
-            in.rhs = P2<T>{static_cast<T>(dv.x), static_cast<T>(dv.y)};
-        } else {
-            in.rhs = ldp<T>(rw, pvec);
-        }
-        ldb<T>(rb0, pblk, in.b[0]);
-        ldb<T>(rb1, pblk, in.b[1]);
-        ldb<T>(rb2, pblk, in.b[2]);
-        if (!FWD) ldb<T>(rdi, pdi, in.di);
-        pblk += sblk;
-        pvec += svec;
-        pdi += sdi;
-        pd += sd;
-    };
-    // cross-tile words of the step the le* offsets point at. They are loaded only PE
-    // steps ahead: a consumer runs just behind its producer, and a word read earlier
-    // would still hold the sentinel and cost a round trip to poll.
-    auto load_edge = [&](EW<T> &e1, EW<T> &e2) {
-        e1 = lde<T>(rex, lex);
-        e2 = lde<T>(rey, ley);
-        lex += sex;
-        ley += sey;
-    };
-
-    // The scheduling barriers keep the prefetches in step order: the wait for a slot
-    // then counts every later step's memory operations, on the first pass through the
-    // loop as on the others (a reordered prologue makes the waitcnt insertion assume
-    // the operands of the first steps are young and wait for nearly all loads).
-#pragma unroll
-    for (int k = 0; k < PD; ++k) {
-        load(ring[k]);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-#pragma unroll
-    for (int k = 0; k < PE; ++k) {
-        load_edge(ex[k], ey[k]);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-
-    P2<T> prev = P2<T>{T(0), T(0)};
-    EW<T> sent;
-    sent.w[0] = kSentinel;
-    if constexpr (sizeof(T) == 8) sent.w[1] = kSentinel;
-    const bool hasL = lx > 0, hasD = ly > 0;
-    const int ixl = hasL ? tid - 1 : tid, ixd = hasD ? tid - TX : tid;
-
-    for (int tau0 = 0; tau0 < nsteps; tau0 += PD) {
-#pragma unroll
-        for (int k = 0; k < PD; ++k) {
-            const int tau = tau0 + k;
-            const int z = tau - lx - ly;
-            const bool active = inplane && z >= 0 && z < nz;
-            const bool needX = active && consX, needY = active && consY;
-            const unsigned offX = needX ? exIn + (unsigned)z * zw : kFar;
-            const unsigned offY = needY ? eyIn + (unsigned)z * zw : kFar;
-            // cross-tile values of this step; poll while not yet published
-            EW<T> cx = ex[k % PE], cy = ey[k % PE];
-            bool wait = (needX && is_sentinel(cx)) || (needY && is_sentinel(cy));
-            if (__builtin_amdgcn_ballot_w64(wait) != 0ull) {
-                unsigned spins = 0;
-                while (__builtin_amdgcn_ballot_w64(wait) != 0ull && spins < kSpinLimit) {
-                    __builtin_amdgcn_s_sleep(1);
-                    const EW<T> nx2 = lde<T>(rex, offX), ny2 = lde<T>(rey, offY);
-                    if (needX) cx = nx2;
-                    if (needY) cy = ny2;
-                    wait = (needX && is_sentinel(cx)) || (needY && is_sentinel(cy));
-                    ++spins;
-                    if ((spins & 63) == 0 &&
-                        __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                        break;
-                }
-                if (wait) atomicExch(a.sync.err, 1);
-            }
-            // recycle the consumed edge words
-            ste<T>(rex, offX, sent);
-            ste<T>(rey, offY, sent);
-            // neighbour values in sweep order: -z (own previous step), -y, -x. Lanes
-            // without an in-tile neighbour take the edge word, which reads as zero
-            // where there is no cross-tile neighbour either (the matching block is
-            // zero then); the z selection keeps values of steps before z = 0 out.
-            const int rbuf = (tau + 1) & 1, wbuf = tau & 1;
-            const P2<T> fx = xch[rbuf][ixl];
-            const P2<T> fy = xch[rbuf][ixd];
-            const P2<T> evx = ew_val(cx), evy = ew_val(cy);
-            P2<T> nx_, ny_, nz_;
-            nx_.x = hasL ? fx.x : evx.x;
-            nx_.y = hasL ? fx.y : evx.y;
-            ny_.x = hasD ? fy.x : evy.x;
-            ny_.y = hasD ? fy.y : evy.y;
-            nz_.x = z > 0 ? prev.x : T(0);
-            nz_.y = z > 0 ? prev.y : T(0);
-            const In<T, FWD> &c = ring[k];
-            T r0 = c.rhs.x, r1 = c.rhs.y;
-            P2<T> res;
-            if (FWD) {
-                // ascending columns: -z, -y, -x
-                mmv(c.b[0], nz_, r0, r1);
-                mmv(c.b[1], ny_, r0, r1);
-                mmv(c.b[2], nx_, r0, r1);
-                res = P2<T>{r0, r1};
-            } else {
-                // ascending columns: +x, +y, +z (sweep -x, -y, -z)
-                mmv(c.b[0], nx_, r0, r1);
-                mmv(c.b[1], ny_, r0, r1);
-                mmv(c.b[2], nz_, r0, r1);
-                T s0 = T(0), s1 = T(0);
-                s0 += c.di[0] * r0; s0 += c.di[1] * r1;
-                s1 += c.di[2] * r0; s1 += c.di[3] * r1;
-                res = P2<T>{s0, s1};
-            }
-            stp<T>(rw, ovec, res);
-            if (!FWD && sizeof(T) == 4)
-                stp<double>(rout, oout,
-                            P2<double>{static_cast<double>(res.x), static_cast<double>(res.y)});
-            ste<T>(rex, prodX && active ? exOut + (unsigned)z * zw : kFar, val_ew(res));
-            ste<T>(rey, prodY && active ? eyOut + (unsigned)z * zw : kFar, val_ew(res));
-            prev = res;
-            xch[wbuf][tid] = res;
-            if (NWAVE > 1) {
-                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the LDS store is done
-                __builtin_amdgcn_s_barrier();
-            }
-            __builtin_amdgcn_wave_barrier();
-            ovec += svec;
-            oout += sd;
-            // operands of step tau + PD and edge words of step tau + PE into the slots
-            // just consumed
-            // (edge words first: the wait for them then covers only older operands)
-            load_edge(ex[k % PE], ey[k % PE]);
-            __builtin_amdgcn_sched_barrier(0);
-            load(ring[k]);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-}
-
 // ----------------------------------------------------------------------------------
 // Warp-specialized sweep: one 8x8 tile per workgroup of two wavefronts.
-//   wave 0 computes the tile's columns as k_sweep does, but reads the values of the
-//          upstream tiles from an LDS ring instead of global memory, so that its vector
+//   wave 0 computes the tile's columns (lane (lx, ly) one z-column, the in-tile
+//          neighbour values through a double-buffered LDS array) and reads the values of
+//          the upstream tiles from an LDS ring instead of global memory, so that its vector
 //          memory counter only holds operand prefetches issued PD steps earlier and its
 //          own stores (a late cross-tile load would make every wait on it also wait for
 //          all older loads and stores). It stores its last column and row straight to
@@ -582,7 +353,8 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
         if (!FWD) ldb<T>(rdi, pblk, in.di);
         pblk += sblk;
     };
-    // the scheduling barriers keep the prefetches in step order (see k_sweep)
+    // the scheduling barriers keep the prefetches in step order: the in-order vector
+    // memory counter then lets each step wait for its own operands only
 #pragma unroll
     for (int k = 0; k < PD; ++k) {
         load(ring[k]);
@@ -718,47 +490,6 @@ __global__ void k_reset_ticket_pair(unsigned *ticket) {
     ticket[0] = 0u;
     ticket[1] = 0u;
 }
-
-// Copy the factors into the wavefront layout: one thread per layout entry. FWD: L (3
-// lower slots, forward order); otherwise U (3 upper slots; UT is double for the
-// Jacobian's blocks) and Dinv (backward order). Entries without a cell get zeros.
-template <class T, class S, bool FWD>
-__global__ void __launch_bounds__(256)
-k_wl_pack(int nx, int ny, int nz, int ntx, int nsteps, unsigned entries, size_t N, const S *L,
-          const S *U0, const S *U1, const S *U2, const S *Dinv, T *o0, T *o1, T *o2, T *od) {
-    const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= entries) return;
-    const int lane = p & 63, q = p >> 6;
-    const int step = q % nsteps, tileId = q / nsteps;
-    const int lx = lane & 7, ly = lane >> 3;
-    int x = (tileId % ntx) * 8 + lx, y = (tileId / ntx) * 8 + ly, z = step - lx - ly;
-    const bool ok = x < nx && y < ny && z >= 0 && z < nz;
-    if (!FWD) {
-        x = nx - 1 - x;
-        y = ny - 1 - y;
-        z = nz - 1 - z;
-    }
-    const size_t cell = ok ? (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z) : 0;
-    // values of type S (the factorization's), stored as T (FP32 rounds to nearest)
-    auto put = [&](T *o, const S *src) {
-        T v[4] = {T(0), T(0), T(0), T(0)};
-        if (ok) {
-            for (int e = 0; e < 4; ++e) v[e] = static_cast<T>(src[cell * 4 + e]);
-        }
-        for (int e = 0; e < 4; ++e) o[(size_t)p * 4 + e] = v[e];
-    };
-    if (FWD) {
-        put(o0, L);
-        put(o1, L + N * 4);
-        put(o2, L + 2 * N * 4);
-    } else {
-        put(o0, U0);
-        put(o1, U1);
-        put(o2, U2);
-        put(od, Dinv);
-    }
-}
-
 
 // ----------------------------------------------------------------------------------
 // The factorization straight into the wavefront layout, warp-specialized like the
@@ -1185,83 +916,8 @@ k_wl_pack_factor_batch(const FwsArgs<T> *args, const int *pmem, const int *poff)
 }
 }  // namespace
 
-template <class T, bool FWD>
-static void launch_sweep(const SweepArgs<T> &a, int variant, hipStream_t s) {
-    hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, a.sync.ticket);
-    if (variant == 0) return;  // the wavefront-layout path has its own launcher
-    else
-        hipLaunchKernelGGL((k_sweep<T, FWD, 8, 8, kSweepPD, kSweepPE>), dim3(a.sync.ntiles), dim3(64), 0, s, a);
-}
-
-void sweep_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
-                   const double *d, double *v, const int *status, const WfSync &sync,
-                   int variant, hipStream_t s) {
-    SweepArgs<double> f{};
-    f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N;
-    f.d = d;
-    f.blk[0] = L; f.blk[1] = L + (size_t)g.N * 4; f.blk[2] = L + (size_t)2 * g.N * 4;
-    f.blk_bytes = 32;
-    f.Dinv = nullptr; f.w = v; f.out = nullptr; f.status = status; f.sync = sync;
-    launch_sweep<double, true>(f, variant, s);
-    SweepArgs<double> b = f;
-    b.d = nullptr;
-    b.blk[0] = A + (size_t)(g.diag + 1) * g.N * 4;
-    b.blk[1] = A + (size_t)(g.diag + 2) * g.N * 4;
-    b.blk[2] = A + (size_t)(g.diag + 3) * g.N * 4;
-    b.Dinv = Dinv;
-    launch_sweep<double, false>(b, variant, s);
-}
-
-void sweep_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
-                   const double *d, float *w, double *v, const int *status, const WfSync &sync,
-                   int variant, hipStream_t s) {
-    SweepArgs<float> f{};
-    f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N;
-    f.d = d;
-    f.blk[0] = L; f.blk[1] = L + (size_t)g.N * 4; f.blk[2] = L + (size_t)2 * g.N * 4;
-    f.blk_bytes = 16;
-    f.Dinv = nullptr; f.w = w; f.out = nullptr; f.status = status; f.sync = sync;
-    launch_sweep<float, true>(f, variant, s);
-    SweepArgs<float> b = f;
-    b.d = nullptr;
-    b.blk[0] = U; b.blk[1] = U + (size_t)g.N * 4; b.blk[2] = U + (size_t)2 * g.N * 4;
-    b.Dinv = Dinv;
-    b.out = v;
-    launch_sweep<float, false>(b, variant, s);
-}
-
 size_t wl_entries(const Grid &g, const WfSync &sync) {
     return (size_t)sync.ntiles * (size_t)(g.nz + 14) * 64;
-}
-
-template <class T, class S>
-static void wl_pack(const Grid &g, const WfSync &sync, const S *L, const S *Dinv, const S *U0,
-                    const S *U1, const S *U2, const WlPrecond<T> &wl, hipStream_t s) {
-    const unsigned entries = (unsigned)wl_entries(g, sync);
-    const int nsteps = g.nz + 14;
-    const dim3 grid((entries + 255) / 256);
-    hipLaunchKernelGGL((k_wl_pack<T, S, true>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz, sync.ntx,
-                       nsteps, entries, (size_t)g.N, L, U0, U1, U2, Dinv, wl.L,
-                       wl.L + (size_t)entries * 4, wl.L + (size_t)2 * entries * 4, nullptr);
-    hipLaunchKernelGGL((k_wl_pack<T, S, false>), grid, dim3(256), 0, s, g.nx, g.ny, g.nz,
-                       sync.ntx, nsteps, entries, (size_t)g.N, L, U0, U1, U2, Dinv, wl.U,
-                       wl.U + (size_t)entries * 4, wl.U + (size_t)2 * entries * 4, wl.D);
-}
-
-void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
-               const double *A, const WlPrecond<double> &wl, hipStream_t s) {
-    const size_t N = g.N;
-    wl_pack<double, double>(g, sync, L, Dinv, A + (size_t)(g.diag + 1) * N * 4,
-                            A + (size_t)(g.diag + 2) * N * 4, A + (size_t)(g.diag + 3) * N * 4,
-                            wl, s);
-}
-
-void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
-               const double *A, const WlPrecond<float> &wl, hipStream_t s) {
-    const size_t N = g.N;
-    wl_pack<float, double>(g, sync, L, Dinv, A + (size_t)(g.diag + 1) * N * 4,
-                           A + (size_t)(g.diag + 2) * N * 4, A + (size_t)(g.diag + 3) * N * 4,
-                           wl, s);
 }
 
 template <class T>

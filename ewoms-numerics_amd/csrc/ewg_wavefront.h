@@ -1,4 +1,5 @@
-// ewg_wavefront.h -- persistent tile-wavefront launches for the 3D ILU(0) recurrences.
+// ewg_wavefront.h -- persistent tile-wavefront launches for the 3D ILU(0) recurrences
+// (factorization and sweeps in ewg_sweep.hip, edge-buffer set-up in ewg_wavefront.hip).
 #pragma once
 
 #include <vector>
@@ -11,8 +12,6 @@ namespace ewg {
 
 constexpr int kWfTX = 8;   // tile extent in x (lanes)
 constexpr int kWfTY = 8;   // tile extent in y
-constexpr int kWfPD = 8;   // prefetch depth (local steps) of each lane's operands
-constexpr int kWfPDFactor = 4;  // factorization operands are twice as wide
 constexpr int kFactorPD = 3;    // prefetch depth of the wavefront-layout factorization
 constexpr int kSweepPD = 6;     // prefetch depth of the triangular sweeps' operands
 constexpr int kSweepPE = 1;     // prefetch depth of their cross-tile words
@@ -39,24 +38,6 @@ struct WfSync {
 // fill both edge buffers with the sentinel (context creation, and after an error)
 void wf_init_edges(const WfSync &sync, hipStream_t s);
 
-void wf_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
-                 const WfSync &sync, hipStream_t s);
-void wf_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
-                const double *d, double *v, const int *status, const WfSync &sync,
-                hipStream_t s);
-void wf_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
-                const double *d, float *w, double *v, const int *status, const WfSync &sync,
-                hipStream_t s);
-
-// the same sweeps by the single-wavefront tile kernel of ewg_sweep.hip on the 8x8 tiles
-// of sync (variant 8)
-void sweep_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
-                   const double *d, double *v, const int *status, const WfSync &sync,
-                   int variant, hipStream_t s);
-void sweep_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
-                   const double *d, float *w, double *v, const int *status, const WfSync &sync,
-                   int variant, hipStream_t s);
-
 // The ILU(0) factors in the wavefront layout of the 8x8 tiles of sync: entry
 // ((tile * (nz + 14) + step) * 64 + lane) holds the 2x2 block of the cell that lane of
 // that tile reaches at that local step (L in forward order, U and Dinv in backward
@@ -69,12 +50,6 @@ struct WlPrecond {
     T *w;  // [entries][2] forward-sweep result
 };
 size_t wl_entries(const Grid &g, const WfSync &sync);
-// copy freshly factored L, Dinv (FP64) and the Jacobian's upper blocks into the layout,
-// as FP64 or rounded to FP32
-void wl_pack64(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
-               const double *A, const WlPrecond<double> &wl, hipStream_t s);
-void wl_pack32(const Grid &g, const WfSync &sync, const double *L, const double *Dinv,
-               const double *A, const WlPrecond<float> &wl, hipStream_t s);
 // the factorization straight into the layout (FP64 arithmetic; FP32 rounds the stored
 // factors and upper blocks); entries without a cell keep what they hold (zeros). Af:
 // scratch for the operands in the forward layout, 7 x wl_entries x 4 doubles

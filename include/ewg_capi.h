@@ -98,12 +98,11 @@ typedef struct ewg_params {
     int32_t precond_fp32;              /* 0: FP64 ILU0 (library default); 1: FP32 ILU0 factors
                                           and preconditioner inside FP64 BiCGStab */
     int32_t ilu_schedule;              /* ILU(0) schedule on 3D grids (2D grids always use
-                                          hyperplane launches); every choice gives the same
-                                          results: 0 tile wavefront, 8x8 tiles with an edge
-                                          wavefront each (default); 1 one launch per
-                                          hyperplane x+y+z = const; 2 tile wavefront, generic
-                                          sweep kernel; 3 tile wavefront, one wavefront per
-                                          tile. The tile schedules need their buffers, which
+                                          hyperplane launches); both give the same results:
+                                          0 tile wavefront, 8x8 tiles with an edge wavefront
+                                          each, factors in the tiles' wavefront layout
+                                          (default); 1 one launch per hyperplane
+                                          x+y+z = const. Schedule 0 needs its buffers, which
                                           ewg_create allocates unless it is given 1 */
     int32_t kernel_events;             /* 1: bracket sampled launches with device events (the
                                           first preconditioner apply and SpMV of each batch

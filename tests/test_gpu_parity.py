@@ -140,17 +140,17 @@ def test_fp32_preconditioner_close_to_fp64():
 @pytest.mark.parametrize("shape", [(20, 13, 7), (32, 24, 16), (8, 8, 3), (37, 21, 5)])
 @pytest.mark.parametrize("fp32", [0, 1])
 def test_wavefront_ilu_matches_hyperplane_schedule(shape, fp32):
-    """The tile-wavefront ILU(0) schedules (one persistent launch per factorization /
-    sweep; 8x8 and 16x16 tiles, lean and generic sweep kernels) and the
-    one-launch-per-hyperplane schedule run the same arithmetic in the same order: the
-    whole BiCGStab solve is bit-identical (partial tiles included)."""
+    """The tile-wavefront ILU(0) schedule (one persistent launch per factorization and
+    sweep, factors in the tiles' wavefront layout) and the one-launch-per-hyperplane
+    schedule run the same arithmetic in the same order: the whole BiCGStab solve is
+    bit-identical (partial tiles included)."""
     import ewg
     from ewg import problems
     nx, ny, nz = shape
     P = problems.heterogeneous_3d(nx, ny, nz, seed=21, patch=4)
     x = problems.perturbed_state(P, seed=22, dS=0.05, dp=50.0)
     out = {}
-    for sched in (1, 0, 2, 3):
+    for sched in (1, 0):
         print(f"shape {shape} fp32 {fp32} schedule {sched}", flush=True)
         prm = ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-8,
                                  precond_fp32=fp32, ilu_schedule=sched)
@@ -160,7 +160,7 @@ def test_wavefront_ilu_matches_hyperplane_schedule(shape, fp32):
         out[sched] = g.solve()
     rc1, dx1, it1 = out[1]
     assert rc1 == 0
-    for sched in (0, 2, 3):
+    for sched in (0,):
         rc, dx, it = out[sched]
         assert rc == 0 and it == it1, (sched, rc, it, it1)
         assert np.array_equal(dx, dx1), sched
