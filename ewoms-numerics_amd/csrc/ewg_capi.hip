@@ -83,8 +83,6 @@ int noalloc_setup(int dev, hipStream_t s) {
     return EWG_OK;
 }
 
-__global__ void k_flag_clear(int *flag) { *flag = 0; }
-
 template <class T>
 hipError_t dalloc(T **p, size_t count) {
     return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(count, 1) * sizeof(T));
@@ -95,7 +93,7 @@ void free_all(ewg_ctx *c) {
                     c->pe, c->region,
                     c->regs, c->bc_type, c->bc_val, c->bc_aux, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
-                    c->part2, c->dscalar, c->anyNonzero, c->ilu_err, c->ks, c->partAny,
+                    c->part2, c->dscalar, c->ilu_err, c->ks, c->ns, c->partAny,
                     c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wl64.L, c->wl64.U,
                     c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w,
                     c->wl_af};
@@ -110,6 +108,7 @@ void free_all(ewg_ctx *c) {
     if (c->warm_e0) (void)hipEventDestroy(c->warm_e0);
     if (c->warm_e1) (void)hipEventDestroy(c->warm_e1);
     if (c->ks_host) (void)hipHostFree(c->ks_host);
+    if (c->ns_host) (void)hipHostFree(c->ns_host);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
     if (c->err_host) (void)hipHostFree(c->err_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -152,17 +151,6 @@ double now_s() {
         .count();
 }
 
-// max |v| over n doubles, to the host
-int device_max_abs(ewg_ctx *c, const double *v, double *out) {
-    launch_max_abs(2 * c->g.N, v, c->part0, c->stream);
-    launch_reduce_max(c->part0, c->dscalar, c->stream);
-    HIP_TRY(c, hipMemcpyAsync(c->scal_host, c->dscalar, sizeof(double), hipMemcpyDeviceToHost,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    *out = c->scal_host[0];
-    return EWG_OK;
-}
-
 // Sampled launches: sample_begin/sample_end bracket one launch with events on the
 // context's stream (one sample per kind in flight); sample_collect, called after a
 // stream synchronisation, adds the finished ones to the totals.
@@ -179,6 +167,11 @@ void sample_end(ewg_ctx *c, int which) {
     if (!sm.armed) return;
     sm.armed = false;
     if (hipEventRecord(sm.e1, c->stream) == hipSuccess) sm.pending = true;
+}
+// a sample whose launches were gated off on the device: not a duration of the kernel
+void sample_drop(ewg_ctx *c, int which) {
+    c->samples[which].armed = false;
+    c->samples[which].pending = false;
 }
 void sample_collect(ewg_ctx *c) {
     for (auto &sm : c->samples) {
@@ -203,18 +196,20 @@ bool use_wavefront_layout(const ewg_ctx *c) {
 // The factorization always runs in FP64; the FP32 preconditioner stores the factors and
 // the upper blocks rounded to FP32 (one rounding of the exact-arithmetic-order FP64
 // factors, the same for both schedules).
-void precond_factor(ewg_ctx *c) {
+// gate (optional): device flag; the launches return at once while it is 0
+void precond_factor(ewg_ctx *c, const int *gate = nullptr) {
     const bool fp32 = c->prm.precond_fp32;
     if (use_wavefront_layout(c)) {
         // straight into the sweeps' layout (no natural-order factors)
         if (fp32)
-            wl_factor32(c->g, c->wf, c->J, c->wl_af, c->wl32, c->ilu_err, c->stream);
+            wl_factor32(c->g, c->wf, c->J, c->wl_af, c->wl32, c->ilu_err, c->stream, gate);
         else
-            wl_factor64(c->g, c->wf, c->J, c->wl_af, c->wl64, c->ilu_err, c->stream);
+            wl_factor64(c->g, c->wf, c->J, c->wl_af, c->wl64, c->ilu_err, c->stream, gate);
         return;
     }
-    launch_ilu_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->stream);
-    if (fp32) launch_round_factors(c->g, c->L64, c->D64, c->J, c->L32, c->D32, c->U32, c->stream);
+    launch_ilu_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->stream, gate);
+    if (fp32)
+        launch_round_factors(c->g, c->L64, c->D64, c->J, c->L32, c->D32, c->U32, c->stream, gate);
 }
 
 // tickets_clear: a kernel queued on the stream since the last apply has cleared the
@@ -234,88 +229,83 @@ void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status,
         launch_ilu_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->stream);
 }
 
-// BiCGStabSolver::apply on J dx = b (b = residual on the device). Returns EWG_OK,
-// EWG_NOT_CONVERGED or EWG_NUMERICAL_ISSUE; *iters as BiCGStab's report.
-int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
+// One BiCGStab iteration (bicgstabsolver.hh:191-322) of J x = b on the device: every
+// kernel returns at once when the device-side status has left KS_RUNNING. tk: the layout
+// sweeps' tickets, cleared by the vector update before each apply; sampled: the
+// iteration's first apply and SpMV are event-timed samples.
+void bicg_iteration(ewg_ctx *c, const double *b, double *x, unsigned *tk, bool sampled) {
+    const int n = 2 * c->g.N;
+    hipStream_t s = c->stream;
+    const int *st = &c->ks->status;
+    launch_ks_beta(c->ks, c->part2, s);
+    launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s, tk);
+    if (sampled) sample_begin(c, EWG_KERNEL_ILU_APPLY);
+    precond_apply(c, c->kp, c->ky, st, tk != nullptr);
+    if (sampled) {
+        sample_end(c, EWG_KERNEL_ILU_APPLY);
+        sample_begin(c, EWG_KERNEL_SPMV);
+    }
+    launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
+    if (sampled) sample_end(c, EWG_KERNEL_SPMV);
+    launch_ks_alpha(c->ks, c->part0, s);
+    launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s, tk);
+    launch_ks_crit1(c->ks, c->part1, c->partAny, s);
+    precond_apply(c, c->kr, c->kz, st, tk != nullptr);
+    launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
+    launch_ks_omega(c->ks, c->part0, c->part1, s);
+    launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
+    launch_ks_crit2(c->ks, c->partAny, s);
+}
+
+unsigned *layout_tickets(ewg_ctx *c) { return use_wavefront_layout(c) ? c->wf.ticket : nullptr; }
+
+// BiCGStabSolver::apply on J dx = b (b = residual on the device), first part: queues the
+// factorization, the initial state (x0 = 0, r = b), the first iteration and the copies the
+// host reads after its next wait (err_host[0]: factorization error, err_host[1]: initial
+// status, ks_host and err_host[2]: state and sweep error after the first iteration).
+// gate (optional): device flag; nothing of it runs while 0 (the state stays KS_IDLE).
+void bicgstab_begin(ewg_ctx *c, const double *b, double *x, const int *gate) {
     const int n = 2 * c->g.N;
     hipStream_t s = c->stream;
     double absTol = c->prm.linear_abs_tolerance;
     if (absTol < 0.0) absTol = c->prm.newton_tolerance / 100.0;
-    HIP_TRY(c, hipMemsetAsync(c->ilu_err, 0, sizeof(int), s));
-    HIP_TRY(c, hipMemsetAsync(c->anyNonzero, 0, sizeof(int), s));
+    launch_flag_clear(c->ilu_err, s);
     sample_begin(c, EWG_KERNEL_ILU_FACTOR);
-    precond_factor(c);
+    precond_factor(c, gate);
     sample_end(c, EWG_KERNEL_ILU_FACTOR);
-    c->have_precond = true;
-    launch_bicg_init(n, b, c->kr, x, c->kp, c->kv, c->part0, c->part1, s);
+    launch_bicg_init(n, b, c->kr, x, c->kp, c->kv, c->part0, c->part1, s, gate);
     launch_ks_init(c->ks, c->part0, c->part1, c->prm.linear_tolerance, absTol,
-                   c->prm.linear_max_error, s);
-    // read back together with the first batch of iterations (one host wait fewer): the
-    // factorization's error flag and the initial status (the batch's kernels return at
-    // once when the initial residual already meets the criterion)
-    HIP_TRY(c, hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipMemcpyAsync(c->err_host + 1, &c->ks->status, sizeof(int),
-                              hipMemcpyDeviceToHost, s));
+                   c->prm.linear_max_error, s, gate);
+    (void)hipMemcpyAsync(c->err_host, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(c->err_host + 1, &c->ks->status, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (c->prm.linear_max_iterations > 0) bicg_iteration(c, b, x, layout_tickets(c), false);
+    (void)hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(c->err_host + 2, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost, s);
+}
+
+// Second part, after the host has waited for the first: the remaining iterations are
+// queued speculatively and the state read back after 2, 4, then every 8 of them (the
+// iterations queued past convergence are no-ops, so the count kept on the device is the
+// one of a solver that stops exactly). Returns EWG_OK, EWG_NOT_CONVERGED or
+// EWG_NUMERICAL_ISSUE; *iters as BiCGStab's report.
+int bicgstab_end(ewg_ctx *c, const double *b, double *x, int *iters) {
+    hipStream_t s = c->stream;
+    sample_collect(c);
+    c->have_precond = true;
     *iters = 0;
-    if (c->prm.linear_max_iterations <= 0) {
-        HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
-                                  s));
+    if (c->err_host[0]) {
+        // the iterations after the failed factorization ran for nothing; a wavefront wait
+        // that ran out may have left words in the edge buffers
+        if (c->wf.ex) wf_init_edges(c->wf, s);
         HIP_TRY(c, hipStreamSynchronize(s));
-        sample_collect(c);
-        if (c->err_host[0]) return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
-        if (c->err_host[1] == KS_CONVERGED) return EWG_OK;
-        return set_err(c, EWG_NOT_CONVERGED, "linear solver did not converge");
+        return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
     }
-    const int *st = &c->ks->status;
-    // the vector update before each preconditioner apply clears the layout sweeps'
-    // tickets (one kernel launch fewer per sweep)
-    unsigned *tk = use_wavefront_layout(c) ? c->wf.ticket : nullptr;
-    // Iterations are queued speculatively and the state is read back after 1, 2, 4, then
-    // every 8 of them: every kernel of an iteration returns at once when the device-side
-    // status has left KS_RUNNING, so the iterations queued past convergence are no-ops
-    // and the count kept on the device is the one of a solver that stops exactly.
-    int batch = 1;
-    for (int it = 0; it < c->prm.linear_max_iterations;) {
-        const int nb = std::min(batch, c->prm.linear_max_iterations - it);
-        for (int q = 0; q < nb; ++q) {
-            // the first iteration of a later batch runs for certain (the state read back
-            // before it was KS_RUNNING): its first apply and SpMV are the sampled launches
-            const bool sampled = q == 0 && it > 0;
-            launch_ks_beta(c->ks, c->part2, s);
-            launch_p_update(n, c->ks, c->kr, c->kv, c->kp, s, tk);
-            if (sampled) sample_begin(c, EWG_KERNEL_ILU_APPLY);
-            precond_apply(c, c->kp, c->ky, st, tk != nullptr);
-            if (sampled) {
-                sample_end(c, EWG_KERNEL_ILU_APPLY);
-                sample_begin(c, EWG_KERNEL_SPMV);
-            }
-            launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
-            if (sampled) sample_end(c, EWG_KERNEL_SPMV);
-            launch_ks_alpha(c->ks, c->part0, s);
-            launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s, tk);
-            launch_ks_crit1(c->ks, c->part1, c->partAny, s);
-            precond_apply(c, c->kr, c->kz, st, tk != nullptr);
-            launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
-            launch_ks_omega(c->ks, c->part0, c->part1, s);
-            launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
-            launch_ks_crit2(c->ks, c->partAny, s);
-        }
-        const bool first = it == 0;
-        it += nb;
-        HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
-                                  s));
-        HIP_TRY(c, hipMemcpyAsync(c->err_host + 2, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost,
-                                  s));
-        HIP_TRY(c, hipStreamSynchronize(s));
-        sample_collect(c);
-        if (first && c->err_host[0]) {
-            // the iterations after the failed factorization ran for nothing; a wavefront
-            // wait that ran out may have left words in the edge buffers
-            if (c->wf.ex) wf_init_edges(c->wf, s);
-            HIP_TRY(c, hipStreamSynchronize(s));
-            return set_err(c, EWG_NUMERICAL_ISSUE, "Creating the preconditioner failed");
-        }
-        if (first && c->err_host[1] == KS_CONVERGED) return EWG_OK;
+    if (c->err_host[1] == KS_CONVERGED) return EWG_OK;
+    if (c->prm.linear_max_iterations <= 0)
+        return set_err(c, EWG_NOT_CONVERGED, "linear solver did not converge");
+    unsigned *tk = layout_tickets(c);
+    int it = 1, batch = 2;
+    while (true) {
         if (c->err_host[2]) {
             // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
             if (c->wf.ex) wf_init_edges(c->wf, s);
@@ -323,7 +313,16 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
             *iters = c->ks_host->iterations;
             return set_err(c, EWG_NUMERICAL_ISSUE, "ILU(0) sweep failed");
         }
-        if (c->ks_host->status != KS_RUNNING) break;
+        if (c->ks_host->status != KS_RUNNING || it >= c->prm.linear_max_iterations) break;
+        const int nb = std::min(batch, c->prm.linear_max_iterations - it);
+        for (int q = 0; q < nb; ++q) bicg_iteration(c, b, x, tk, q == 0);
+        it += nb;
+        HIP_TRY(c, hipMemcpyAsync(c->ks_host, c->ks, sizeof(KrylovState), hipMemcpyDeviceToHost,
+                                  s));
+        HIP_TRY(c, hipMemcpyAsync(c->err_host + 2, c->ilu_err, sizeof(int), hipMemcpyDeviceToHost,
+                                  s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        sample_collect(c);
         batch = std::min(batch * 2, 8);
     }
     *iters = c->ks_host->iterations;
@@ -332,6 +331,14 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
         case KS_BREAKDOWN: return set_err(c, EWG_NUMERICAL_ISSUE, "Breakdown of the BiCGStab solver");
         default: return set_err(c, EWG_NOT_CONVERGED, "linear solver did not converge");
     }
+}
+
+// BiCGStabSolver::apply on J dx = b for ewg_solve: both parts with the one wait between.
+int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
+    bicgstab_begin(c, b, x, nullptr);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return bicgstab_end(c, b, x, iters);
 }
 
 int linearize_device(ewg_ctx *c, double dt, int newton_iter) {
@@ -367,72 +374,93 @@ int ensure_fp32(ewg_ctx *c) {
     return EWG_OK;
 }
 
-bool proceed(const ewg_ctx *c) {  // newtonmethod.hh:866-884
-    if (c->num_iterations < 1) return true;
-    if (c->error <= c->prm.newton_tolerance) return false;
-    if (c->num_iterations >= c->prm.newton_max_iterations) return c->error * 4.0 < c->last_error;
-    return true;
-}
-
+// NewtonMethod::apply (newtonmethod.hh:308-517) with the loop's decisions on the device
+// (k_newton_begin / k_newton_linearized / k_newton_updated): per iteration the host queues
+// the loop head, the linearization, its max-norm and proceed_, gated by the device state,
+// and reads the state back. In the first iteration of a solve, which nearly always goes
+// on to a linear solve, the start of that solve (factorization, initial state, first
+// BiCGStab iteration) is queued behind it, gated by the device's decision, so that one
+// wait covers both the Newton and the Krylov state. Later iterations converge more often
+// than not; there the solve is queued only once the state says so (a converged
+// iteration then costs no launches of a solve that does not happen).
 int newton(ewg_ctx *c, double dt) {
-    c->num_iterations = 0;
-    const size_t vbytes = sizeof(double) * 2 * c->g.N;
-    int rc;
-    // The finiteness check of an update is read back with the next host wait (the next
-    // residual norm, or the end of the loop) instead of a wait of its own.
-    bool update_pending = false;
-    auto update_failed = [&]() {
-        update_pending = false;
-        if (std::isfinite(c->scal_host[1])) return false;
-        c->num_iterations = 2 * c->prm.newton_target_iterations;
-        return true;
+    hipStream_t s = c->stream;
+    const int n2 = 2 * c->g.N;
+    NewtonState &h = *c->ns_host;
+    h = NewtonState{};
+    h.error = 1e100;  // newtonmethod.hh:197
+    h.last_error = 1e100;
+    h.dt = dt;
+    h.tol = c->prm.newton_tolerance;
+    h.max_error = c->prm.newton_max_error;
+    h.target = c->prm.newton_target_iterations;
+    h.max_it = c->prm.newton_max_iterations;
+    h.status = NS_RUNNING;
+    HIP_TRY(c, hipMemcpyAsync(c->ns, c->ns_host, sizeof(NewtonState), hipMemcpyHostToDevice, s));
+    const int *active = &c->ns->active, *solve = &c->ns->solve;
+    auto mirror = [&]() {
+        c->error = h.error;
+        c->last_error = h.last_error;
+        c->first_error = h.first_error;
+        c->num_iterations = h.it;
     };
-    while (proceed(c)) {
-        c->last_error = c->error;  // beginIteration_
-        HIP_TRY(c, hipMemcpyAsync(c->xcur, c->x0, vbytes, hipMemcpyDeviceToDevice, c->stream));
-        double t0 = now_s();
-        if ((rc = linearize_device(c, dt, c->num_iterations)) != EWG_OK) return rc;
-        c->stats.newton_iterations++;
-        c->last_error = c->error;  // preSolve_
-        double e;
-        if ((rc = device_max_abs(c, c->R, &e)) != EWG_OK) return rc;
-        if (update_pending && update_failed())
-            return set_err(c, EWG_NUMERICAL_ISSUE, "Non-finite update!");
-        double t1 = now_s();
-        c->stats.time_linearize_s += t1 - t0;
-        c->error = e;
-        if (c->num_iterations == 0) c->first_error = e;
-        if (c->error > c->prm.newton_max_error) {
-            c->num_iterations = 2 * c->prm.newton_target_iterations;
-            return set_err(c, EWG_NUMERICAL_ISSUE, "Newton: error above NewtonMaxError");
-        }
-        if (!proceed(c)) {
-            ++c->num_iterations;
+    for (int round = 0;; ++round) {
+        const bool merged = round == 0;
+        const double t0 = now_s();
+        launch_newton_begin(c->ns, s);
+        launch_copy_gated(n2, c->x0, c->xcur, active, s);
+        LinArgs a = lin_args(c, dt, 0);
+        a.ns = c->ns;
+        sample_begin(c, EWG_KERNEL_LINEARIZE);
+        launch_linearize(a, s);
+        sample_end(c, EWG_KERNEL_LINEARIZE);
+        launch_max_abs(n2, c->R, c->part0, s);
+        launch_newton_linearized(c->ns, c->part0, s);
+        if (merged) bicgstab_begin(c, c->R, c->kx, solve);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(c->ns_host, c->ns, sizeof(NewtonState), hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        if (!h.active) {
+            // the loop ended at its head: nothing of this round ran
+            sample_drop(c, EWG_KERNEL_LINEARIZE);
+            sample_drop(c, EWG_KERNEL_ILU_FACTOR);
             break;
         }
+        c->have_system = true;
+        c->stats.newton_iterations++;
+        const double t1 = now_s();
+        c->stats.time_linearize_s += t1 - t0;
+        if (!h.solve) {  // converged, or failed, at this linearization
+            sample_drop(c, EWG_KERNEL_ILU_FACTOR);
+            sample_collect(c);
+            break;
+        }
+        if (!merged) {
+            bicgstab_begin(c, c->R, c->kx, nullptr);
+            HIP_TRY(c, hipGetLastError());
+            HIP_TRY(c, hipStreamSynchronize(s));
+        }
         int its = 0;
-        rc = bicgstab(c, c->R, c->kx, &its);
-        double t2 = now_s();
+        const int rc = bicgstab_end(c, c->R, c->kx, &its);
+        const double t2 = now_s();
         c->stats.time_solve_s += t2 - t1;
         c->last_linear_iterations = its;
         c->stats.linear_iterations += its;
         c->stats.linear_solves++;
         if (rc != EWG_OK) {
+            mirror();
             c->num_iterations = 2 * c->prm.newton_target_iterations;
             return rc;
         }
-        launch_newton_update(2 * c->g.N, c->xcur, c->kx, c->x0, c->part0, c->stream);
-        launch_reduce_sum(c->part0, c->dscalar, c->stream);
-        HIP_TRY(c, hipMemcpyAsync(c->scal_host + 1, c->dscalar, sizeof(double),
-                                  hipMemcpyDeviceToHost, c->stream));
-        update_pending = true;
+        launch_newton_update(n2, c->xcur, c->kx, c->x0, c->part0, s);
+        launch_newton_updated(c->ns, c->part0, s);
         c->stats.time_update_s += now_s() - t2;
-        ++c->num_iterations;
     }
-    if (update_pending) {
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        if (update_failed()) return set_err(c, EWG_NUMERICAL_ISSUE, "Non-finite update!");
-    }
+    mirror();
+    if (h.status == NS_FAILED)
+        return set_err(c, EWG_NUMERICAL_ISSUE, h.code == NS_FAIL_MAX_ERROR
+                                                   ? "Newton: error above NewtonMaxError"
+                                                   : "Non-finite update!");
     if (!(c->error <= c->prm.newton_tolerance)) {
         c->num_iterations = 2 * c->prm.newton_target_iterations;
         return set_err(c, EWG_NOT_CONVERGED, "Newton did not converge");
@@ -674,9 +702,9 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->part1, kRedBlocks);
     ALLOC(c->part2, kRedBlocks);
     ALLOC(c->dscalar, 8);
-    ALLOC(c->anyNonzero, 1);
     ALLOC(c->ilu_err, 1);
     ALLOC(c->ks, 1);
+    ALLOC(c->ns, 1);
     ALLOC(c->partAny, kRedBlocks);
     // the tile-wavefront buffers (3D; every schedule but the hyperplane launches)
     if (g.dim == 3 && N <= kWfMaxCells && params->ilu_schedule != 1) {
@@ -730,6 +758,8 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
 #undef ALLOC
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->ks_host), sizeof(KrylovState))) != hipSuccess)
         return fail("ks_host", e);
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->ns_host), sizeof(NewtonState))) != hipSuccess)
+        return fail("ns_host", e);
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->scal_host), 8 * sizeof(double))) != hipSuccess)
         return fail("scal_host", e);
     if ((e = hipHostMalloc(reinterpret_cast<void **>(&c->err_host), 4 * sizeof(int))) != hipSuccess)
@@ -1044,7 +1074,7 @@ int ewg_apply_preconditioner_noalloc(ewg_ctx *c, const double *d, double *v) {
         w.ex = reinterpret_cast<unsigned long long *>(base + kScratchTickets);
         w.ey = w.ex + c->wf.edge_words;
         w.err = c->ilu_err;
-        hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, c->ilu_err);
+        launch_flag_clear(c->ilu_err, s);
         if (fp32) {
             WlPrecond<float> wl = c->wl32;
             wl.w = reinterpret_cast<float *>(base + kScratchTickets + kScratchEdges);

@@ -51,7 +51,7 @@ struct ewg_ctx {
            *kx = nullptr;
     double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr;
     double *dscalar = nullptr;
-    int *anyNonzero = nullptr, *ilu_err = nullptr;
+    int *ilu_err = nullptr;
     int *partAny = nullptr;
     // tile-wavefront synchronisation (3D grids)
     unsigned *wf_ticket = nullptr;
@@ -71,8 +71,10 @@ struct ewg_ctx {
     } samples[4];
     double *copy_src = nullptr, *copy_dst = nullptr;  // the bandwidth-reference copy's buffers
     KrylovState *ks = nullptr;
+    ewg::NewtonState *ns = nullptr;   // the device Newton loop's state
     // pinned host mirrors
     KrylovState *ks_host = nullptr;
+    ewg::NewtonState *ns_host = nullptr;
     double *scal_host = nullptr;
     int *err_host = nullptr;
     // host copies of the BCRS pattern (rows ascending columns) and slot mapping

@@ -36,6 +36,8 @@ struct Grid {
     double area[3];
 };
 
+struct NewtonState;
+
 struct LinArgs {
     Grid g;
     double grav[3];
@@ -62,7 +64,29 @@ struct LinArgs {
     double *iqv;          // [N][4] intensive quantities of each cell (values)
     double *iqd;          // [N][6] their derivatives w.r.t. the cell's primary variables
     int flux_fp32;        // 1: face fluxes and their derivatives in FP32
+    const NewtonState *ns;  // device Newton state (nullptr: dt and newton_iter above): the
+                            // kernels return unless ns->active, and read dt, newton_iter
+                            // from it
 };
+
+// The state of one Newton solve (NewtonMethod::apply, newtonmethod.hh:308-517) on the
+// device: the kernels of each iteration evaluate the proceed_ rules (:866-884) and the
+// error checks, and gate the iteration's launches, so that the host reads the state once
+// per iteration together with the linear solver's.
+struct NewtonState {
+    double error, last_error, first_error;
+    double upd;                // sum |dx| of the last update (its finiteness is checked)
+    double dt, tol, max_error;
+    int it;                    // numIterations
+    int target, max_it;
+    int status;                // NS_RUNNING, NS_DONE, NS_FAILED
+    int active;                // this iteration linearizes
+    int solve;                 // this iteration solves the linear system
+    int has_update;            // an update awaits its finiteness check
+    int code;                  // NS_FAILED: NS_FAIL_MAX_ERROR or NS_FAIL_UPDATE
+};
+enum { NS_RUNNING = 0, NS_DONE = 1, NS_FAILED = 2 };
+enum { NS_FAIL_MAX_ERROR = 1, NS_FAIL_UPDATE = 2 };
 
 // BiCGStab scalars, kept on the device so that one host read per iteration suffices
 struct KrylovState {
@@ -75,7 +99,7 @@ struct KrylovState {
     int iterations;
     int pad;
 };
-enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_FAILED = 2, KS_BREAKDOWN = 3 };
+enum { KS_RUNNING = 0, KS_CONVERGED = 1, KS_FAILED = 2, KS_BREAKDOWN = 3, KS_IDLE = 4 };
 
 constexpr int kRedBlocks = 1024;   // fixed grid of the reducing vector kernels
 constexpr int kRedThreads = 256;
@@ -85,12 +109,13 @@ void launch_linearize(const LinArgs &a, hipStream_t s);
 void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s);
 void launch_bc_aux(const LinArgs &a, double *aux, hipStream_t s);  // per-face boundary values
 
-// ILU(0)
+// ILU(0). gate (optional): a device flag; the launches return at once while it is 0
 void launch_ilu_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
-                         hipStream_t s);
+                         hipStream_t s, const int *gate = nullptr);
 // FP32 preconditioner: the FP64 factors and the Jacobian's upper blocks rounded to FP32
 void launch_round_factors(const Grid &g, const double *L, const double *Dinv, const double *A,
-                          float *L32, float *D32, float *U32, hipStream_t s);
+                          float *L32, float *D32, float *U32, hipStream_t s,
+                          const int *gate = nullptr);
 void launch_ilu_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                         const double *d, double *v, const int *status, hipStream_t s);
 void launch_ilu_apply32(const Grid &g, const float *L, const float *Dinv, const float *U,
@@ -104,10 +129,14 @@ void launch_spmv(const Grid &g, const double *A, const double *x, double *y,
                  const int *status, hipStream_t s);
 
 // Krylov helpers (n = 2N doubles)
+// gate (optional): while *gate == 0 no solve starts (ks_init leaves the state KS_IDLE,
+// which every iteration kernel skips)
 void launch_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v,
-                      double *partMax, double *partDot, hipStream_t s);
+                      double *partMax, double *partDot, hipStream_t s,
+                      const int *gate = nullptr);
 void launch_ks_init(KrylovState *ks, const double *partMax, const double *partDot,
-                    double rel_tol, double abs_tol, double max_resid, hipStream_t s);
+                    double rel_tol, double abs_tol, double max_resid, hipStream_t s,
+                    const int *gate = nullptr);
 void launch_ks_beta(KrylovState *ks, const double *partRho, hipStream_t s);
 // tickets (optional): two tile tickets the kernel clears for the preconditioner apply
 // that follows it
@@ -130,6 +159,14 @@ void launch_reduce_max(const double *part, double *out, hipStream_t s);
 void launch_newton_update(int n, const double *cur, const double *dx, double *x, double *part,
                           hipStream_t s);
 void launch_reduce_sum(const double *part, double *out, hipStream_t s);
+// the device Newton loop: begin of an iteration (proceed_ at the loop head), after the
+// linearization's max-norm partials, after an update's sum |dx| partials
+void launch_newton_begin(NewtonState *ns, hipStream_t s);
+void launch_newton_linearized(NewtonState *ns, const double *partMax, hipStream_t s);
+void launch_newton_updated(NewtonState *ns, const double *partSum, hipStream_t s);
+// dst = src (n doubles) while *gate
+void launch_copy_gated(int n, const double *src, double *dst, const int *gate, hipStream_t s);
+void launch_flag_clear(int *flag, hipStream_t s);
 
 // dst = src (n doubles, n even)
 void launch_copy(size_t n, const double *src, double *dst, hipStream_t s);

@@ -346,8 +346,11 @@ __device__ __forceinline__ IQ<double> iq_values(const LinArgs &a, int cell) {
     return q;
 }
 
+// dt and newton_iter are the time-step size and Newton iteration of this linearization
+// (a.dt and a.newton_iter unless a device Newton state supplies them)
 template <int DIM, class S>
-__device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
+__device__ __forceinline__ void linearize_cell(const LinArgs &a, int i, double dt,
+                                               int newton_iter) {
     const Grid &g = a.g;
     int ijk[3];
     cell_coords(g, i, ijk);
@@ -392,7 +395,7 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
         double Mold[2];
         double2 *sto = reinterpret_cast<double2 *>(a.sto);
         if (a.enable_cache) {
-            if (a.newton_iter == 0) {
+            if (newton_iter == 0) {
                 if (a.recycle) {
                     Mold[0] = M[0].v;
                     Mold[1] = M[1].v;
@@ -419,7 +422,7 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
         double scv = g.V * 1.0;
         for (int k = 0; k < 2; ++k) {
             Ev tmp = M[k] - Mold[k];
-            tmp = tmp * (scv / a.dt);
+            tmp = tmp * (scv / dt);
             res0[k] = res0[k] + tmp;
         }
     }
@@ -446,6 +449,11 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
 }
 
 
+
+template <int DIM, class S>
+__device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
+    linearize_cell<DIM, S>(a, i, a.dt, a.newton_iter);
+}
 
 }  // namespace lin
 }  // namespace ewg

@@ -19,7 +19,9 @@ using namespace cell;
 
 template <class T>
 __global__ void __launch_bounds__(256)
-k_ilu_factor_level(Grid g, int level, const double *A, T *L, T *Dinv, T *U, int *err) {
+k_ilu_factor_level(Grid g, int level, const double *A, T *L, T *Dinv, T *U, int *err,
+                   const int *gate) {
+    if (gate && *gate == 0) return;
     int ijk[3];
     int i = level_cell(g, level, blockIdx.x * blockDim.x + threadIdx.x, ijk);
     if (i < 0) return;
@@ -135,7 +137,8 @@ constexpr double kBreakdownEps = 2.2250738585072014e-308 * 1e10;  // DBL_MIN * 1
 
 __global__ void __launch_bounds__(kRedThreads)
 k_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v, double *partMax,
-            double *partDot) {
+            double *partDot, const int *gate) {
+    if (gate && *gate == 0) return;
     double m = 0.0, d = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double bi = b[i];
@@ -156,7 +159,14 @@ k_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v, 
 
 __global__ void __launch_bounds__(kRedThreads)
 k_ks_init(KrylovState *ks, const double *partMax, const double *partDot, double rel_tol,
-          double abs_tol, double max_resid) {
+          double abs_tol, double max_resid, const int *gate) {
+    if (gate && *gate == 0) {
+        if (threadIdx.x == 0) {
+            ks->iterations = 0;
+            ks->status = KS_IDLE;
+        }
+        return;
+    }
     double m = reduce_partials_max(partMax);
     __syncthreads();
     double d = reduce_partials_sum(partDot);
@@ -387,19 +397,107 @@ k_newton_update(int n, const double *cur, const double *dx, double *x, double *p
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// ----------------------------------------------------------------------------------
+// The Newton loop's decisions on the device (newtonmethod.hh:308-517): one thread of one
+// block evaluates the rules with the host driver's arithmetic; the reductions are the
+// ones of k_reduce_max / k_reduce_sum.
+// ----------------------------------------------------------------------------------
+__device__ __forceinline__ bool ns_proceed(const NewtonState &n) {  // newtonmethod.hh:866-884
+    if (n.it < 1) return true;
+    if (n.error <= n.tol) return false;
+    if (n.it >= n.max_it) return n.error * 4.0 < n.last_error;
+    return true;
+}
+
+// head of an iteration: the loop condition, then beginIteration_ (lastError_ = error_)
+__global__ void k_newton_begin(NewtonState *ns) {
+    NewtonState &n = *ns;
+    n.solve = 0;
+    if (n.status != NS_RUNNING) {
+        n.active = 0;
+        return;
+    }
+    if (!ns_proceed(n)) {
+        n.active = 0;
+        if (n.has_update && !isfinite(n.upd)) {  // the last update, checked at the loop's end
+            n.status = NS_FAILED;
+            n.code = NS_FAIL_UPDATE;
+            n.it = 2 * n.target;
+        } else {
+            n.status = NS_DONE;
+        }
+        return;
+    }
+    n.last_error = n.error;
+    n.active = 1;
+}
+
+// after the linearization: preSolve_ (lastError_ = error_, error_ = max |R|), the check of
+// the previous update, NewtonMaxError, and proceed_ before the linear solve
+__global__ void __launch_bounds__(kRedThreads)
+k_newton_linearized(NewtonState *ns, const double *partMax) {
+    if (!ns->active) return;
+    const double e = reduce_partials_max(partMax);
+    if (threadIdx.x != 0) return;
+    NewtonState &n = *ns;
+    n.last_error = n.error;
+    if (n.has_update && !isfinite(n.upd)) {
+        n.status = NS_FAILED;
+        n.code = NS_FAIL_UPDATE;
+        n.it = 2 * n.target;
+        return;
+    }
+    n.has_update = 0;
+    n.error = e;
+    if (n.it == 0) n.first_error = e;
+    if (n.error > n.max_error) {
+        n.status = NS_FAILED;
+        n.code = NS_FAIL_MAX_ERROR;
+        n.it = 2 * n.target;
+        return;
+    }
+    if (!ns_proceed(n)) {
+        ++n.it;
+        n.status = NS_DONE;
+        return;
+    }
+    n.solve = 1;
+}
+
+// after an update x = xcur - dx: sum |dx| for the finiteness check, ++numIterations
+__global__ void __launch_bounds__(kRedThreads)
+k_newton_updated(NewtonState *ns, const double *partSum) {
+    const double s = reduce_partials_sum(partSum);
+    if (threadIdx.x != 0) return;
+    ns->upd = s;
+    ns->has_update = 1;
+    ++ns->it;
+}
+
+__global__ void __launch_bounds__(256)
+k_copy_gated(int n, const double *src, double *dst, const int *gate) {
+    if (!*gate) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_flag_clear(int *flag) { *flag = 0; }
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------
 void launch_ilu_factor64(const Grid &g, const double *A, double *L, double *Dinv, int *err,
-                         hipStream_t s) {
+                         hipStream_t s, const int *gate) {
     int threads = level_threads(g), blocks = (threads + 255) / 256;
     for (int l = 0; l < g.nlevels; ++l)
         hipLaunchKernelGGL(k_ilu_factor_level<double>, dim3(blocks), dim3(256), 0, s, g, l, A, L,
-                           Dinv, static_cast<double *>(nullptr), err);
+                           Dinv, static_cast<double *>(nullptr), err, gate);
 }
 // FP32 copies of FP64 factors and of the Jacobian's upper blocks (rounded to nearest)
 __global__ void k_round_factors(size_t nl, size_t nd, const double *L, const double *D,
-                                const double *Aup, float *L32, float *D32, float *U32) {
+                                const double *Aup, float *L32, float *D32, float *U32,
+                                const int *gate) {
+    if (gate && *gate == 0) return;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < nl) {
         L32[e] = static_cast<float>(L[e]);
@@ -408,10 +506,10 @@ __global__ void k_round_factors(size_t nl, size_t nd, const double *L, const dou
     if (e < nd) D32[e] = static_cast<float>(D[e]);
 }
 void launch_round_factors(const Grid &g, const double *L, const double *Dinv, const double *A,
-                          float *L32, float *D32, float *U32, hipStream_t s) {
+                          float *L32, float *D32, float *U32, hipStream_t s, const int *gate) {
     const size_t nl = (size_t)g.dim * g.N * 4, nd = (size_t)g.N * 4;
     hipLaunchKernelGGL(k_round_factors, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, nl,
-                       nd, L, Dinv, A + (size_t)(g.diag + 1) * g.N * 4, L32, D32, U32);
+                       nd, L, Dinv, A + (size_t)(g.diag + 1) * g.N * 4, L32, D32, U32, gate);
 }
 void launch_ilu_apply64(const Grid &g, const double *L, const double *Dinv, const double *A,
                         const double *d, double *v, const int *status, hipStream_t s) {
@@ -443,14 +541,15 @@ void launch_spmv(const Grid &g, const double *A, const double *x, double *y, con
 }
 
 void launch_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v,
-                      double *partMax, double *partDot, hipStream_t s) {
+                      double *partMax, double *partDot, hipStream_t s, const int *gate) {
     hipLaunchKernelGGL(k_bicg_init, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, b, r, x, p, v,
-                       partMax, partDot);
+                       partMax, partDot, gate);
 }
 void launch_ks_init(KrylovState *ks, const double *partMax, const double *partDot,
-                    double rel_tol, double abs_tol, double max_resid, hipStream_t s) {
+                    double rel_tol, double abs_tol, double max_resid, hipStream_t s,
+                    const int *gate) {
     hipLaunchKernelGGL(k_ks_init, dim3(1), dim3(kRedThreads), 0, s, ks, partMax, partDot, rel_tol,
-                       abs_tol, max_resid);
+                       abs_tol, max_resid, gate);
 }
 void launch_ks_beta(KrylovState *ks, const double *partRho, hipStream_t s) {
     hipLaunchKernelGGL(k_ks_beta, dim3(1), dim3(kRedThreads), 0, s, ks, partRho);
@@ -503,6 +602,22 @@ void launch_newton_update(int n, const double *cur, const double *dx, double *x,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_newton_update, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, cur, dx, x,
                        part);
+}
+
+void launch_newton_begin(NewtonState *ns, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_begin, dim3(1), dim3(1), 0, s, ns);
+}
+void launch_newton_linearized(NewtonState *ns, const double *partMax, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_linearized, dim3(1), dim3(kRedThreads), 0, s, ns, partMax);
+}
+void launch_newton_updated(NewtonState *ns, const double *partSum, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_updated, dim3(1), dim3(kRedThreads), 0, s, ns, partSum);
+}
+void launch_copy_gated(int n, const double *src, double *dst, const int *gate, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_gated, dim3(2048), dim3(256), 0, s, n, src, dst, gate);
+}
+void launch_flag_clear(int *flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, flag);
 }
 
 }  // namespace ewg

@@ -10,9 +10,11 @@ namespace {
 
 using namespace lin;
 
+// With a device Newton state (a.ns) the kernels do nothing unless this iteration
+// linearizes, and take dt and the Newton iteration (storage-cache semantics) from it.
 __global__ void __launch_bounds__(256) k_iq(LinArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.g.N) return;
+    if (i >= a.g.N || (a.ns && !a.ns->active)) return;
     iq_cell(a, i);
 }
 
@@ -20,7 +22,14 @@ template <int DIM, class S>
 __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.g.N) return;
-    linearize_cell<DIM, S>(a, i);
+    double dt = a.dt;
+    int newton_iter = a.newton_iter;
+    if (a.ns) {
+        if (!a.ns->active) return;
+        dt = a.ns->dt;
+        newton_iter = a.ns->it;
+    }
+    linearize_cell<DIM, S>(a, i, dt, newton_iter);
 }
 
 __global__ void __launch_bounds__(256) k_bc_aux(LinArgs a, double *aux) {

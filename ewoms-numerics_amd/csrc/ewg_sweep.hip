@@ -968,8 +968,9 @@ static FwsArgs<T> fws_args(const Grid &g, const WfSync &sync, const double *A, d
 
 template <class T>
 static void wl_factor(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                      const WlPrecond<T> &wl, int *err, hipStream_t s) {
-    const FwsArgs<T> f = fws_args<T>(g, sync, A, Af, wl.L, wl.U, wl.D, err);
+                      const WlPrecond<T> &wl, int *err, hipStream_t s, const int *gate) {
+    FwsArgs<T> f = fws_args<T>(g, sync, A, Af, wl.L, wl.U, wl.D, err);
+    f.active = gate;
     hipLaunchKernelGGL(k_wl_pack_factor<T>, dim3((unsigned)(sync.ntiles * ((g.nz + 3) / 4))), dim3(256), 0,
                        s, f);
     hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
@@ -977,13 +978,13 @@ static void wl_factor(const Grid &g, const WfSync &sync, const double *A, double
 }
 
 void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<double> &wl, int *err, hipStream_t s) {
-    wl_factor<double>(g, sync, A, Af, wl, err, s);
+                 const WlPrecond<double> &wl, int *err, hipStream_t s, const int *gate) {
+    wl_factor<double>(g, sync, A, Af, wl, err, s, gate);
 }
 
 void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<float> &wl, int *err, hipStream_t s) {
-    wl_factor<float>(g, sync, A, Af, wl, err, s);
+                 const WlPrecond<float> &wl, int *err, hipStream_t s, const int *gate) {
+    wl_factor<float>(g, sync, A, Af, wl, err, s, gate);
 }
 
 struct WlBatch {

@@ -53,10 +53,11 @@ size_t wl_entries(const Grid &g, const WfSync &sync);
 // the factorization straight into the layout (FP64 arithmetic; FP32 rounds the stored
 // factors and upper blocks); entries without a cell keep what they hold (zeros). Af:
 // scratch for the operands in the forward layout, 7 x wl_entries x 4 doubles
+// gate (optional): a device flag; both launches return at once while it is 0
 void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<double> &wl, int *err, hipStream_t s);
+                 const WlPrecond<double> &wl, int *err, hipStream_t s, const int *gate = nullptr);
 void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<float> &wl, int *err, hipStream_t s);
+                 const WlPrecond<float> &wl, int *err, hipStream_t s, const int *gate = nullptr);
 // v = (LU)^-1 d with one compute and one edge wavefront per tile. The forward sweep takes
 // tickets from sync.ticket[0], the backward sweep from sync.ticket[1]; tickets_clear:
 // a kernel queued on the stream after the last sweep that took tickets has set both to

@@ -227,3 +227,49 @@ def test_mixed_precision_steps_close_to_fp64_oracle(form):
         assert rcg == 0 and rco == 0
         assert abs(ig - io) <= 1
         assert dS < 1e-6 and dp < 1e-6
+
+
+def test_run_lens_with_step_halving():
+    """The time loop with failing Newton solves (NewtonMaxIterations = 4 with the
+    reference's continuation rule, dt0 = 2000 s): dt is halved and the step redone from
+    solution(1) (fvbaseproblem.hh:507-553, fvbasediscretization.hh:1370-1387) exactly as
+    in the oracle: the same accepted steps, halvings and linearizations."""
+    import ewg
+    from ewg import problems
+    P = problems.lens_problem()
+    prm = ewg.default_params(newton_tolerance=1e-8, linear_tolerance=1e-10,
+                             newton_max_iterations=4)
+    gpu, ora = _pair(P, prm)
+    ng = gpu.run(0.0, 4000.0, 2000.0)
+    no = ora.run(0.0, 4000.0, 2000.0)
+    sg, so = gpu.stats(), ora.stats()
+    print("gpu", sg, "\noracle", so, flush=True)
+    assert so["failed_steps"] >= 5
+    assert ng == no
+    assert sg["failed_steps"] == so["failed_steps"]
+    assert sg["time_steps"] == so["time_steps"]
+    assert sg["newton_iterations"] == so["newton_iterations"]
+    xg, xo = gpu.solution(0), ora.solution(0)
+    assert np.abs(xg[1::2] - xo[1::2]).max() < 1e-8
+
+
+@pytest.mark.parametrize("case", ["max_error", "linear_solver"])
+def test_newton_failures_match_oracle(case):
+    """The Newton loop's failure exits: NewtonMaxError exceeded at the first
+    linearization (decided on the device: numerical issue), and a linear solve that does
+    not converge within LinearSolverMaxIterations = 1 (not converged); both report
+    numIterations = 2 * NewtonTargetIterations and the error the oracle reports."""
+    import ewg
+    from ewg import problems
+    P = problems.lens_problem()
+    kw = ({"newton_max_error": 1e-9} if case == "max_error"
+          else {"linear_max_iterations": 1})
+    prm = ewg.default_params(linear_tolerance=1e-10, **kw)
+    gpu, ora = _pair(P, prm)
+    rcg, itg, eg = gpu.newton(4 * P.dt)
+    rco, ito, eo = ora.newton(4 * P.dt)
+    print(case, "gpu", rcg, itg, eg, "oracle", rco, ito, eo, flush=True)
+    assert rcg == rco and itg == ito == 2 * prm.newton_target_iterations
+    assert rcg == (ewg.capi.EWG_NUMERICAL_ISSUE if case == "max_error"
+                   else ewg.capi.EWG_NOT_CONVERGED)
+    assert abs(eg - eo) <= 1e-10 * abs(eo)
