@@ -93,7 +93,7 @@ void free_all(ewg_ctx *c) {
                     c->pe, c->region,
                     c->regs, c->bc_type, c->bc_val, c->bc_aux, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
-                    c->part2, c->dscalar, c->ilu_err, c->ks, c->ns, c->partAny,
+                    c->part2, c->dscalar, c->ilu_err, c->ks, c->ns, c->pref, c->partAny,
                     c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wl64.L, c->wl64.U,
                     c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w,
                     c->wl_af};
@@ -143,6 +143,9 @@ LinArgs lin_args(ewg_ctx *c, double dt, int newton_iter) {
     a.iqv = c->iqv;
     a.iqd = c->iqd;
     a.flux_fp32 = c->prm.linearize_fp32;
+    a.fd = c->prm.local_linearizer;
+    a.fd_method = c->prm.numeric_difference_method;
+    a.pref = c->pref;
     return a;
 }
 
@@ -197,6 +200,25 @@ bool use_wavefront_layout(const ewg_ctx *c) {
 // the upper blocks rounded to FP32 (one rounding of the exact-arithmetic-order FP64
 // factors, the same for both schedules).
 // gate (optional): device flag; the launches return at once while it is 0
+// the parameter values the library accepts beyond the reference's own ranges
+const char *check_params(const ewg_params *p) {
+    if (p->ilu_schedule < 0 || p->ilu_schedule > 1) return "ilu_schedule must be 0 or 1";
+    if (p->linearize_fp32 != 0 && p->linearize_fp32 != 1) return "linearize_fp32 must be 0 or 1";
+    if (p->local_linearizer != 0 && p->local_linearizer != 1)
+        return "local_linearizer must be 0 (automatic differentiation) or 1 (finite differences)";
+    if (p->numeric_difference_method < -1 || p->numeric_difference_method > 1)
+        return "numeric_difference_method must be -1, 0 or +1";
+    return nullptr;
+}
+
+// FvBaseDiscretization::updateBegin of the immiscible model (immisciblemodel.hh:278-293):
+// the reference pressure of the finite-difference weights is the first cell's pressure
+// at the start of a solve
+int capture_reference_pressure(ewg_ctx *c) {
+    HIP_TRY(c, hipMemcpyAsync(c->pref, c->x0, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return EWG_OK;
+}
+
 void precond_factor(ewg_ctx *c, const int *gate = nullptr) {
     const bool fp32 = c->prm.precond_fp32;
     if (use_wavefront_layout(c)) {
@@ -397,6 +419,7 @@ int newton(ewg_ctx *c, double dt) {
     h.max_it = c->prm.newton_max_iterations;
     h.status = NS_RUNNING;
     HIP_TRY(c, hipMemcpyAsync(c->ns, c->ns_host, sizeof(NewtonState), hipMemcpyHostToDevice, s));
+    if (int rc = capture_reference_pressure(c)) return rc;
     const int *active = &c->ns->active, *solve = &c->ns->solve;
     auto mirror = [&]() {
         c->error = h.error;
@@ -597,6 +620,7 @@ void ewg_default_params(ewg_params *p) {
     p->min_time_step_size = 0.0;
     p->max_time_step_size = std::numeric_limits<double>::infinity();
     p->precond_fp32 = 0;
+    p->numeric_difference_method = 1;
 }
 
 int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
@@ -614,8 +638,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     if (material->nregions < 1 || !material->region_params) return EWG_INVALID_ARGUMENT;
     if (material->law != EWG_LAW_VAN_GENUCHTEN && material->law != EWG_LAW_BROOKS_COREY)
         return EWG_INVALID_ARGUMENT;
-    if (params->ilu_schedule < 0 || params->ilu_schedule > 1) return EWG_INVALID_ARGUMENT;
-    if (params->linearize_fp32 != 0 && params->linearize_fp32 != 1) return EWG_INVALID_ARGUMENT;
+    if (check_params(params)) return EWG_INVALID_ARGUMENT;
     auto *c = new ewg_ctx();
     c->grid = *grid;
     c->fluid = *fluid;
@@ -705,6 +728,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->ilu_err, 1);
     ALLOC(c->ks, 1);
     ALLOC(c->ns, 1);
+    ALLOC(c->pref, 1);
     ALLOC(c->partAny, kRedBlocks);
     // the tile-wavefront buffers (3D; every schedule but the hyperplane launches)
     if (g.dim == 3 && N <= kWfMaxCells && params->ilu_schedule != 1) {
@@ -822,13 +846,10 @@ const char *ewg_last_error(const ewg_ctx *c) { return c ? c->err.c_str() : "null
 
 int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
     if (!c || !p) return EWG_INVALID_ARGUMENT;
-    if (p->ilu_schedule < 0 || p->ilu_schedule > 1)
-        return set_err(c, EWG_INVALID_ARGUMENT, "ilu_schedule must be 0 or 1");
+    if (const char *msg = check_params(p)) return set_err(c, EWG_INVALID_ARGUMENT, msg);
     if (p->ilu_schedule != 1 && c->g.dim == 3 && c->g.N <= kWfMaxCells && !c->wf.ticket)
         return set_err(c, EWG_INVALID_ARGUMENT,
                        "a tile-wavefront schedule needs a context created with one");
-    if (p->linearize_fp32 != 0 && p->linearize_fp32 != 1)
-        return set_err(c, EWG_INVALID_ARGUMENT, "linearize_fp32 must be 0 or 1");
     if (p->precond_fp32) {
         int rc = ensure_fp32(c);
         if (rc) return rc;
@@ -943,6 +964,7 @@ int ewg_get_pattern(const ewg_ctx *c, int32_t *rowptr, int32_t *colidx) {
 int ewg_linearize(ewg_ctx *c, double dt, int newton_iter, double *resid, double *jac) {
     if (!c || !(dt > 0.0) || newton_iter < 0) return EWG_INVALID_ARGUMENT;
     if (!c->static_set) return set_err(c, EWG_INVALID_STATE, "static fields not set");
+    if (int rc = capture_reference_pressure(c)) return rc;
     int rc = linearize_device(c, dt, newton_iter);
     if (rc) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1231,6 +1253,8 @@ int ewg_time_kernel(ewg_ctx *c, int which, int reps, double *mean_ms) {
     hipEvent_t e0, e1;
     HIP_TRY(c, hipEventCreate(&e0));
     HIP_TRY(c, hipEventCreate(&e1));
+    if (which == EWG_KERNEL_LINEARIZE)
+        if (int rc = capture_reference_pressure(c)) return rc;
     LinArgs a = lin_args(c, 1.0, 1);
     HIP_TRY(c, hipEventRecord(e0, s));
     for (int r = 0; r < reps; ++r) {

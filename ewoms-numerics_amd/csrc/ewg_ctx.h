@@ -72,6 +72,7 @@ struct ewg_ctx {
     double *copy_src = nullptr, *copy_dst = nullptr;  // the bandwidth-reference copy's buffers
     KrylovState *ks = nullptr;
     ewg::NewtonState *ns = nullptr;   // the device Newton loop's state
+    double *pref = nullptr;           // reference pressure of the finite-difference weights
     // pinned host mirrors
     KrylovState *ks_host = nullptr;
     ewg::NewtonState *ns_host = nullptr;

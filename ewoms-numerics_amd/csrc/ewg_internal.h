@@ -64,6 +64,9 @@ struct LinArgs {
     double *iqv;          // [N][4] intensive quantities of each cell (values)
     double *iqd;          // [N][6] their derivatives w.r.t. the cell's primary variables
     int flux_fp32;        // 1: face fluxes and their derivatives in FP32
+    int fd;               // 1: finite-difference local linearizer (FP64)
+    int fd_method;        // its NumericDifferenceMethod: +1 forward, 0 central, -1 backward
+    const double *pref;   // its reference pressure (the pressure's weight is 10 / pref)
     const NewtonState *ns;  // device Newton state (nullptr: dt and newton_iter above): the
                             // kernels return unless ns->active, and read dt, newton_iter
                             // from it

@@ -455,5 +455,146 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {
     linearize_cell<DIM, S>(a, i, a.dt, a.newton_iter);
 }
 
+// ----------------------------------------------------------------------------------
+// Finite-difference local linearizer (FvBaseFdLocalLinearizer,
+// fvbasefdlocallinearizer.hh:131-515): the element-local residual of cell i's context is
+// evaluated in values at the focus DOF's primary variables and with one of them deflected
+// by eps = BaseEpsilon / primaryVarWeight (:260-283; weights 10 / p_ref for the pressure
+// and 1 for the saturation, immisciblemodel.hh:298-310), and the differences give column
+// i of the Jacobian (:370-470). The neighbours' intensive quantities stay those of the
+// undeflected state (the context deflects only its focus DOF, :373-379), and the storage
+// term of the previous level is the one cached by the undeflected evaluation
+// (fvbaselocalresidual.hh:538-575: no refresh while the focus quantities are stashed).
+// The operations are those of linearize_cell, so the undeflected residual is the AD
+// path's value.
+// ----------------------------------------------------------------------------------
+constexpr double kFdBaseEpsilon = 0.9123e-10;  // max(0.9123e-10, 1.23e3 * DBL_EPSILON)
+
+// the element-local residual (values): res[0] of the focus cell, res[1 + dir] the face
+// flux's contribution to the neighbour across direction dir (zero without a neighbour)
+template <int DIM>
+__device__ __forceinline__ void element_residual(const LinArgs &a, int i, const int ijk[3],
+                                                 double p, double Sw, const double Mold[2],
+                                                 double dt, double res[2 * DIM + 1][2]) {
+    const Grid &g = a.g;
+    const double Ki = a.K[i];
+    const IQ<Ev> in = intensive<Ev>(a, i, ev(p), ev(Sw));
+    Ev res0[2] = {ev(0.0), ev(0.0)};
+    constexpr int nfaces = 2 * DIM;
+#pragma unroll
+    for (int dir = 0; dir < nfaces; ++dir) {
+        res[1 + dir][0] = res[1 + dir][1] = 0.0;
+        if (!has_neighbour(g, ijk, dir)) continue;
+        const int nb = i + ((dir & 1) ? 1 : -1) * stride_of(g, dir >> 1);
+        const IQ<double> ex = iq_values(a, nb);
+        Ev F[2];
+        interior_flux<DIM, double>(a, in, ex, Ki, a.K[nb], ijk, dir, F);
+        double alpha = 1.0;
+        alpha *= g.area[dir >> 1];
+        for (int k = 0; k < 2; ++k) {
+            F[k] = F[k] * alpha;
+            res0[k] = res0[k] + F[k];
+            res[1 + dir][k] = div_p2(ev(0.0) - F[k], g.V).v;
+        }
+    }
+    {
+        Ev M[2];
+        storage(a, a.phi[i], in.Sw, M);
+        const double scv = g.V * 1.0;
+        for (int k = 0; k < 2; ++k) {
+            Ev tmp = M[k] - Mold[k];
+            tmp = tmp * (scv / dt);
+            res0[k] = res0[k] + tmp;
+        }
+    }
+#pragma unroll
+    for (int dir = 0; dir < nfaces; ++dir) {
+        if (has_neighbour(g, ijk, dir)) continue;
+        const long long bf = boundary_face(a, ijk, dir);
+        const int t = a.bc_type[bf];
+        Ev Bv[2] = {ev(0.0), ev(0.0)};
+        if (t == 1) {
+            freeflow_flux<DIM, double>(a, in, i, Ki, ijk, dir, a.bc_val[2 * bf],
+                                       a.bc_val[2 * bf + 1], bf, Bv);
+        } else if (t == 2) {
+            Bv[0] = ev(a.bc_val[2 * bf]);
+            Bv[1] = ev(a.bc_val[2 * bf + 1]);
+        }
+        const double mult = g.area[dir >> 1] * 1.0;
+        for (int k = 0; k < 2; ++k) res0[k] = res0[k] + Bv[k] * mult;
+    }
+    for (int k = 0; k < 2; ++k) res[0][k] = div_p2(res0[k], g.V).v;
+}
+
+// method: +1 forward, 0 central, -1 backward differences (NumericDifferenceMethod);
+// pref: the reference pressure of the pressure's weight
+template <int DIM>
+__device__ __forceinline__ void linearize_cell_fd(const LinArgs &a, int i, double dt,
+                                                  int newton_iter, double pref, int method) {
+    constexpr int ND = 2 * DIM + 1;
+    const Grid &g = a.g;
+    int ijk[3];
+    cell_coords(g, i, ijk);
+    const double2 xi = reinterpret_cast<const double2 *>(a.x)[i];
+    // storage of the previous level with linearize_cell's cache semantics
+    double Mold[2];
+    {
+        const double phi = a.phi[i];
+        double2 *sto = reinterpret_cast<double2 *>(a.sto);
+        if (a.enable_cache && newton_iter != 0) {
+            const double2 c = sto[i];
+            Mold[0] = c.x;
+            Mold[1] = c.y;
+        } else {
+            const double Sold = (a.enable_cache && a.recycle) ? xi.y
+                                : reinterpret_cast<const double2 *>(a.x1)[i].y;
+            storage<double>(a, phi, Sold, Mold);
+            if (a.enable_cache) sto[i] = make_double2(Mold[0], Mold[1]);
+        }
+    }
+    double base[ND][2];
+    element_residual<DIM>(a, i, ijk, xi.x, xi.y, Mold, dt, base);
+    reinterpret_cast<double2 *>(a.R)[i] = make_double2(base[0][0], base[0][1]);
+    double d[2][ND][2];  // [pv][dof][eq]
+    for (int pv = 0; pv < 2; ++pv) {
+        const double x = pv == 0 ? xi.x : xi.y;
+        const double pvWeight = pv == 0 ? 10.0 / pref : 1.0;
+        const double eps = kFdBaseEpsilon / pvWeight;
+        double hi[ND][2], lo[ND][2];
+        double delta = 0.0;
+        double xp = x;
+        if (method >= 0) {
+            xp += eps;
+            delta += eps;
+            element_residual<DIM>(a, i, ijk, pv == 0 ? xp : xi.x, pv == 1 ? xp : xi.y, Mold, dt, hi);
+        } else {
+            for (int l = 0; l < ND; ++l) hi[l][0] = base[l][0], hi[l][1] = base[l][1];
+        }
+        if (method <= 0) {
+            xp -= delta + eps;
+            delta += eps;
+            element_residual<DIM>(a, i, ijk, pv == 0 ? xp : xi.x, pv == 1 ? xp : xi.y, Mold, dt, lo);
+        } else {
+            for (int l = 0; l < ND; ++l) lo[l][0] = base[l][0], lo[l][1] = base[l][1];
+        }
+        for (int l = 0; l < ND; ++l)
+            for (int k = 0; k < 2; ++k) {
+                double v = hi[l][k];
+                v -= lo[l][k];
+                d[pv][l][k] = v / delta;
+            }
+    }
+    // column i: the focus block and the neighbours' blocks towards the focus
+    Ev r[2];
+    for (int k = 0; k < 2; ++k) r[k] = Ev{0.0, d[0][0][k], d[1][0][k]};
+    store_block(a.J, g.diag, g.N, i, r);
+    for (int dir = 0; dir < 2 * DIM; ++dir) {
+        if (!has_neighbour(g, ijk, dir)) continue;
+        const int nb = i + ((dir & 1) ? 1 : -1) * stride_of(g, dir >> 1);
+        for (int k = 0; k < 2; ++k) r[k] = Ev{0.0, d[0][1 + dir][k], d[1][1 + dir][k]};
+        store_block(a.J, slot_of_dir(g, dir ^ 1), g.N, nb, r);
+    }
+}
+
 }  // namespace lin
 }  // namespace ewg

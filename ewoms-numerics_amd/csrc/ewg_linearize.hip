@@ -1,7 +1,8 @@
-// ewg_linearize.hip -- ECFV forward-AD linearization of the immiscible two-phase model over
-// the whole grid: one thread per cell, first the intensive quantities (k_iq), then the
-// element context of every cell as focus DOF (k_linearize); the per-cell code, with the
-// reference functions it follows, is in ewg_lin_device.h.
+// ewg_linearize.hip -- ECFV linearization of the immiscible two-phase model over the whole
+// grid: one thread per cell, first the intensive quantities (k_iq), then the element
+// context of every cell as focus DOF, by forward-mode AD (k_linearize) or finite
+// differences (k_linearize_fd); the per-cell code, with the reference functions it
+// follows, is in ewg_lin_device.h.
 #include "ewg_lin_device.h"
 
 namespace ewg {
@@ -32,6 +33,20 @@ __global__ void __launch_bounds__(256) k_linearize(LinArgs a) {
     linearize_cell<DIM, S>(a, i, dt, newton_iter);
 }
 
+template <int DIM>
+__global__ void __launch_bounds__(256) k_linearize_fd(LinArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.g.N) return;
+    double dt = a.dt;
+    int newton_iter = a.newton_iter;
+    if (a.ns) {
+        if (!a.ns->active) return;
+        dt = a.ns->dt;
+        newton_iter = a.ns->it;
+    }
+    linearize_cell_fd<DIM>(a, i, dt, newton_iter, *a.pref, a.fd_method);
+}
+
 __global__ void __launch_bounds__(256) k_bc_aux(LinArgs a, double *aux) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.g.N) return;
@@ -52,6 +67,13 @@ __global__ void __launch_bounds__(256) k_storage(LinArgs a, const double *x, dou
 void launch_linearize(const LinArgs &a, hipStream_t s) {
     int blocks = (a.g.N + 255) / 256;
     hipLaunchKernelGGL(k_iq, dim3(blocks), dim3(256), 0, s, a);
+    if (a.fd) {
+        if (a.g.dim == 3)
+            hipLaunchKernelGGL(k_linearize_fd<3>, dim3(blocks), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_linearize_fd<2>, dim3(blocks), dim3(256), 0, s, a);
+        return;
+    }
     if (a.flux_fp32) {
         if (a.g.dim == 3)
             hipLaunchKernelGGL((k_linearize<3, float>), dim3(blocks), dim3(256), 0, s, a);

@@ -64,7 +64,9 @@ class Params(C.Structure):
                 ("kernel_events", C.c_int32),
                 ("linearize_fp32", C.c_int32),
                 ("warm_start", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("local_linearizer", C.c_int32),
+                ("numeric_difference_method", C.c_int32),
+                ("reserved", C.c_int32 * 1)]
 
 
 class Stats(C.Structure):
@@ -106,6 +108,8 @@ def default_params(**overrides) -> Params:
     p.kernel_events = 0
     p.linearize_fp32 = 0
     p.warm_start = 0
+    p.local_linearizer = 0
+    p.numeric_difference_method = 1
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

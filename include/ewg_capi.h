@@ -116,7 +116,14 @@ typedef struct ewg_params {
     int32_t warm_start;                /* 1: start each Newton solve from the prediction of
                                           the network attached with ewg_set_warm_start
                                           (ewg_cnn.h); 0: from the previous time level */
-    int32_t reserved[3];
+    int32_t local_linearizer;          /* LocalLinearizerSplice: 0 automatic differentiation
+                                          (AutoDiffLocalLinearizer, default); 1 finite
+                                          differences (FiniteDifferenceLocalLinearizer,
+                                          fvbasefdlocallinearizer.hh:131-515) in FP64 */
+    int32_t numeric_difference_method; /* NumericDifferenceMethod of the finite differences:
+                                          +1 forward (default), 0 central, -1 backward
+                                          (fvbasefdlocallinearizer.hh:83) */
+    int32_t reserved[1];
 } ewg_params;
 
 /* Counters and timers of the last calls; the split follows

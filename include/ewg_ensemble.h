@@ -69,7 +69,9 @@ typedef struct ewg_ensemble_tuning {
     double cand_ms[EWG_ENSEMBLE_MAX_CANDIDATES];  /* one step; < 0: candidate failed */
 } ewg_ensemble_tuning;
 
-/* An ensemble over n contexts (not owned; they must outlive it). */
+/* An ensemble over n contexts (not owned; they must outlive it). Members linearize by
+ * automatic differentiation: a context with params.local_linearizer = 1 is rejected
+ * (EWG_INVALID_ARGUMENT). */
 int ewg_ensemble_create(ewg_ctx *const *members, int32_t n, ewg_ensemble **out);
 void ewg_ensemble_destroy(ewg_ensemble *ens);
 const char *ewg_ensemble_last_error(const ewg_ensemble *ens);

@@ -347,6 +347,7 @@ struct ewo_ctx {
     std::string err;
     bool static_set = false;
     int threads = 1;  // worker threads of the linearization (ThreadsPerProcess)
+    double pref = 1e5;  // reference pressure of the finite-difference weights
 };
 
 namespace {
@@ -387,10 +388,12 @@ inline int64_t boundary_face_index(const ewo_ctx &c, const int ijk[3], int dir) 
     return c.side_off[dir] + loc;
 }
 
-IQ intensive(const ewo_ctx &c, int64_t cell, const double *x) {
+// the intensive quantities of DOF `cell` at primary variables (p_w, S_w), both derivative
+// slots seeded
+IQ intensive_at(const ewo_ctx &c, int64_t cell, double pw, double sw) {
     IQ q;
-    Ev p0 = Ev::var(x[2 * cell], 0);
-    Ev Sw = Ev::var(x[2 * cell + 1], 1);
+    Ev p0 = Ev::var(pw, 0);
+    Ev Sw = Ev::var(sw, 1);
     q.S[0] = Sw;
     q.S[1] = 1.0 - Sw;
     const MatReg &r = c.regs[c.region[cell]];
@@ -402,6 +405,10 @@ IQ intensive(const ewo_ctx &c, int64_t cell, const double *x) {
     q.p[1] = p0 + pc;
     for (int k = 0; k < 2; ++k) q.lam[k] = kr[k] / Ev(c.fluid.mu[k]);
     return q;
+}
+
+IQ intensive(const ewo_ctx &c, int64_t cell, const double *x) {
+    return intensive_at(c, cell, x[2 * cell], x[2 * cell + 1]);
 }
 
 // storage term M_k = phi S_k rho_k (immisciblelocalresidual.hh:69-103)
@@ -540,14 +547,120 @@ int64_t find_block(const ewo_ctx &c, int64_t row, int64_t col) {
     return -1;
 }
 
+// The element context of `cell` (FvBaseLocalResidual::eval, fvbaselocalresidual.hh:140-187):
+// the local residual of every stencil DOF with the focus DOF's intensive quantities iq0
+// (the neighbours' from iqs, values only where the focus is not upstream) and the storage
+// term of the previous level Mold. Stencil: dofs[0] = the element, then one DOF per
+// interior intersection in intersection order; res[l] is DOF l's residual.
+void element_local(const ewo_ctx &c, int64_t cell, const int ijk[3], const IQ &iq0,
+                   const std::vector<IQ> &iqs, const double Mold[2], double dt, int &nd,
+                   int64_t dofs[7], Ev res[7][2]) {
+    const int nfaces = 2 * c.dim;
+    int dirs[6];
+    int ni = 0;
+    nd = 1;
+    dofs[0] = cell;
+    for (int dir = 0; dir < nfaces; ++dir) {
+        int64_t nb = neighbour(c, ijk, dir);
+        if (nb >= 0) {
+            dirs[ni++] = dir;
+            dofs[nd++] = nb;
+        }
+    }
+    for (int l = 0; l < nd; ++l) res[l][0] = res[l][1] = Ev(0.0);
+    // fluxes (fvbaselocalresidual.hh:295-367)
+    for (int f = 0; f < ni; ++f) {
+        Ev F[2];
+        interior_flux(c, iq0, iqs[dofs[f + 1]], cell, dofs[f + 1], ijk, dirs[f], F);
+        double alpha = 1.0;
+        alpha *= c.area[dirs[f] >> 1];
+        for (int k = 0; k < 2; ++k) {
+            F[k] = F[k] * alpha;
+            res[0][k] = res[0][k] + F[k];
+            res[f + 1][k] = res[f + 1][k] - F[k];
+        }
+    }
+    // storage (fvbaselocalresidual.hh:494-629)
+    {
+        Ev M[2];
+        storage_of(c, cell, iq0, M);
+        double scv = c.V * 1.0;
+        for (int k = 0; k < 2; ++k) {
+            Ev tmp = M[k] - Mold[k];
+            tmp = tmp * (scv / dt);
+            res[0][k] = res[0][k] + tmp;
+        }
+    }
+    // boundary (fvbaselocalresidual.hh:425-487)
+    for (int dir = 0; dir < nfaces; ++dir) {
+        if (neighbour(c, ijk, dir) >= 0) continue;
+        int64_t bf = boundary_face_index(c, ijk, dir);
+        int t = c.bc_type[bf];
+        Ev Bv[2];
+        if (t == EWG_BC_FREE_FLOW) {
+            freeflow_flux(c, iq0, cell, ijk, dir, c.bc_val[2 * bf], c.bc_val[2 * bf + 1], Bv);
+        } else if (t == EWG_BC_MASS_RATE) {
+            Bv[0] = Ev(c.bc_val[2 * bf]);
+            Bv[1] = Ev(c.bc_val[2 * bf + 1]);
+        } else {
+            Bv[0] = Ev(0.0);
+            Bv[1] = Ev(0.0);
+        }
+        double mult = c.area[dir >> 1] * 1.0;
+        for (int k = 0; k < 2; ++k) res[0][k] = res[0][k] + Bv[k] * mult;
+    }
+    // volumetric residual (fvbaselocalresidual.hh:170-186)
+    for (int l = 0; l < nd; ++l)
+        for (int k = 0; k < 2; ++k) res[l][k] = res[l][k] / c.V;
+}
+
+// the storage term of the previous time level for element `cell` with the cache semantics
+// of fvbaselocalresidual.hh:538-575 (iteration 0 refreshes the cache from the focus
+// quantities iq0, or from solution(1) without first-iteration recycling)
+void previous_storage(ewo_ctx &c, int64_t cell, const IQ &iq0, int newton_iter, double Mold[2]) {
+    auto from_level1 = [&]() {
+        IQ qo = intensive(c, cell, c.x1.data());
+        Ev Mo[2];
+        storage_of(c, cell, qo, Mo);
+        Mold[0] = Mo[0].v;
+        Mold[1] = Mo[1].v;
+    };
+    if (!c.prm.enable_storage_cache) {
+        from_level1();
+        return;
+    }
+    if (newton_iter != 0) {
+        Mold[0] = c.storage_old[2 * cell];
+        Mold[1] = c.storage_old[2 * cell + 1];
+        return;
+    }
+    if (c.prm.recycle_first_iteration_storage) {
+        Ev M[2];
+        storage_of(c, cell, iq0, M);
+        Mold[0] = M[0].v;
+        Mold[1] = M[1].v;
+    } else {
+        from_level1();
+    }
+    c.storage_old[2 * cell] = Mold[0];
+    c.storage_old[2 * cell + 1] = Mold[1];
+}
+
 // FvBaseLinearizer::linearize_ + linearizeElement_ (fvbaselinearizer.hh:401-512) with
-// FvBaseAdLocalLinearizer::linearize (fvbaseadlocallinearizer.hh:178-204) and
-// FvBaseLocalResidual::eval (fvbaselocalresidual.hh:140-187).
+// FvBaseAdLocalLinearizer::linearize (fvbaseadlocallinearizer.hh:178-204), or with
+// FvBaseFdLocalLinearizer::linearize (fvbasefdlocallinearizer.hh:223-245, 370-470) when
+// params.local_linearizer = 1: the focus DOF's primary variable pvIdx deflected by
+// eps = BaseEpsilon / primaryVarWeight (:260-283), the element-local residual
+// re-evaluated with the neighbours' quantities and the cached storage unchanged.
 void linearize_all(ewo_ctx &c, double dt, int newton_iter, double *R, double *J) {
     const double *x = c.x0.data();
     std::fill(R, R + 2 * c.N, 0.0);
     std::fill(J, J + 4 * static_cast<int64_t>(c.colidx.size()), 0.0);
-    const int nfaces = 2 * c.dim;
+    const bool fd = c.prm.local_linearizer == 1;
+    const int method = c.prm.numeric_difference_method;
+    // immisciblemodel.hh:298-310: weight 10 / p_ref for the pressure, 1 for the saturation
+    const double pvWeight[2] = {10.0 / c.pref, 1.0};
+    const double baseEpsilon = std::max(0.9123e-10, std::numeric_limits<double>::epsilon() * 1.23e3);
     // intensive-quantity cache (EnableIntensiveQuantityCache, fvbasediscretization.hh:650-681):
     // every DOF is evaluated once per linearization with both derivative slots seeded
     std::vector<IQ> iqs(static_cast<size_t>(c.N));
@@ -573,99 +686,56 @@ void linearize_all(ewo_ctx &c, double dt, int newton_iter, double *R, double *J)
         for (int64_t cell = c0; cell < c1; ++cell) {
             int ijk[3];
             cell_coords(c, cell, ijk);
-            // stencil: DOF 0 = the element, then one DOF per interior intersection
+            double Mold[2];
+            previous_storage(c, cell, iqs[cell], newton_iter, Mold);
             int64_t dofs[7];
-            int dirs[6];
-            int nd = 1, ni = 0;
-            dofs[0] = cell;
-            for (int dir = 0; dir < nfaces; ++dir) {
-                int64_t nb = neighbour(c, ijk, dir);
-                if (nb >= 0) {
-                    dirs[ni++] = dir;
-                    dofs[nd++] = nb;
-                }
-            }
-            IQ iq[7];
-            for (int l = 0; l < nd; ++l) iq[l] = iqs[dofs[l]];
+            int nd = 0;
             Ev res[7][2];
-            // fluxes (fvbaselocalresidual.hh:295-367)
-            for (int f = 0; f < ni; ++f) {
-                Ev F[2];
-                interior_flux(c, iq[0], iq[f + 1], cell, dofs[f + 1], ijk, dirs[f], F);
-                double alpha = 1.0;
-                alpha *= c.area[dirs[f] >> 1];
-                for (int k = 0; k < 2; ++k) {
-                    F[k] = F[k] * alpha;
-                    res[0][k] = res[0][k] + F[k];
-                    res[f + 1][k] = res[f + 1][k] - F[k];
-                }
-            }
-            // storage (fvbaselocalresidual.hh:494-629)
-            {
-                Ev M[2];
-                storage_of(c, cell, iq[0], M);
-                double Mold[2];
-                if (c.prm.enable_storage_cache) {
-                    if (newton_iter == 0) {
-                        if (c.prm.recycle_first_iteration_storage) {
-                            Mold[0] = M[0].v;
-                            Mold[1] = M[1].v;
-                        } else {
-                            IQ qo = intensive(c, cell, c.x1.data());
-                            Ev Mo[2];
-                            storage_of(c, cell, qo, Mo);
-                            Mold[0] = Mo[0].v;
-                            Mold[1] = Mo[1].v;
-                        }
-                        c.storage_old[2 * cell] = Mold[0];
-                        c.storage_old[2 * cell + 1] = Mold[1];
-                    } else {
-                        Mold[0] = c.storage_old[2 * cell];
-                        Mold[1] = c.storage_old[2 * cell + 1];
-                    }
-                } else {
-                    IQ qo = intensive(c, cell, c.x1.data());
-                    Ev Mo[2];
-                    storage_of(c, cell, qo, Mo);
-                    Mold[0] = Mo[0].v;
-                    Mold[1] = Mo[1].v;
-                }
-                double scv = c.V * 1.0;
-                for (int k = 0; k < 2; ++k) {
-                    Ev tmp = M[k] - Mold[k];
-                    tmp = tmp * (scv / dt);
-                    res[0][k] = res[0][k] + tmp;
-                }
-            }
-            // boundary (fvbaselocalresidual.hh:425-487)
-            for (int dir = 0; dir < nfaces; ++dir) {
-                if (neighbour(c, ijk, dir) >= 0) continue;
-                int64_t bf = boundary_face_index(c, ijk, dir);
-                int t = c.bc_type[bf];
-                Ev Bv[2];
-                if (t == EWG_BC_FREE_FLOW) {
-                    freeflow_flux(c, iq[0], cell, ijk, dir, c.bc_val[2 * bf], c.bc_val[2 * bf + 1], Bv);
-                } else if (t == EWG_BC_MASS_RATE) {
-                    Bv[0] = Ev(c.bc_val[2 * bf]);
-                    Bv[1] = Ev(c.bc_val[2 * bf + 1]);
-                } else {
-                    Bv[0] = Ev(0.0);
-                    Bv[1] = Ev(0.0);
-                }
-                double mult = c.area[dir >> 1] * 1.0;
-                for (int k = 0; k < 2; ++k) res[0][k] = res[0][k] + Bv[k] * mult;
-            }
-            // volumetric residual (fvbaselocalresidual.hh:170-186)
-            for (int l = 0; l < nd; ++l)
-                for (int k = 0; k < 2; ++k) res[l][k] = res[l][k] / c.V;
+            element_local(c, cell, ijk, iqs[cell], iqs, Mold, dt, nd, dofs, res);
             // scatter (fvbaselinearizer.hh:495-508)
             R[2 * cell] += res[0][0].v;
             R[2 * cell + 1] += res[0][1].v;
+            double d[7][2][2];  // [dof][eq][pv]
+            for (int l = 0; l < nd; ++l)
+                for (int k = 0; k < 2; ++k)
+                    for (int pv = 0; pv < 2; ++pv) d[l][k][pv] = res[l][k].d[pv];
+            if (fd) {
+                for (int pv = 0; pv < 2; ++pv) {
+                    const double eps = baseEpsilon / pvWeight[pv];
+                    double pvs[2] = {x[2 * cell], x[2 * cell + 1]};
+                    double delta = 0.0;
+                    Ev hi[7][2], lo[7][2];
+                    int nd2 = 0;
+                    int64_t dofs2[7];
+                    if (method >= 0) {
+                        pvs[pv] += eps;
+                        delta += eps;
+                        element_local(c, cell, ijk, intensive_at(c, cell, pvs[0], pvs[1]), iqs,
+                                      Mold, dt, nd2, dofs2, hi);
+                    } else {
+                        for (int l = 0; l < nd; ++l) hi[l][0] = res[l][0], hi[l][1] = res[l][1];
+                    }
+                    if (method <= 0) {
+                        pvs[pv] -= delta + eps;
+                        delta += eps;
+                        element_local(c, cell, ijk, intensive_at(c, cell, pvs[0], pvs[1]), iqs,
+                                      Mold, dt, nd2, dofs2, lo);
+                    } else {
+                        for (int l = 0; l < nd; ++l) lo[l][0] = res[l][0], lo[l][1] = res[l][1];
+                    }
+                    for (int l = 0; l < nd; ++l)
+                        for (int k = 0; k < 2; ++k) {
+                            double v = hi[l][k].v;
+                            v -= lo[l][k].v;
+                            d[l][k][pv] = v / delta;
+                        }
+                }
+            }
             for (int l = 0; l < nd; ++l) {
                 int64_t p = find_block(c, dofs[l], cell);
                 double *blk = J + 4 * p;
                 for (int k = 0; k < 2; ++k)
-                    for (int pv = 0; pv < 2; ++pv) blk[2 * k + pv] += res[l][k].d[pv];
+                    for (int pv = 0; pv < 2; ++pv) blk[2 * k + pv] += d[l][k][pv];
             }
         }
     });
@@ -948,6 +1018,7 @@ bool proceed(const ewo_ctx &c) {  // newtonmethod.hh:866-884
 int newton(ewo_ctx &c, double dt) {
     c.num_iterations = 0;
     c.newton_errors.clear();
+    c.pref = c.x0[0];  // ImmiscibleModel::updateBegin (immisciblemodel.hh:278-293)
     std::vector<double> cur;
     std::vector<double> dx;
     c.resid.assign(2 * c.N, 0.0);
@@ -1140,6 +1211,7 @@ int ewo_linearize(ewo_ctx *c, double dt, int newton_iter, double *resid, double 
     if (!c->static_set) return fail(c, EWG_INVALID_STATE, "static fields not set");
     c->resid.assign(2 * c->N, 0.0);
     c->jac.assign(4 * c->colidx.size(), 0.0);
+    c->pref = c->x0[0];
     linearize_all(*c, dt, newton_iter, c->resid.data(), c->jac.data());
     c->have_system = true;
     if (resid) std::copy(c->resid.begin(), c->resid.end(), resid);

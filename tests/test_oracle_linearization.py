@@ -165,3 +165,46 @@ def test_threaded_linearization_is_bitwise_identical():
         out.append(_residual_at(o, P, x, P.dt, problems.perturbed_state(P, seed=11, dS=0.1)))
         out.append(o.linearize(P.dt, 1)[1])
     assert np.array_equal(out[0], out[2]) and np.array_equal(out[1], out[3])
+
+
+def _fd_vs_ad(P, method, seeds=(1, 2)):
+    """Residual and Jacobian of the AD and the FD local linearizer at the same state."""
+    import ewg
+    from ewg import problems
+    x1 = problems.perturbed_state(P, seed=seeds[0], dS=0.1)
+    x = problems.perturbed_state(P, seed=seeds[1], dS=0.2)
+    out = []
+    for ll in (0, 1):
+        o = _oracle(P, ewg.default_params(local_linearizer=ll, numeric_difference_method=method))
+        out.append((_residual_at(o, P, x, P.dt, x1), o.linearize(P.dt, 1)[1]))
+    return out
+
+
+def block_column_error(J, J_ref):
+    """max over blocks and primary variables of |J - J_ref| / max_eq |J_ref| of the
+    block's column (2x2 row-major blocks)."""
+    Jb, Rb = J.reshape(-1, 2, 2), J_ref.reshape(-1, 2, 2)
+    scale = np.maximum(np.abs(Rb).max(axis=1, keepdims=True), 1e-300)
+    return float((np.abs(Jb - Rb) / scale).max())
+
+
+@pytest.mark.parametrize("method", [1, 0, -1])
+@pytest.mark.parametrize("which", ["lens", "het3d"])
+def test_fd_local_linearizer_matches_ad(which, method):
+    """FvBaseFdLocalLinearizer semantics (fvbasefdlocallinearizer.hh:370-470): the
+    residual is the undeflected one, bit for bit the AD residual; the Jacobian differs
+    from the AD Jacobian by the differences' truncation and round-off only (eps = 9.1e-11
+    for the saturation, 9.1e-11 p_ref / 10 for the pressure): below 2e-3 of each block
+    column, with the reference's forward (+1), central (0) and backward (-1) differences."""
+    from ewg import problems
+    P = (problems.lens_problem(nx=12, ny=8) if which == "lens"
+         else problems.heterogeneous_3d(8, 6, 5, seed=3, patch=2))
+    (r_ad, J_ad), (r_fd, J_fd) = _fd_vs_ad(P, method)
+    assert np.array_equal(r_fd, r_ad)
+    err = block_column_error(J_fd, J_ad)
+    print(which, method, err)
+    assert err < 2e-3
+    # a dropped derivative term is far outside that bar: half of the saturation column
+    J_bad = J_ad.copy().reshape(-1, 2, 2)
+    J_bad[:, :, 1] *= 0.5
+    assert block_column_error(J_fd, J_bad.ravel()) > 0.5
