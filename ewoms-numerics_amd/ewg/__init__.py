@@ -39,6 +39,14 @@ def load():
                                            C.POINTER(C.c_double)]
         lib.ewg_device_info.restype = C.c_int
         _D = C.POINTER(C.c_double)
+        from .capi import RestartClock
+        lib.ewg_restart_file_name.restype = C.c_int
+        lib.ewg_restart_file_name.argtypes = [C.c_char_p, C.c_char_p, C.c_double, C.c_int32,
+                                              C.c_char_p, C.c_int32]
+        for name in ("write_restart", "read_restart"):
+            fn = getattr(lib, "ewg_" + name)
+            fn.restype = C.c_int
+            fn.argtypes = [vp, C.c_char_p, C.POINTER(RestartClock)]
         for name, args in (("get_preconditioner", [vp, _D]),
                            ("apply_preconditioner", [vp, _D, _D]),
                            ("apply_operator", [vp, _D, _D])):
@@ -87,6 +95,19 @@ class GpuContext(Context):
         self._check(self.lib.ewg_apply_operator(self.h, x.ctypes.data_as(P), y.ctypes.data_as(P)))
         return y
 
+    def write_restart(self, path, **clock):
+        """Restart file of solution(0) with the given clock fields (ewg_write_restart)."""
+        from .capi import RestartClock
+        k = RestartClock(**clock)
+        self._check(self.lib.ewg_write_restart(self.h, str(path).encode(), C.byref(k)))
+
+    def read_restart(self, path):
+        """Load a restart file (solution(0) = solution(1) = its state); returns its clock."""
+        from .capi import RestartClock
+        k = RestartClock()
+        self._check(self.lib.ewg_read_restart(self.h, str(path).encode(), C.byref(k)))
+        return {n: getattr(k, n) for n, _ in k._fields_}
+
     def kernel_samples(self, which: int):
         """(count, total_ms) of the event-timed launches of kernel `which` (needs
         params.kernel_events = 1)."""
@@ -94,6 +115,16 @@ class GpuContext(Context):
         ms = C.c_double(0.0)
         self._check(self.lib.ewg_kernel_samples(self.h, which, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+
+def restart_file_name(output_dir, sim_name, t, rank=0) -> str:
+    """The reference's restart file name (Restart::restartFileName_)."""
+    buf = C.create_string_buffer(4096)
+    rc = load().ewg_restart_file_name(str(output_dir).encode(), sim_name.encode(), t, rank, buf,
+                                      4096)
+    if rc != 0:
+        raise ValueError("restart file name too long")
+    return buf.value.decode()
 
 
 def device_info() -> str:

@@ -69,6 +69,12 @@ class Params(C.Structure):
                 ("reserved", C.c_int32 * 1)]
 
 
+class RestartClock(C.Structure):
+    _fields_ = [("episode_idx", C.c_int32), ("episode_start_time", C.c_double),
+                ("episode_length", C.c_double), ("start_time", C.c_double),
+                ("time", C.c_double), ("time_step_idx", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [("newton_iterations", C.c_int64),
                 ("linear_iterations", C.c_int64),

@@ -245,6 +245,31 @@ int ewg_newton(ewg_ctx *ctx, double dt, int32_t *iterations, double *final_error
  * initial step dt0; dt is capped by params.max_time_step_size. steps_out may be NULL. */
 int ewg_run(ewg_ctx *ctx, double t0, double t_end, double dt0, int32_t *steps_out);
 
+/* The simulator clock a restart file carries (Simulator::serialize,
+ * ewoms/numerics/utils/simulator.hh:947-958). */
+typedef struct ewg_restart_clock {
+    int32_t episode_idx;
+    double episode_start_time;
+    double episode_length;
+    double start_time;
+    double time;
+    int32_t time_step_idx;
+} ewg_restart_clock;
+
+/* Restart::restartFileName_ (ewoms/numerics/io/restart.hh:70-85):
+ * "<output_dir>/<sim_name>_time=<t>_rank=<rank>.ers" into buf (len bytes). */
+int ewg_restart_file_name(const char *output_dir, const char *sim_name, double t, int32_t rank,
+                          char *buf, int32_t len);
+/* Simulator::serialize (utils/simulator.hh:923-937) of one process with VTK output off:
+ * Restart::serializeBegin (restart.hh:91-106), the clock, then solution(0) of every
+ * element (EcfvDiscretization::serialize, ecfvdiscretization.hh:186-188;
+ * FvBaseDiscretization::serializeEntity, fvbasediscretization.hh:1445-1460). */
+int ewg_write_restart(ewg_ctx *ctx, const char *path, const ewg_restart_clock *clock);
+/* The inverse (Simulator::run's restart branch, utils/simulator.hh:670-681;
+ * EcfvDiscretization::deserialize, ecfvdiscretization.hh:197-202): checks the cookie
+ * against the context's grid, loads solution(0) and sets solution(1) = solution(0). */
+int ewg_read_restart(ewg_ctx *ctx, const char *path, ewg_restart_clock *clock);
+
 int ewg_get_stats(const ewg_ctx *ctx, ewg_stats *stats);
 int ewg_reset_stats(ewg_ctx *ctx);
 
