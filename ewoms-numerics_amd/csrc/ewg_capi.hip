@@ -796,6 +796,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     if ((e = hipMemset(c->bc_type, 0, c->B * sizeof(int))) != hipSuccess) return fail("bc", e);
     if ((e = hipMemset(c->bc_val, 0, 2 * c->B * sizeof(double))) != hipSuccess) return fail("bcv", e);
     if ((e = hipMemset(c->bc_aux, 0, 3 * c->B * sizeof(double))) != hipSuccess) return fail("bca", e);
+    c->regs_host = regs;
     if ((e = hipMemcpy(c->regs, regs.data(), regs.size() * sizeof(MatReg), hipMemcpyHostToDevice)) !=
         hipSuccess)
         return fail("regs", e);

@@ -36,6 +36,9 @@ struct ewg_ctx {
     double *K = nullptr, *phi = nullptr, *pe = nullptr;
     int *region = nullptr;
     MatReg *regs = nullptr;
+    std::vector<MatReg> regs_host;     // the same regions on the host (output fields)
+    // VTK output (ewg_write_vtk): the steps written so far, for the collection file
+    std::vector<std::pair<double, std::string>> vtk_steps;
     int *bc_type = nullptr;
     double *bc_val = nullptr;
     double *bc_aux = nullptr;  // material-law values of the free-flow boundary states

@@ -43,6 +43,8 @@ def load():
         lib.ewg_restart_file_name.restype = C.c_int
         lib.ewg_restart_file_name.argtypes = [C.c_char_p, C.c_char_p, C.c_double, C.c_int32,
                                               C.c_char_p, C.c_int32]
+        lib.ewg_write_vtk.restype = C.c_int
+        lib.ewg_write_vtk.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_double]
         for name in ("write_restart", "read_restart"):
             fn = getattr(lib, "ewg_" + name)
             fn.restype = C.c_int
@@ -107,6 +109,10 @@ class GpuContext(Context):
         k = RestartClock()
         self._check(self.lib.ewg_read_restart(self.h, str(path).encode(), C.byref(k)))
         return {n: getattr(k, n) for n, _ in k._fields_}
+
+    def write_vtk(self, output_dir, sim_name, t):
+        """VTK output of solution(0) (ewg_write_vtk)."""
+        self._check(self.lib.ewg_write_vtk(self.h, str(output_dir).encode(), sim_name.encode(), t))
 
     def kernel_samples(self, which: int):
         """(count, total_ms) of the event-timed launches of kernel `which` (needs

@@ -270,6 +270,14 @@ int ewg_write_restart(ewg_ctx *ctx, const char *path, const ewg_restart_clock *c
  * against the context's grid, loads solution(0) and sets solution(1) = solution(0). */
 int ewg_read_restart(ewg_ctx *ctx, const char *path, ewg_restart_clock *clock);
 
+/* VtkMultiWriter::beginWrite/endWrite (ewoms/numerics/io/vtkmultiwriter.hh:180-260) with
+ * the default fields of VtkMultiPhaseModule (io/vtkmultiphasemodule.hh:61-72, :311-337):
+ * "<output_dir>/<sim_name>-<5-digit index>.vtu" (the index counts this context's calls)
+ * with cell data pressure_w/n, density_w/n, saturation_w/n, relativePerm_w/n and
+ * porosity of solution(0), and "<output_dir>/<sim_name>.pvd" listing every file written
+ * with its time t. ASCII Float64, 17 significant digits. */
+int ewg_write_vtk(ewg_ctx *ctx, const char *output_dir, const char *sim_name, double t);
+
 int ewg_get_stats(const ewg_ctx *ctx, ewg_stats *stats);
 int ewg_reset_stats(ewg_ctx *ctx);
 
