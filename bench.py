@@ -338,22 +338,10 @@ def main():
             from ewg import cnn
             cnn.attach(ctx, net)
         n, el, st = timed_run(ctx, P, args.warmup, args.steps)
-        # the same form over the CPU baseline's sample (the first steps from t = 0)
-        ctx.set_solution(P.x0, 0)
-        ctx.set_solution(P.x0, 1)
-        nf, elf, _ = timed_run(ctx, P, 0, args.cpu_sample_steps)
-        forms[form] = {"value": P.N * n / el, "steps": n, "seconds": el,
-                       "first_steps": {"steps": nf, "seconds": elf, "value": P.N * nf / elf},
-                       "newton_iterations": st["newton_iterations"],
-                       "linear_iterations": st["linear_iterations"],
-                       "linear_solves": st["linear_solves"],
-                       "failed_steps": st["failed_steps"],
-                       "split_s": {"linearize": st["time_linearize_s"],
-                                   "solve": st["time_solve_s"],
-                                   "update": st["time_update_s"]}}
         # launches inside the timed run, bracketed by device events on the library's
         # stream (the first apply and SpMV of each batch of BiCGStab iterations, every
-        # factorization and linearization); mean duration and algorithmic bandwidth
+        # factorization and linearization); mean duration and algorithmic bandwidth.
+        # Read before the sample run below, which resets them.
         ab = algorithmic_bytes(P, ctx.nnzb, fp32)
         ks = {}
         for name, kid in (("linearize", EWG_KERNEL_LINEARIZE), ("spmv", EWG_KERNEL_SPMV),
@@ -366,14 +354,29 @@ def main():
             gbs = ab[name] / (ms * 1e-3) / 1e9
             ks[name] = {"mean_ms": ms, "samples": cnt, "bytes": ab[name], "achieved_GBs": gbs,
                         "frac": gbs / HBM_PEAK_GBS}
-        forms[form]["kernels"] = ks
+        warm = None
         if FORMS[form].get("warm_start"):
             from ewg import cnn
             ws = cnn.warm_stats(ctx)
-            forms[form]["warm_start"] = {"accepted": ws["accepted"], "rejected": ws["rejected"],
-                                         "predict_s": ws["predict_s"],
-                                         "last_region_cells": net.last_inference_cells(),
-                                         "network": net_info}
+            warm = {"accepted": ws["accepted"], "rejected": ws["rejected"],
+                    "predict_s": ws["predict_s"],
+                    "last_region_cells": net.last_inference_cells(), "network": net_info}
+        # the same form over the CPU baseline's sample (the first steps from t = 0)
+        ctx.set_solution(P.x0, 0)
+        ctx.set_solution(P.x0, 1)
+        nf, elf, _ = timed_run(ctx, P, 0, args.cpu_sample_steps)
+        forms[form] = {"value": P.N * n / el, "steps": n, "seconds": el,
+                       "first_steps": {"steps": nf, "seconds": elf, "value": P.N * nf / elf},
+                       "newton_iterations": st["newton_iterations"],
+                       "linear_iterations": st["linear_iterations"],
+                       "linear_solves": st["linear_solves"],
+                       "failed_steps": st["failed_steps"],
+                       "split_s": {"linearize": st["time_linearize_s"],
+                                   "solve": st["time_solve_s"],
+                                   "update": st["time_update_s"]},
+                       "kernels": ks}
+        if warm is not None:
+            forms[form]["warm_start"] = warm
         print(f"# {form} kernels {ks}", file=sys.stderr, flush=True)
         if form == "fp64":
             ctx.time_kernel(EWG_KERNEL_COPY, 1)
