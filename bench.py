@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--kernel", choices=["linearize", "spmv", "ilu_apply", "ilu_factor", "copy"],
                     help="time one kernel only (the first form of --forms) and exit")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--cold", action="store_true",
+                    help="with --kernel: read 1 GiB of other data before every launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -483,11 +485,15 @@ def kernel_mode(args):
     ctx = ewg.GpuContext(P, form_params(form, max_time_step_size=P.dt))
     ctx.run(0.0, 2 * P.dt, P.dt)
     ctx.linearize(P.dt, 1, want=False)
+    if args.cold:
+        from ewg.capi import EWG_KERNEL_COLD
+        kid |= EWG_KERNEL_COLD
     ctx.time_kernel(kid, 1)
     ms = ctx.time_kernel(kid, args.reps)
     nbytes = (2 * EWG_COPY_BYTES if args.kernel == "copy" else
               algorithmic_bytes(P, ctx.nnzb, bool(FORMS[form].get("precond_fp32")))[args.kernel])
-    print(json.dumps({"kernel": args.kernel, "form": form, "reps": args.reps, "mean_ms": ms,
+    print(json.dumps({"kernel": args.kernel, "form": form, "reps": args.reps, "cold": args.cold,
+                      "mean_ms": ms,
                       "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 

@@ -13,7 +13,10 @@ namespace ewg {
 constexpr int kWfTX = 8;   // tile extent in x (lanes)
 constexpr int kWfTY = 8;   // tile extent in y
 constexpr int kFactorPD = 3;    // prefetch depth of the wavefront-layout factorization
-constexpr int kSweepPD = 6;     // prefetch depth of the triangular sweeps' operands
+#ifndef EWG_SWEEP_PD
+#define EWG_SWEEP_PD 6
+#endif
+constexpr int kSweepPD = EWG_SWEEP_PD;  // prefetch depth of the triangular sweeps' operands
 constexpr int kSweepPE = 1;     // prefetch depth of their cross-tile words
 
 constexpr int kWfMaxWords = 4;  // 64-bit words per exchanged value (FP64 2x2 block)

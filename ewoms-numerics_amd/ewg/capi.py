@@ -29,6 +29,7 @@ EWG_KERNEL_SPMV = 1
 EWG_KERNEL_ILU_APPLY = 2
 EWG_KERNEL_ILU_FACTOR = 3
 EWG_KERNEL_COPY = 5
+EWG_KERNEL_COLD = 0x100  # or-ed into a kernel id: the caches hold other data before each launch
 
 
 class GridDesc(C.Structure):

@@ -288,6 +288,10 @@ int ewg_reset_stats(ewg_ctx *ctx);
  * first use (reads and writes EWG_COPY_BYTES each: the bandwidth reference). */
 enum { EWG_KERNEL_LINEARIZE = 0, EWG_KERNEL_SPMV = 1, EWG_KERNEL_ILU_APPLY = 2,
        EWG_KERNEL_ILU_FACTOR = 3, EWG_KERNEL_COPY = 5 };
+/* or-ed into `which` (not with EWG_KERNEL_COPY): before every launch a read-only pass over
+ * EWG_COPY_BYTES fills the caches with other data, so that each launch starts with none of
+ * its operands cached; only the launches are timed */
+#define EWG_KERNEL_COLD 0x100
 #define EWG_COPY_BYTES (1ull << 30)
 int ewg_time_kernel(ewg_ctx *ctx, int which, int reps, double *mean_ms);
 

@@ -187,9 +187,10 @@ def test_rejected_prediction_falls_back_to_the_plain_guess():
 
 def test_trained_warm_start_on_a_held_out_member():
     """Train on pairs harvested from plain runs of ensemble members, then step a held-out
-    member from the prediction: every step converges from it (none is rejected), none
-    takes more Newton iterations than from the plain guess, and the saturations stay
-    within the Newton tolerance's reach of the plain run."""
+    member: each step is taken by both contexts from the same state (the plain run's), the
+    one from the prediction, the other from the plain guess. Every step converges from the
+    prediction (none is rejected), none takes more Newton iterations than from the plain
+    guess, and the two results of a step agree within 1e-6 in saturation."""
     import ewg
     from ewg import cnn
     log = lambda m: print(m, flush=True)
@@ -202,18 +203,20 @@ def test_trained_warm_start_on_a_held_out_member():
     warm = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6, warm_start=1))
     cnn.attach(warm, pred)
     for step in range(12):
+        x = plain.solution(0)
+        warm.set_solution(x, 0)
+        warm.set_solution(x, 1)
         a = plain.newton(P.dt)
         e_plain = cnn.warm_stats(plain)["last_initial_error"]
         b = warm.newton(P.dt)
         e_warm = cnn.warm_stats(warm)["last_initial_error"]
-        plain.advance_time_level()
-        warm.advance_time_level()
         dS = np.abs(plain.solution(0)[1::2] - warm.solution(0)[1::2]).max()
+        plain.advance_time_level()
         log(f"step {step}: newton plain {a[1]} warm {b[1]}; initial error plain {e_plain:.3e} "
             f"warm {e_warm:.3e}; max |dS| vs plain {dS:.2e}")
         assert a[0] == 0 and b[0] == 0
         assert b[1] <= a[1]
-        assert dS < 1e-5
+        assert dS < 1e-6
     st = cnn.warm_stats(warm)
     log(f"warm start: {st}")
     assert st["accepted"] == 12 and st["rejected"] == 0
@@ -221,41 +224,51 @@ def test_trained_warm_start_on_a_held_out_member():
 
 def test_trained_warm_start_saves_work_on_the_1m_cell_case():
     """The 128x128x64 heterogeneous case, 20 steps of 500 s at Newton tolerance 1e-6 with
-    the FP32 preconditioner and face fluxes, from the plain guess and from the network
-    (trained on 2000 pairs of other realizations): the prediction removes the steps whose
-    second Newton iteration just misses the tolerance, and with them most of the BiCGStab
-    work after the first step."""
+    the FP32 preconditioner and face fluxes, each step taken from the plain run's state
+    twice: from the plain guess and from the network's (trained on 2000 pairs of other
+    realizations). The prediction removes the steps whose second Newton iteration just
+    misses the tolerance, and with them most of the BiCGStab work after the first step;
+    every step's saturations agree with the plain step's within 1e-6 (the two solves stop
+    at different iterates inside the Newton tolerance; over whole trajectories these
+    differences add up, which the benchmark reports separately)."""
     import ewg
     from ewg import cnn, problems
     log = lambda m: print(m, flush=True)
     P = problems.heterogeneous_3d(128, 128, 64)
     pred, info = cnn.train_warm_start(P.nx, P.ny, P.nz, n_pairs=2000, epochs=30, seed=2)
     log(f"training: {info}")
-    totals = {}
+    ctxs = {}
     for form in ("plain", "warm"):
         prm = ewg.default_params(newton_tolerance=1e-6, max_time_step_size=P.dt,
                                  precond_fp32=1, linearize_fp32=1,
                                  warm_start=1 if form == "warm" else 0)
-        ctx = ewg.GpuContext(P, prm)
-        if form == "warm":
-            cnn.attach(ctx, pred)
-        newton, krylov = [], []
-        for step in range(20):
+        ctxs[form] = ewg.GpuContext(P, prm)
+    plain, warm = ctxs["plain"], ctxs["warm"]
+    cnn.attach(warm, pred)
+    counts = {"plain": ([], []), "warm": ([], [])}
+    worst = 0.0
+    for step in range(20):
+        x = plain.solution(0)
+        warm.set_solution(x, 0)
+        warm.set_solution(x, 1)
+        for form, ctx in ctxs.items():
             k0 = ctx.stats()["linear_iterations"]
             rc, it, err = ctx.newton(P.dt)
             assert rc == 0
-            ctx.advance_time_level()
-            newton.append(it)
-            krylov.append(ctx.stats()["linear_iterations"] - k0)
-        totals[form] = (sum(newton), sum(krylov), ctx.solution(0))
-        log(f"{form}: newton {newton} krylov {krylov}")
-        if form == "warm":
-            st = cnn.warm_stats(ctx)
-            log(f"warm start: {st}")
-            assert st["accepted"] == 20 and st["rejected"] == 0
+            counts[form][0].append(it)
+            counts[form][1].append(ctx.stats()["linear_iterations"] - k0)
+        dS = np.abs(plain.solution(0)[1::2] - warm.solution(0)[1::2]).max()
+        worst = max(worst, dS)
+        log(f"step {step}: newton {counts['plain'][0][-1]} / {counts['warm'][0][-1]}, "
+            f"bicgstab {counts['plain'][1][-1]} / {counts['warm'][1][-1]}, max |dS| {dS:.2e}")
+        assert dS < 1e-6
+        plain.advance_time_level()
+    st = cnn.warm_stats(warm)
+    log(f"warm start: {st}")
+    assert st["accepted"] == 20 and st["rejected"] == 0
+    for ctx in ctxs.values():
         ctx.close()
-    (np_, kp, xp), (nw, kw, xw) = totals["plain"], totals["warm"]
-    dS = np.abs(xp[1::2] - xw[1::2]).max()
-    log(f"newton {np_} -> {nw}, bicgstab {kp} -> {kw}, max |dS| warm vs plain {dS:.2e}")
+    np_, kp = sum(counts["plain"][0]), sum(counts["plain"][1])
+    nw, kw = sum(counts["warm"][0]), sum(counts["warm"][1])
+    log(f"newton {np_} -> {nw}, bicgstab {kp} -> {kw}, max |dS| of a step {worst:.2e}")
     assert nw <= np_ and kw < 0.75 * kp
-    assert dS < 1e-5

@@ -92,6 +92,37 @@ def test_fp64_members_match_the_oracle(mode, schedule):
         assert dS < 1e-8
 
 
+@pytest.mark.parametrize("mode,schedule", [("batched", 0), ("batched", 1), ("persistent", 0)])
+def test_fp32_preconditioner_members_close_to_the_fp64_oracle(mode, schedule):
+    """The ensemble's own form (FP32 ILU(0) inside FP64 BiCGStab, Newton tolerance 1e-6)
+    against the FP64 oracle at the same tolerance, step by step: saturations within 1e-6
+    and Newton iterations within one of the oracle's in every step."""
+    import ewg
+    from ewg import ensemble
+    from oracle.pyoracle import OracleContext
+    probs = _problems()[:3]
+    prm = ensemble.member_params(precond_fp32=1, newton_tolerance=1e-6, ilu_schedule=schedule)
+    ctxs = [ewg.GpuContext(P, prm) for P in probs]
+    ens = ensemble.Ensemble(ctxs)
+    ens.set_mode(ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT, 512)
+    oracles = [OracleContext(P, ewg.default_params(newton_tolerance=1e-6)) for P in probs]
+    for step in range(3):
+        before = [ens.stats(i)["newton_iterations"] for i in range(len(probs))]
+        ens.run(probs[0].dt, 1)
+        for i, (P, o) in enumerate(zip(probs, oracles)):
+            rc, it, _ = o.newton(P.dt)
+            assert rc == 0
+            o.advance_time_level()
+            st = ens.stats(i)
+            ie = st["newton_iterations"] - before[i]
+            dS = np.abs(ctxs[i].solution(0)[1::2] - o.solution(0)[1::2]).max()
+            print(f"{mode} schedule {schedule} step {step} member {i}: newton {ie} vs oracle "
+                  f"{it}; max |dS| {dS:.2e}", flush=True)
+            assert st["status"] == 0
+            assert abs(ie - it) <= 1
+            assert dS < 1e-6
+
+
 @pytest.mark.parametrize("mode", ["batched", "persistent"])
 def test_failing_member_is_isolated(mode):
     """A member whose Newton solve fails (here: NewtonMaxError below its first error) is
