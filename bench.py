@@ -300,6 +300,9 @@ def main():
                     help="skip the untimed per-step comparison of the reduced-precision forms")
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble leg (config 3)")
+    ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--copy-reps", type=int, default=10)
     ap.add_argument("--kernel", choices=["linearize", "spmv", "ilu_apply", "ilu_factor", "copy"],
                     help="time one kernel only (the first form of --forms) and exit")
@@ -402,6 +405,17 @@ def main():
                   f"{max(dev['max_dS_vs_converged']):.3e} (fp64 at the same tolerance "
                   f"{max(dev['max_dS_fp64_vs_converged']):.3e})", file=sys.stderr, flush=True)
 
+    ens_line = None
+    if not args.no_ensemble:
+        ens_line = ensemble_leg(args)
+        print(f"# ensemble {ens_line['members']} members {ens_line['cells']} cells: "
+              f"{ens_line['value'] / 1e6:.1f}M cell-timesteps/s, replay {ens_line['tuning']['replay']}"
+              + ("; spot check " + ", ".join(
+                  f"member {c['member']} max dS {c['max_dS_vs_fp64_oracle']:.1e} newton "
+                  f"{c['newton']}/{c['newton_oracle']}"
+                  for c in ens_line["cpu_baseline"]["spot_check"])
+                 if ens_line.get("cpu_baseline") else ""), file=sys.stderr, flush=True)
+
     # the metric's form: learned warm start + FP32 preconditioner (+ FP32 face fluxes) when
     # it ran, compared with the plain FP64 path
     head_form = next((f for f in ("learned_mixed", "fp64") if f in forms), next(iter(forms)))
@@ -456,6 +470,7 @@ def main():
         "copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ensemble": ens_line,
     }
     print(json.dumps(line), flush=True)
     # one compact closing line on stderr (the tail a log keeps)
@@ -466,8 +481,89 @@ def main():
           + (f"; max |dS| vs fp64 {max(dev['max_dS']):.2e} (one step "
              f"{max(dev['max_dS_one_step']):.2e})" if dev else "")
           + (f"; roofline {roofline['kernel']} {roofline['frac']:.3f}" if roofline else "")
-          + (f"; copy {line['copy_GBs']} GB/s" if line["copy_GBs"] else ""),
+          + (f"; copy {line['copy_GBs']} GB/s" if line["copy_GBs"] else "")
+          + (f"; ensemble {ens_line['value'] / 1e6:.1f}M" if ens_line else ""),
           file=sys.stderr, flush=True)
+
+
+def ensemble_leg(args):
+    """The config-3 ensemble: realizations of the same physics in the sizes of
+    problems.ENSEMBLE_SIZES (seed 3), FP64 state with the FP32 preconditioner, Newton
+    tolerance 1e-6, dt 500 s, `steps` steps, all members time-stepped together in the
+    batched mode; the tuner picks the batched mode's BiCGStab replay depth on one trial
+    step first (the persistent mode's trials, one step of every member in one workgroup
+    each, take seconds apiece and are left to `python -m ewg.ensemble --tune`)."""
+    import numpy as np
+    from ewg import ensemble, problems
+    t0 = time.perf_counter()
+    probs, ctxs = ensemble.build_members(args.ensemble_members, 3, args.steps)
+    ens = ensemble.Ensemble(ctxs)
+    t1 = time.perf_counter()
+    dt = probs[0].dt
+    tuning = ens.tune(dt, modes=ensemble.TUNE_BATCHED)
+    t2 = time.perf_counter()
+    sec = ens.run(dt, args.steps)
+    st = [ens.stats(i) for i in range(len(probs))]
+    cells = np.array([P.N for P in probs])
+    done = np.array([x["steps"] for x in st])
+    out = {"metric": "ensemble cell-timesteps/s (cells x completed steps over all members / wall time)",
+           "value": float((cells * done).sum() / sec), "unit": "cell-timesteps/s",
+           "members": len(probs), "cells": int(cells.sum()), "steps": args.steps,
+           "seconds": sec, "completed_steps": int(done.sum()),
+           "failed_members": int(sum(1 for x in st if x["status"] != 0)),
+           "step_divisions": int(sum(x["step_divisions"] for x in st)),
+           "newton_iterations": int(sum(x["newton_iterations"] for x in st)),
+           "linear_iterations": int(sum(x["linear_iterations"] for x in st)),
+           "sizes": {f"{a}x{b}x{c}": int(sum(1 for P in probs if (P.nx, P.ny, P.nz) == (a, b, c)))
+                     for (a, b, c) in problems.ENSEMBLE_SIZES},
+           "mode": "batched", "tuning": tuning, "setup_s": round(t1 - t0, 2),
+           "tune_s": round(t2 - t1, 2),
+           "workload": f"{len(probs)} realizations, {int(cells.sum())} cells, FP64 state, FP32 "
+                       f"ILU(0) preconditioner, Newton tol 1e-6, dt {dt:g} s, seed 3"}
+    states = {i: ctxs[i].solution(0) for i in range(len(probs))}
+    ens.close()
+    for c in ctxs:
+        c.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = ensemble_cpu_baseline(probs, st, states, args.steps)
+    return out
+
+
+def ensemble_cpu_baseline(probs, st, states, steps):
+    """The oracle (FP64 throughout, one thread) on the two smallest members that completed
+    every step undivided: its rate over those members, and, as a spot check of the
+    ensemble's results, each member's saturations and Newton iterations against it."""
+    import numpy as np
+    import ewg
+    from oracle.pyoracle import OracleContext
+    pick = sorted((i for i, x in enumerate(st) if x["status"] == 0 and x["step_divisions"] == 0
+                   and x["steps"] == steps), key=lambda i: (probs[i].N, i))[:2]
+    checks, cells, seconds = [], 0, 0.0
+    for i in pick:
+        P = probs[i]
+        o = OracleContext(P, ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3))
+        t0 = time.perf_counter()
+        newton, ok = 0, True
+        for _ in range(steps):
+            rc, it, _ = o.newton(P.dt)
+            newton += it
+            if rc != 0:
+                ok = False
+                break
+            o.advance_time_level()
+        seconds += time.perf_counter() - t0
+        cells += P.N * steps
+        x = o.solution(0)
+        o.close()
+        checks.append({"member": i, "grid": f"{P.nx}x{P.ny}x{P.nz}", "oracle_ok": ok,
+                       "max_dS_vs_fp64_oracle": float(np.abs(states[i][1::2] - x[1::2]).max()),
+                       "max_rel_dp": float(np.abs(states[i][0::2] - x[0::2]).max() /
+                                           np.abs(x[0::2]).max()),
+                       "newton": int(st[i]["newton_iterations"]), "newton_oracle": newton})
+    return {"value": cells / seconds if seconds else None, "unit": "cell-timesteps/s",
+            "cores": 1, "kind": "port",
+            "sample": f"{len(pick)} smallest members, all {steps} steps, in {seconds:.1f} s",
+            "spot_check": checks}
 
 
 def kernel_mode(args):

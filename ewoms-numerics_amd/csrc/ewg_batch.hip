@@ -863,6 +863,7 @@ struct ewg_ensemble {
     int mode = 1;  // 0: persistent (one workgroup per member), 1: batched launches
     int threads = 512;
     bool largest_first = true;
+    int replay = 8;  // batched mode: most BiCGStab iterations queued between state reads
     hipGraphExec_t graph = nullptr;
     int graph_steps = -1, graph_threads = 0;
     bool graph_largest = true;
@@ -1195,8 +1196,8 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
             hipLaunchKernelGGL(k_b_set, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->sweep_err, e->force_mask);
         }
         ENS_TRY(e, hipGetLastError());
-        // iterations are replayed speculatively, the states read after 1, 2, 4, then every
-        // 8 of them (members that left the running state skip the rest)
+        // iterations are replayed speculatively, the states read after 1, 2, 4, ... then
+        // every `replay` of them (members that left the running state skip the rest)
         int batch = 1;
         while (true) {
             ENS_TRY(e, hipMemcpyAsync(e->h_ks, e->ks, n * sizeof(KrylovState), hipMemcpyDeviceToHost, s));
@@ -1206,7 +1207,7 @@ int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vec
             for (int m = 0; m < n; ++m) running |= e->h_ks[m].status == KS_RUN;
             if (!running) break;
             for (int q = 0; q < batch; ++q) ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
-            batch = std::min(batch * 2, 8);
+            batch = std::min(batch * 2, e->replay);
         }
         // a member whose wavefront wait ran out may have left unconsumed words in its
         // tile edge buffers: back to the sentinel before its next sweep
@@ -1429,8 +1430,10 @@ int ewg_ensemble_force_sweep_failure(ewg_ensemble *e, int32_t member, int32_t so
     return EWG_OK;
 }
 
-int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
-    if (!e || !(dt > 0.0)) return EWG_INVALID_ARGUMENT;
+int ewg_ensemble_tune_modes(ewg_ensemble *e, double dt, int32_t modes, ewg_ensemble_tuning *out) {
+    if (!e || !(dt > 0.0) || (modes & ~(EWG_ENSEMBLE_TUNE_BATCHED | EWG_ENSEMBLE_TUNE_PERSISTENT)) ||
+        modes == 0)
+        return EWG_INVALID_ARGUMENT;
     const int n = static_cast<int>(e->members.size());
     ewg_ensemble_tuning t{};
     double best = 1e300;
@@ -1438,27 +1441,46 @@ int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
     ENS_TRY(e, hipEventCreate(&e0));
     ENS_TRY(e, hipEventCreate(&e1));
     const std::vector<Outcome> saved = e->outcome;
-    const int cand[4][2] = {{EWG_ENSEMBLE_BATCHED, 0}, {EWG_ENSEMBLE_PERSISTENT, 256},
-                            {EWG_ENSEMBLE_PERSISTENT, 512}, {EWG_ENSEMBLE_PERSISTENT, 1024}};
-    for (const auto &c : cand) {
+    const int keep_replay = e->replay;
+    struct Cand {
+        int mode, threads, largest_first, replay;
+    };
+    const Cand cand[] = {{EWG_ENSEMBLE_BATCHED, 0, 1, 4},      {EWG_ENSEMBLE_BATCHED, 0, 1, 8},
+                         {EWG_ENSEMBLE_BATCHED, 0, 1, 16},     {EWG_ENSEMBLE_BATCHED, 0, 1, 32},
+                         {EWG_ENSEMBLE_PERSISTENT, 256, 1, 0}, {EWG_ENSEMBLE_PERSISTENT, 512, 1, 0},
+                         {EWG_ENSEMBLE_PERSISTENT, 1024, 1, 0}, {EWG_ENSEMBLE_PERSISTENT, 512, 0, 0}};
+    static_assert(sizeof(cand) / sizeof(cand[0]) <= EWG_ENSEMBLE_MAX_CANDIDATES, "candidates");
+    if (modes & EWG_ENSEMBLE_TUNE_BATCHED) {
+        // an untimed trial first: the first batched step also captures the iteration graph
+        if (int rc = batched_step(e, dt, false)) return rc;
+        e->outcome = saved;
+    }
+    for (const Cand &c : cand) {
         const int k = t.ncandidates++;
-        t.cand_mode[k] = c[0];
-        t.cand_threads[k] = c[1];
-        t.cand_largest_first[k] = 1;
+        t.cand_mode[k] = c.mode;
+        t.cand_threads[k] = c.threads;
+        t.cand_largest_first[k] = c.largest_first;
+        t.cand_replay[k] = c.replay;
+        t.cand_ms[k] = -1.0;
+        const bool batched = c.mode == EWG_ENSEMBLE_BATCHED;
+        if (!(modes & (batched ? EWG_ENSEMBLE_TUNE_BATCHED : EWG_ENSEMBLE_TUNE_PERSISTENT))) continue;
         bool ok = true;
         double ms = 0.0;
-        if (c[0] == EWG_ENSEMBLE_BATCHED) {
+        if (batched) {
+            e->replay = c.replay;
             const double t0 = now_s();
-            if (int rc = batched_step(e, dt, false)) return rc;
+            const int rc = batched_step(e, dt, false);
+            e->replay = keep_replay;
+            if (rc) return rc;
             ms = 1e3 * (now_s() - t0);
             for (int i = 0; i < n; ++i) ok &= e->outcome[i].status == 0 || saved[i].status != 0;
             e->outcome = saved;
         } else {
-            if (int rc = upload_order(e, true)) return rc;
+            if (int rc = upload_order(e, c.largest_first != 0)) return rc;
             ENS_TRY(e, hipMemcpyAsync(e->dout, saved.data(), n * sizeof(Outcome),
                                       hipMemcpyHostToDevice, e->stream));
             ENS_TRY(e, hipEventRecord(e0, e->stream));
-            if (int rc = launch(e, c[1], 1, dt, 0)) return rc;
+            if (int rc = launch(e, c.threads, 1, dt, 0)) return rc;
             ENS_TRY(e, hipEventRecord(e1, e->stream));
             std::vector<Outcome> trial(n);
             ENS_TRY(e, hipMemcpyAsync(trial.data(), e->dout, n * sizeof(Outcome),
@@ -1472,9 +1494,10 @@ int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
         t.cand_ms[k] = ok ? ms : -1.0;
         if (ok && ms < best) {
             best = ms;
-            t.mode = c[0];
-            t.threads = c[1];
-            t.largest_first = 1;
+            t.mode = c.mode;
+            t.threads = c.threads;
+            t.largest_first = c.largest_first;
+            t.replay = c.replay;
         }
     }
     (void)hipEventDestroy(e0);
@@ -1482,10 +1505,19 @@ int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
     ENS_TRY(e, hipMemcpy(e->dout, saved.data(), n * sizeof(Outcome), hipMemcpyHostToDevice));
     if (best >= 1e300) return set_err(e, EWG_NUMERICAL_ISSUE, "every candidate failed");
     e->mode = t.mode;
-    if (t.mode == EWG_ENSEMBLE_PERSISTENT) e->threads = t.threads;
-    e->largest_first = true;
+    if (t.mode == EWG_ENSEMBLE_PERSISTENT) {
+        e->threads = t.threads;
+        e->largest_first = t.largest_first != 0;
+    } else {
+        e->replay = t.replay;
+    }
     if (out) *out = t;
     return EWG_OK;
+}
+
+int ewg_ensemble_tune(ewg_ensemble *e, double dt, ewg_ensemble_tuning *out) {
+    return ewg_ensemble_tune_modes(e, dt, EWG_ENSEMBLE_TUNE_BATCHED | EWG_ENSEMBLE_TUNE_PERSISTENT,
+                                   out);
 }
 
 int ewg_ensemble_run(ewg_ensemble *e, double dt, int32_t steps, double *seconds) {
@@ -1543,6 +1575,13 @@ int ewg_ensemble_set_mode(ewg_ensemble *e, int32_t mode, int32_t threads) {
         return set_err(e, EWG_INVALID_ARGUMENT, "mode / threads");
     e->mode = mode;
     e->threads = threads;
+    return EWG_OK;
+}
+
+int ewg_ensemble_set_replay(ewg_ensemble *e, int32_t depth) {
+    if (!e) return EWG_INVALID_ARGUMENT;
+    if (depth < 1 || depth > 64) return set_err(e, EWG_INVALID_ARGUMENT, "replay depth must be 1 to 64");
+    e->replay = depth;
     return EWG_OK;
 }
 

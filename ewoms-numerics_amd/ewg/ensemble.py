@@ -39,9 +39,14 @@ MODES = {PERSISTENT: "persistent", BATCHED: "batched"}
 
 class Tuning(C.Structure):
     _fields_ = [("mode", C.c_int32), ("threads", C.c_int32), ("largest_first", C.c_int32),
-                ("ncandidates", C.c_int32), ("cand_mode", C.c_int32 * MAXC),
-                ("cand_threads", C.c_int32 * MAXC), ("cand_largest_first", C.c_int32 * MAXC),
+                ("replay", C.c_int32), ("ncandidates", C.c_int32),
+                ("cand_mode", C.c_int32 * MAXC), ("cand_threads", C.c_int32 * MAXC),
+                ("cand_largest_first", C.c_int32 * MAXC), ("cand_replay", C.c_int32 * MAXC),
                 ("cand_ms", C.c_double * MAXC)]
+
+
+TUNE_BATCHED = 1     # EWG_ENSEMBLE_TUNE_BATCHED
+TUNE_PERSISTENT = 2  # EWG_ENSEMBLE_TUNE_PERSISTENT
 
 
 _BOUND = False
@@ -56,6 +61,8 @@ def _lib():
             "ewg_ensemble_destroy": (None, [_vp]),
             "ewg_ensemble_last_error": (C.c_char_p, [_vp]),
             "ewg_ensemble_tune": (C.c_int, [_vp, C.c_double, C.POINTER(Tuning)]),
+            "ewg_ensemble_tune_modes": (C.c_int, [_vp, C.c_double, C.c_int32, C.POINTER(Tuning)]),
+            "ewg_ensemble_set_replay": (C.c_int, [_vp, C.c_int32]),
             "ewg_ensemble_set_mode": (C.c_int, [_vp, C.c_int32, C.c_int32]),
             "ewg_ensemble_run": (C.c_int, [_vp, C.c_double, C.c_int32,
                                            C.POINTER(C.c_double)]),
@@ -108,12 +115,17 @@ class Ensemble:
         except Exception:
             pass
 
-    def tune(self, dt):
+    def tune(self, dt, modes=TUNE_BATCHED | TUNE_PERSISTENT):
+        """Time the candidates on one step and keep the fastest (ewg_ensemble_tune_modes);
+        candidates of a mode left out of `modes` report ms = -1."""
         t = Tuning()
-        self._check(self.lib.ewg_ensemble_tune(self.h, dt, C.byref(t)))
+        self._check(self.lib.ewg_ensemble_tune_modes(self.h, dt, modes, C.byref(t)))
         return {"mode": MODES[t.mode], "threads": t.threads,
+                "largest_first": bool(t.largest_first), "replay": t.replay,
                 "candidates": [{"mode": MODES[t.cand_mode[k]], "threads": t.cand_threads[k],
-                                "ms": t.cand_ms[k]} for k in range(t.ncandidates)]}
+                                "largest_first": bool(t.cand_largest_first[k]),
+                                "replay": t.cand_replay[k], "ms": t.cand_ms[k]}
+                               for k in range(t.ncandidates)]}
 
     def force_sweep_failure(self, member, solves=1):
         """Test hook: the member's next `solves` batched solves start with its sweep error
@@ -122,6 +134,10 @@ class Ensemble:
 
     def set_mode(self, mode, threads=512):
         self._check(self.lib.ewg_ensemble_set_mode(self.h, mode, threads))
+
+    def set_replay(self, depth):
+        """Batched mode: most BiCGStab iterations queued between state reads."""
+        self._check(self.lib.ewg_ensemble_set_replay(self.h, depth))
 
     def run(self, dt, steps):
         sec = C.c_double(0)

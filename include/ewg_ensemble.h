@@ -21,9 +21,14 @@
  * member stays at the same time level as the others; a member whose step still fails
  * keeps its last converged state and stops (its status says why).
  *
- * ewg_ensemble_tune times the candidates (the batched mode; the persistent mode with
- * 256, 512 and 1024 threads per member) on one step from the current state, restoring it
- * afterwards, and keeps the fastest for the following runs (default: batched).
+ * ewg_ensemble_tune times the candidates on one step from the current state, restoring it
+ * afterwards, and keeps the fastest for the following runs (default: batched, replay
+ * depth 8): the batched mode with BiCGStab replay depths 4, 8, 16 and 32 (the most
+ * iterations of all members queued between two reads of their Krylov states), and the
+ * persistent mode with 256, 512 and 1024 threads per member, members largest first, and
+ * with 512 threads in creation order. ewg_ensemble_tune_modes restricts the search to the
+ * modes in `modes` (bit EWG_ENSEMBLE_TUNE_BATCHED, bit EWG_ENSEMBLE_TUNE_PERSISTENT; a
+ * persistent trial runs one whole step of every member in one workgroup each).
  */
 #ifndef EWG_ENSEMBLE_H
 #define EWG_ENSEMBLE_H
@@ -62,12 +67,17 @@ typedef struct ewg_ensemble_tuning {
     int32_t mode;              /* EWG_ENSEMBLE_* chosen */
     int32_t threads;           /* persistent mode: workgroup size (threads per member) */
     int32_t largest_first;     /* persistent mode: members ordered by decreasing size */
+    int32_t replay;            /* batched mode: BiCGStab replay depth */
     int32_t ncandidates;
     int32_t cand_mode[EWG_ENSEMBLE_MAX_CANDIDATES];
     int32_t cand_threads[EWG_ENSEMBLE_MAX_CANDIDATES];
     int32_t cand_largest_first[EWG_ENSEMBLE_MAX_CANDIDATES];
-    double cand_ms[EWG_ENSEMBLE_MAX_CANDIDATES];  /* one step; < 0: candidate failed */
+    int32_t cand_replay[EWG_ENSEMBLE_MAX_CANDIDATES];
+    double cand_ms[EWG_ENSEMBLE_MAX_CANDIDATES];  /* one step; < 0: candidate failed or not tried */
 } ewg_ensemble_tuning;
+
+#define EWG_ENSEMBLE_TUNE_BATCHED 1
+#define EWG_ENSEMBLE_TUNE_PERSISTENT 2
 
 /* An ensemble over n contexts (not owned; they must outlive it). Members linearize by
  * automatic differentiation: a context with params.local_linearizer = 1 is rejected
@@ -77,8 +87,13 @@ void ewg_ensemble_destroy(ewg_ensemble *ens);
 const char *ewg_ensemble_last_error(const ewg_ensemble *ens);
 
 int ewg_ensemble_tune(ewg_ensemble *ens, double dt, ewg_ensemble_tuning *out);
+int ewg_ensemble_tune_modes(ewg_ensemble *ens, double dt, int32_t modes,
+                            ewg_ensemble_tuning *out);
 /* Choose the mode without tuning (threads: 256, 512 or 1024 for the persistent mode). */
 int ewg_ensemble_set_mode(ewg_ensemble *ens, int32_t mode, int32_t threads);
+/* The batched mode's BiCGStab replay depth (1 to 64; default 8): results do not depend on
+ * it, only the number of host reads and of iterations queued past convergence. */
+int ewg_ensemble_set_replay(ewg_ensemble *ens, int32_t depth);
 int ewg_ensemble_run(ewg_ensemble *ens, double dt, int32_t steps, double *seconds);
 int ewg_ensemble_stats(const ewg_ensemble *ens, int32_t i,
                        ewg_ensemble_member_stats *out);

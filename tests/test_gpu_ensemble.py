@@ -166,14 +166,23 @@ def test_tuner_and_graph_replay(mode):
     print("tuning", t, flush=True)
     assert t["mode"] in ("batched", "persistent")
     assert all(c["ms"] > 0 for c in t["candidates"])
+    assert sorted({c["replay"] for c in t["candidates"] if c["mode"] == "batched"}) == [4, 8, 16, 32]
     assert all(np.array_equal(c.solution(0), x) for c, x in zip(t_ctx, x0))
     assert all(et.stats(i)["steps"] == 0 for i in range(4))
+    tb = et.tune(probs[0].dt, modes=ensemble.TUNE_BATCHED)
+    assert tb["mode"] == "batched" and tb["replay"] in (4, 8, 16, 32)
+    assert all((c["ms"] > 0) == (c["mode"] == "batched") for c in tb["candidates"])
+    assert all(np.array_equal(c.solution(0), x) for c, x in zip(t_ctx, x0))
     a = [ewg.GpuContext(P, prm) for P in probs]
     b = [ewg.GpuContext(P, prm) for P in probs]
     ea, eb = ensemble.Ensemble(a), ensemble.Ensemble(b)
     m = ensemble.BATCHED if mode == "batched" else ensemble.PERSISTENT
     ea.set_mode(m, 512)
     eb.set_mode(m, 512)
+    if mode == "batched":
+        # the replay depth changes only what is queued past convergence
+        ea.set_replay(2)
+        eb.set_replay(32)
     ea.run(probs[0].dt, 3)
     for _ in range(3):
         eb.run(probs[0].dt, 1)
