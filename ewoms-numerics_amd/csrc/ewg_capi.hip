@@ -93,7 +93,7 @@ void free_all(ewg_ctx *c) {
                     c->pe, c->region,
                     c->regs, c->bc_type, c->bc_val, c->bc_aux, c->L64, c->D64, c->L32, c->D32, c->U32,
                     c->w32, c->kr, c->kp, c->kv, c->ky, c->kz, c->kx, c->part0, c->part1,
-                    c->part2, c->dscalar, c->ilu_err, c->ks, c->ns, c->pref, c->partAny,
+                    c->part2, c->dscalar, c->ilu_err, c->ks, c->ns, c->pref, c->fgeo, c->partAny,
                     c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wl64.L, c->wl64.U,
                     c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w,
                     c->wl_af};
@@ -146,6 +146,7 @@ LinArgs lin_args(ewg_ctx *c, double dt, int newton_iter) {
     a.fd = c->prm.local_linearizer;
     a.fd_method = c->prm.numeric_difference_method;
     a.pref = c->pref;
+    a.fgeo = c->fgeo_const ? c->fgeo : nullptr;
     return a;
 }
 
@@ -729,6 +730,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
     ALLOC(c->ks, 1);
     ALLOC(c->ns, 1);
     ALLOC(c->pref, 1);
+    ALLOC(c->fgeo, 1);
     ALLOC(c->partAny, kRedBlocks);
     // the tile-wavefront buffers (3D; every schedule but the hyperplane launches)
     if (g.dim == 3 && N <= kWfMaxCells && params->ilu_schedule != 1) {
@@ -831,6 +833,23 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
         free_all(c);
         delete c;
         return EWG_DEVICE_ERROR;
+    }
+    // the interior faces' geometry once per direction when every spacing is a power of two
+    // (every position, difference and square the flux evaluates is then exact, the same for
+    // every face of a direction, and the per-face evaluation gives these bits)
+    {
+        bool pow2 = true;
+        for (int d = 0; d < g.dim; ++d) {
+            int ex = 0;
+            const double m = std::frexp(g.h[d], &ex);
+            const double n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+            pow2 = pow2 && m == 0.5 && (n + 1.0) * g.h[d] < 0x1p40;
+        }
+        if (pow2) {
+            launch_face_geo(lin_args(c, 1.0, 0), c->fgeo, c->stream);
+            if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail("face geometry", e);
+            c->fgeo_const = true;
+        }
     }
     *out = c;
     return EWG_OK;

@@ -75,6 +75,8 @@ struct ewg_ctx {
     double *copy_src = nullptr, *copy_dst = nullptr;  // the bandwidth-reference copy's buffers
     KrylovState *ks = nullptr;
     ewg::NewtonState *ns = nullptr;   // the device Newton loop's state
+    ewg::FaceGeoSet *fgeo = nullptr;  // per-direction interior face geometry
+    bool fgeo_const = false;          // fgeo holds it (every spacing a power of two)
     double *pref = nullptr;           // reference pressure of the finite-difference weights
     // pinned host mirrors
     KrylovState *ks_host = nullptr;

@@ -38,6 +38,21 @@ struct Grid {
 
 struct NewtonState;
 
+// Geometry of the interior faces of one direction as the flux evaluates it per face: the
+// axis component of (x_ex - x_in) / |x_ex - x_in|^2, and, per phase, the hydrostatic term
+// added to the gradient (as the dual number the flux adds). When every grid spacing is a
+// power of two every position, difference and square of the evaluation is exact, so the
+// values are the same for every face of the direction and are evaluated once.
+template <class S>
+struct FaceGeo {
+    S gscale;
+    Dual<S> cterm[2];
+};
+struct FaceGeoSet {
+    FaceGeo<double> d[6];
+    FaceGeo<float> f[6];
+};
+
 struct LinArgs {
     Grid g;
     double grav[3];
@@ -67,6 +82,7 @@ struct LinArgs {
     int fd;               // 1: finite-difference local linearizer (FP64)
     int fd_method;        // its NumericDifferenceMethod: +1 forward, 0 central, -1 backward
     const double *pref;   // its reference pressure (the pressure's weight is 10 / pref)
+    const FaceGeoSet *fgeo; // per-direction face geometry (nullptr: evaluated per face)
     const NewtonState *ns;  // device Newton state (nullptr: dt and newton_iter above): the
                             // kernels return unless ns->active, and read dt, newton_iter
                             // from it
@@ -110,6 +126,7 @@ constexpr int kRedThreads = 256;
 // linearization
 void launch_linearize(const LinArgs &a, hipStream_t s);
 void launch_storage(const LinArgs &a, const double *x, double *M, hipStream_t s);
+void launch_face_geo(const LinArgs &a, FaceGeoSet *out, hipStream_t s);  // one-time face geometry
 void launch_bc_aux(const LinArgs &a, double *aux, hipStream_t s);  // per-face boundary values
 
 // ILU(0). gate (optional): a device flag; the launches return at once while it is 0

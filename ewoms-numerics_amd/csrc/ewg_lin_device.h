@@ -169,9 +169,7 @@ __device__ __forceinline__ void storage(const LinArgs &a, double phi, T Sw, T M[
 // values are therefore the reference's bit for bit (dropping "+ (+-0)" terms can change
 // only the sign of a zero result, which no comparison or nonzero value depends on).
 template <int DIM, class S>
-__device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in,
-                                              const IQ<double> &ex, double Kin, double Kex,
-                                              const int ijk[3], int dir, Ev F[2]) {
+__device__ __forceinline__ FaceGeo<S> face_geo(const LinArgs &a, const int ijk[3], int dir) {
     using D = Dual<S>;
     const Grid &g = a.g;
     constexpr int dim = DIM;
@@ -182,8 +180,6 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
     centre(g, ijk, posIn);
     centre(g, q, posEx);
     face_centre(g, ijk, dir, posFace);
-    const S normal = (dir & 1) ? S(1) : S(-1);  // the axis component of the unit normal
-
     double distSq = 0.0;
     for (int d = 0; d < dim; ++d) {
         double t = posEx[d] - posIn[d];
@@ -198,16 +194,40 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
     double absDistTotSq = 0.0;
     for (int d = 0; d < dim; ++d) absDistTotSq += dvTot[d] * dvTot[d];
     double gIn = dot_g<DIM>(a, dvIn), gEx = dot_g<DIM>(a, dvEx);
-    const S Kf = static_cast<S>(harmonic_mean(Kin, Kex));
-    const S gscale = static_cast<S>(div_p2(posEx[ax] - posIn[ax], distSq));
+    FaceGeo<S> fg;
+    fg.gscale = static_cast<S>(div_p2(posEx[ax] - posIn[ax], distSq));
     for (int k = 0; k < 2; ++k) {
-        // gradient along the axis: two-point difference plus the hydrostatic correction
-        const D deltay = narrow<S>(ev(ex.p[k]) - in.p[k]);
-        D grad = deltay * gscale;
         const D pStatIn = -cst<S>(a.rho[k]) * static_cast<S>(gIn);
         const D pStatEx = cst<S>(-a.rho[k] * gEx);
         const D qq = div_p2(pStatEx - pStatIn, static_cast<S>(absDistTotSq));
-        grad = grad + cst<S>(dvTot[ax]) * qq;
+        fg.cterm[k] = cst<S>(dvTot[ax]) * qq;
+    }
+    return fg;
+}
+
+template <class S>
+__device__ __forceinline__ const FaceGeo<S> &face_geo_of(const FaceGeoSet *fs, int dir);
+template <>
+__device__ __forceinline__ const FaceGeo<double> &face_geo_of<double>(const FaceGeoSet *fs, int dir) {
+    return fs->d[dir];
+}
+template <>
+__device__ __forceinline__ const FaceGeo<float> &face_geo_of<float>(const FaceGeoSet *fs, int dir) {
+    return fs->f[dir];
+}
+
+template <int DIM, class S>
+__device__ __forceinline__ void interior_flux_geo(const LinArgs &a, const IQ<Ev> &in,
+                                                  const IQ<double> &ex, double Kin, double Kex,
+                                                  const FaceGeo<S> &fg, int dir, Ev F[2]) {
+    using D = Dual<S>;
+    const S normal = (dir & 1) ? S(1) : S(-1);  // the axis component of the unit normal
+    const S Kf = static_cast<S>(harmonic_mean(Kin, Kex));
+    for (int k = 0; k < 2; ++k) {
+        // gradient along the axis: two-point difference plus the hydrostatic correction
+        const D deltay = narrow<S>(ev(ex.p[k]) - in.p[k]);
+        D grad = deltay * fg.gscale;
+        grad = grad + fg.cterm[k];
         // upstream mobility by the sign of grad . n, Darcy velocity -lambda K grad
         const bool upIsEx = grad.v * normal > S(0);
         const D lam = upIsEx ? cst<S>(ex.lam[k]) : narrow<S>(in.lam[k]);
@@ -216,6 +236,25 @@ __device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in
         const D vol = v * normal;
         F[k] = widen(cst<S>(0.0) + vol * cst<S>(a.rho[k]));
     }
+}
+
+// Interior face flux from the focus side: fvbasegradientcalculator.hh:195-240,
+// darcyfluxmodule.hh:185-324, :461-527, immisciblelocalresidual.hh:123-149.
+//
+// The reference evaluates the gradient, the Darcy velocity K grad and the normal flux as
+// dim-vectors with a permeability tensor. On this structured Cartesian grid the face
+// normal and the centre-to-centre vector lie on the face's axis and the permeability is
+// K I, so every other component is a sum of products with an exact zero: only the axis
+// component is evaluated here, with the same operations in the same order. Its nonzero
+// values are therefore the reference's bit for bit (dropping "+ (+-0)" terms can change
+// only the sign of a zero result, which no comparison or nonzero value depends on). The
+// geometry (face_geo) and the phases' fluxes (interior_flux_geo) are split so that the
+// geometry can be evaluated once per direction where it is the same for every face.
+template <int DIM, class S>
+__device__ __forceinline__ void interior_flux(const LinArgs &a, const IQ<Ev> &in,
+                                              const IQ<double> &ex, double Kin, double Kex,
+                                              const int ijk[3], int dir, Ev F[2]) {
+    interior_flux_geo<DIM, S>(a, in, ex, Kin, Kex, face_geo<DIM, S>(a, ijk, dir), dir, F);
 }
 
 // Dirichlet free-flow boundary: darcyfluxmodule.hh:332-453, :488-509,
@@ -346,27 +385,16 @@ __device__ __forceinline__ IQ<double> iq_values(const LinArgs &a, int cell) {
     return q;
 }
 
-// dt and newton_iter are the time-step size and Newton iteration of this linearization
-// (a.dt and a.newton_iter unless a device Newton state supplies them)
-template <int DIM, class S>
-__device__ __forceinline__ void linearize_cell(const LinArgs &a, int i, double dt,
-                                               int newton_iter) {
+// The element context of cell i as focus DOF: `in` holds the cell's intensive quantities
+// as dual numbers in its (p_w, S_w), nbf(dir, nb, K) returns the values of the neighbour nb
+// across direction dir (and sets K to its permeability). dt and newton_iter are the
+// time-step size and Newton iteration of this linearization (a.dt and a.newton_iter
+// unless a device Newton state supplies them).
+template <int DIM, class S, class NbFn, bool CONST_GEO = false>
+__device__ __forceinline__ void linearize_context(const LinArgs &a, int i, const int ijk[3],
+                                                  double Ki, const IQ<Ev> &in, NbFn nbf,
+                                                  double dt, int newton_iter) {
     const Grid &g = a.g;
-    int ijk[3];
-    cell_coords(g, i, ijk);
-    const double2 xi = reinterpret_cast<const double2 *>(a.x)[i];
-    const double Ki = a.K[i];
-    IQ<Ev> in;
-    {
-        const double4 v = reinterpret_cast<const double4 *>(a.iqv)[i];
-        const double2 *d = reinterpret_cast<const double2 *>(a.iqd + (size_t)i * 6);
-        const double2 d0 = d[0], d1 = d[1], d2 = d[2];
-        in.p[0] = Ev{xi.x, 1.0, 0.0};
-        in.Sw = Ev{xi.y, 0.0, 1.0};
-        in.p[1] = Ev{v.y, d0.x, d0.y};
-        in.lam[0] = Ev{v.z, d1.x, d1.y};
-        in.lam[1] = Ev{v.w, d2.x, d2.y};
-    }
     Ev res0[2] = {ev(0.0), ev(0.0)};
     constexpr int nfaces = 2 * DIM;
     // fluxes over the interior faces, intersection order (fvbaselocalresidual.hh:295-367)
@@ -374,9 +402,13 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i, double d
     for (int dir = 0; dir < nfaces; ++dir) {
         if (!has_neighbour(g, ijk, dir)) continue;
         int nb = i + ((dir & 1) ? 1 : -1) * stride_of(g, dir >> 1);
-        const IQ<double> ex = iq_values(a, nb);
+        double Kex;
+        const IQ<double> ex = nbf(dir, nb, Kex);
         Ev F[2];
-        interior_flux<DIM, S>(a, in, ex, Ki, a.K[nb], ijk, dir, F);
+        if constexpr (CONST_GEO)
+            interior_flux_geo<DIM, S>(a, in, ex, Ki, Kex, face_geo_of<S>(a.fgeo, dir), dir, F);
+        else
+            interior_flux<DIM, S>(a, in, ex, Ki, Kex, ijk, dir, F);
         double alpha = 1.0;
         alpha *= g.area[dir >> 1];
         Ev rj[2];
@@ -448,7 +480,32 @@ __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i, double d
     store_block(a.J, g.diag, g.N, i, res0);
 }
 
-
+// the element context of cell i with its own and its neighbours' intensive quantities
+// from the first pass (iq_cell)
+template <int DIM, class S>
+__device__ __forceinline__ void linearize_cell(const LinArgs &a, int i, double dt,
+                                               int newton_iter) {
+    const Grid &g = a.g;
+    int ijk[3];
+    cell_coords(g, i, ijk);
+    const double2 xi = reinterpret_cast<const double2 *>(a.x)[i];
+    IQ<Ev> in;
+    {
+        const double4 v = reinterpret_cast<const double4 *>(a.iqv)[i];
+        const double2 *d = reinterpret_cast<const double2 *>(a.iqd + (size_t)i * 6);
+        const double2 d0 = d[0], d1 = d[1], d2 = d[2];
+        in.p[0] = Ev{xi.x, 1.0, 0.0};
+        in.Sw = Ev{xi.y, 0.0, 1.0};
+        in.p[1] = Ev{v.y, d0.x, d0.y};
+        in.lam[0] = Ev{v.z, d1.x, d1.y};
+        in.lam[1] = Ev{v.w, d2.x, d2.y};
+    }
+    auto nbf = [&a](int, int nb, double &K) {
+        K = a.K[nb];
+        return iq_values(a, nb);
+    };
+    linearize_context<DIM, S>(a, i, ijk, a.K[i], in, nbf, dt, newton_iter);
+}
 
 template <int DIM, class S>
 __device__ __forceinline__ void linearize_cell(const LinArgs &a, int i) {

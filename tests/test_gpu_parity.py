@@ -51,6 +51,33 @@ def test_linearize_parity(case, newton_iter):
     assert np.array_equal(rp_g, rp_o) and np.array_equal(ci_g, ci_o)
 
 
+@pytest.mark.parametrize("shape", [(20, 13, 7), (37, 21, 5), (33, 17, 9), (48, 40, 19)])
+@pytest.mark.parametrize("h", [1.0, 0.75])
+def test_fused_3d_linearization_is_bitwise(shape, h):
+    """The fused 3D pass (16x16 column tiles marching in z in 6-plane chunks, intensive
+    quantities in LDS) gives the oracle's residual and Jacobian bit for bit on shapes with
+    partial tiles and partial chunks, at Newton iterations 0 and 1: with a power-of-two
+    spacing (the faces' geometry evaluated once per direction) and with another one (the
+    geometry evaluated per face)."""
+    import ewg
+    from ewg import problems
+    P = problems.heterogeneous_3d(*shape, seed=sum(shape), patch=4, h=h)
+    gpu, ora = _pair(P)
+    x = problems.perturbed_state(P, seed=3, dS=0.2)
+    x1 = problems.perturbed_state(P, seed=4, dS=0.1)
+    for c in (gpu, ora):
+        c.set_solution(x1, 1)
+        c.set_solution(x, 0)
+    for it in (0, 1):
+        rg, jg = gpu.linearize(P.dt, it)
+        ro, jo = ora.linearize(P.dt, it)
+        print(f"{shape} h {h} newton_iter {it}: residual bitwise {np.array_equal(rg, ro)}, "
+              f"Jacobian bitwise {np.array_equal(jg, jo)}", flush=True)
+        assert np.array_equal(rg, ro) and np.array_equal(jg, jo)
+    gpu.close()
+    ora.close()
+
+
 def test_linear_solve_parity():
     from ewg import problems
     P = problems.lens_problem()
