@@ -110,6 +110,7 @@ FORMS = {
     "fp32_flux": {"linearize_fp32": 1},
     "mixed": {"precond_fp32": 1, "linearize_fp32": 1},
     "learned_mixed": {"precond_fp32": 1, "linearize_fp32": 1, "warm_start": 1},
+    "learned": {"precond_fp32": 1, "warm_start": 1},
 }
 FORM_DETAIL = {
     "fp64": "FP64 throughout",
@@ -120,12 +121,16 @@ FORM_DETAIL = {
     "learned_mixed": "mixed + Newton initial guess from the BF16 MFMA network (saturation "
                      "update), judged by the unchanged Newton test with fallback; FP64 state, "
                      "residual, Newton norms, Krylov recurrences and update",
+    "learned": "FP32 ILU(0) factors and sweeps inside FP64 BiCGStab + Newton initial guess "
+               "from the BF16 MFMA network (saturation update), judged by the unchanged Newton "
+               "test with fallback; FP64 state, fluxes, residual, Jacobian, Newton norms, "
+               "Krylov recurrences and update",
 }
 # the arithmetic each form computes in: FP64 state, residual, Newton norms, Krylov
 # recurrences and update everywhere; FP32 ILU(0) factors and sweeps, FP32 face fluxes and
 # the BF16 network (FP32 accumulation) where the form uses them
 DTYPE = {"fp64": "f64", "fp32_precond": "f64+f32", "fp32_flux": "f64+f32",
-         "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16"}
+         "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16", "learned": "f64+f32+bf16"}
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 CNN_FLOP_PER_CELL = 2 * 88992  # multiply-adds of the five 3x3x3 convolutions, x2
 

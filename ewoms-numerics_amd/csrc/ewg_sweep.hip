@@ -193,9 +193,14 @@ __device__ __forceinline__ void lds_st(int *p, int v) {
 template <class T, bool FWD, int PD>
 __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     constexpr int TX = 8, TY = 8, NT = 64;
-    static_assert(PD < kRing, "the edge wave must be able to run ahead of the compute wave");
-    __shared__ P2<T> xch[2][NT];
-    __shared__ P2<T> ering[2][8][kRing];  // [x edge | y edge][edge lane][z % kRing]
+    // the compute loop is unrolled over kRing steps, and the neighbour values a lane uses at
+    // a step are kept in slot (step mod kRing) of its own column of xv (-x neighbour) and
+    // yv (-y neighbour): every read is one LDS load at a fixed offset. The lanes of the
+    // tile write the next step's slot of their +x / +y neighbour (the last column and row
+    // write into spare columns), the edge wave the slots of the first column / row.
+    static_assert(kRing % PD == 0 && PD < kRing, "prefetch depth must divide the ring");
+    __shared__ P2<T> xv[kRing][NT + 1];
+    __shared__ P2<T> yv[kRing][NT + TX];
     __shared__ int filled[16];            // values of each edge lane in place (z < filled)
     __shared__ int progress;              // compute steps completed (per PD steps)
     __shared__ unsigned s_tk;
@@ -209,10 +214,10 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     const int tx = tile.x, ty = tile.y;
     const int nz = a.nz;
     const int tileId = ty * a.sync.ntx + tx;
-    // edge lanes without an upstream tile read zeros for good (the matching blocks are
-    // zero as well) and count as complete
-    for (int i = tid; i < 2 * 8 * kRing; i += 2 * NT) (&ering[0][0][0])[i] = P2<T>{T(0), T(0)};
-    for (int i = tid; i < 2 * NT; i += 2 * NT) (&xch[0][0])[i] = P2<T>{T(0), T(0)};
+    // slots without an upstream tile read zeros for good (the matching blocks are zero as
+    // well) and their edge lanes count as complete
+    for (int i = tid; i < kRing * (NT + 1); i += 2 * NT) (&xv[0][0])[i] = P2<T>{T(0), T(0)};
+    for (int i = tid; i < kRing * (NT + TX); i += 2 * NT) (&yv[0][0])[i] = P2<T>{T(0), T(0)};
     if (tid < 16) {
         const int dir = tid >> 3, e = tid & 7;
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + e < a.ny) : (ty > 0 && tx * TX + e < a.nx);
@@ -233,7 +238,10 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
             !valid ? 0u
             : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nz) * zw)
                        : (unsigned)((((size_t)(tileId - a.sync.ntx) * TX + idx) * nz) * zw);
-        P2<T> *ring = &ering[dir][idx][0];
+        // value z of this edge lane is used by lane (0, idx) (x edge) or (idx, 0) (y edge)
+        // at its step z + idx
+        P2<T> *const ring = dir == 0 ? &xv[0][TX * idx] : &yv[0][idx];
+        const int sstride = dir == 0 ? NT + 1 : NT + TX;  // elements from one slot to the next
         EW<T> sent;
         sent.w[0] = kSentinel;
         if constexpr (sizeof(T) == 8) sent.w[1] = kSentinel;
@@ -244,8 +252,8 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
         int published = valid ? 0 : nz;
         while (__builtin_amdgcn_ballot_w64(z < nz) != 0ull) {
             const bool mine = z < nz;
-            // the slot held z - kRing, consumed once the compute wave has completed the
-            // step that read it (step z - kRing + idx)
+            // the slot held the value of step z + idx - kRing, consumed once the compute
+            // wave has completed that step
             const bool room = z - kRing + idx < lds_ld(&progress);
             const unsigned go = mine ? gbase + (unsigned)z * zw : kFar;
             const EW<T> vx = lde<T>(rex, dir == 0 ? go : kFar);
@@ -253,7 +261,7 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
             const EW<T> v = dir == 0 ? vx : vy;
             const bool deliver = mine && room && !is_sentinel(v);
             if (deliver) {
-                ring[z & (kRing - 1)] = ew_val(v);
+                ring[((z + idx) & (kRing - 1)) * sstride] = ew_val(v);
                 ste<T>(rex, dir == 0 ? go : kFar, sent);
                 ste<T>(rey, dir == 1 ? go : kFar, sent);
                 z += 4;
@@ -362,15 +370,16 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     }
 
     P2<T> prev = P2<T>{T(0), T(0)};
-    const bool hasL = lx > 0, hasD = ly > 0;
     // this tile's words in the edge buffers: its last column (edge lane ly) and last row
     // (edge lane lx), z-value words consecutive
     const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nz) * zw);
     const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nz) * zw);
-    // neighbour sources: the in-tile lane of the previous step, or the ring slot of this
-    // lane's z (first column / row of the tile)
-    const P2<T> *const eX = &ering[0][ly][0];
-    const P2<T> *const eY = &ering[1][lx][0];
+    // this lane's exchange columns, and the columns it writes (its +x / +y neighbour's, or
+    // a spare column)
+    const P2<T> *const rx = &xv[0][tid];
+    const P2<T> *const ry = &yv[0][tid];
+    P2<T> *const wx = lx < TX - 1 ? &xv[0][tid + 1] : &xv[0][NT];
+    P2<T> *const wy = ly < TY - 1 ? &yv[0][tid + TX] : &yv[0][NT + lx];
     const int *const fX = &filled[ly];
     const int *const fY = &filled[8 + lx];
     const bool edgeX = lx == 0, edgeY = ly == 0;
@@ -379,30 +388,23 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     // before, so that their LDS latency overlaps that step's tail and the check; the
     // counts come first (a ring value read after a count that covers it is in place),
     // and the rare wait re-reads the values.
-    auto nbr = [&](int tau, P2<T> &vx, P2<T> &vy) {
-        const int z = tau - lx - ly;
-        const int rbuf = (tau + 1) & 1;
-        const int slot = z & (kRing - 1);
-        vx = hasL ? xch[rbuf][tid - 1] : eX[slot];
-        vy = hasD ? xch[rbuf][tid - TX] : eY[slot];
-    };
     int cntX = lds_ld(fX), cntY = lds_ld(fY);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    P2<T> nx_, ny_;
-    nbr(0, nx_, ny_);
-    for (int tau0 = 0; tau0 < nsteps; tau0 += PD) {
-        // steps before tau0 are complete: their ring slots may be refilled (the compiler
-        // fence keeps the earlier steps' ring reads ahead of the announcement; LDS
-        // executes them in that order)
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (tid == 0) lds_st(&progress, tau0);
+    P2<T> nx_ = rx[0], ny_ = ry[0];
+    for (int tau0 = 0; tau0 < nsteps; tau0 += kRing) {
 #pragma unroll
-        for (int k = 0; k < PD; ++k) {
+        for (int k = 0; k < kRing; ++k) {
             const int tau = tau0 + k;
+            if (k % PD == 0) {
+                // steps before tau are complete: their slots may be refilled (the
+                // compiler fence keeps the earlier steps' reads ahead of the
+                // announcement; LDS executes them in that order)
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (tid == 0) lds_st(&progress, tau);
+            }
             const int z = tau - lx - ly;
             const bool zin = (unsigned)z < (unsigned)nz;
-            const int wbuf = tau & 1;
-            // this step's ring values must be in place: value z of the lane's edge
+            // this step's edge values must be in place: value z of the lane's edge
             bool wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
             if (__builtin_amdgcn_ballot_w64(wait) != 0ull) {
                 unsigned spins = 0;
@@ -418,12 +420,13 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 } while (__builtin_amdgcn_ballot_w64(wait) != 0ull && spins < kSpinLimit);
                 if (wait) atomicExch(a.sync.err, 1);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                nbr(tau, nx_, ny_);
+                nx_ = rx[k * (NT + 1)];
+                ny_ = ry[k * (NT + TX)];
             }
             P2<T> nz_;
             nz_.x = z > 0 ? prev.x : T(0);
             nz_.y = z > 0 ? prev.y : T(0);
-            const In<T, FWD> &c = ring[k];
+            const In<T, FWD> &c = ring[k % PD];
             T r0 = c.rhs.x, r1 = c.rhs.y;
             P2<T> res;
             if (FWD) {
@@ -453,15 +456,18 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 onat += snat;
             }
             prev = res;
-            xch[wbuf][tid] = res;
+            const int kn = (k + 1) & (kRing - 1);
+            wx[kn * (NT + 1)] = res;
+            wy[kn * (NT + TX)] = res;
             __builtin_amdgcn_wave_barrier();
             // counts and neighbour values for the next step
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             cntX = lds_ld(fX);
             cntY = lds_ld(fY);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            nbr(tau + 1, nx_, ny_);
-            load(ring[k]);
+            nx_ = rx[kn * (NT + 1)];
+            ny_ = ry[kn * (NT + TX)];
+            load(ring[k % PD]);
             __builtin_amdgcn_sched_barrier(0);
         }
     }

@@ -14,7 +14,7 @@ constexpr int kWfTX = 8;   // tile extent in x (lanes)
 constexpr int kWfTY = 8;   // tile extent in y
 constexpr int kFactorPD = 3;    // prefetch depth of the wavefront-layout factorization
 #ifndef EWG_SWEEP_PD
-#define EWG_SWEEP_PD 6
+#define EWG_SWEEP_PD 8
 #endif
 constexpr int kSweepPD = EWG_SWEEP_PD;  // prefetch depth of the triangular sweeps' operands
 constexpr int kSweepPE = 1;     // prefetch depth of their cross-tile words
