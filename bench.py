@@ -13,7 +13,10 @@ Forms (each timed the same way):
   learned_mixed  mixed, with each Newton solve started from the warm-start network's
                  prediction (BF16 matrix-core convolutions); the network is trained
                  before, outside every timed region, on pairs harvested from other
-                 realizations.
+                 realizations;
+  mixed_op, learned_mixed_op (not run by default)
+                 the same with BiCGStab's operator products reading the Jacobian's
+                 blocks rounded to FP32.
 `value` is the metric's form, learned_mixed, when it runs (else fp64); the plain FP64
 value and the ratio are reported beside it. For each reduced-precision form a second,
 untimed pass steps it beside the FP64 path and reports the pressure and saturation
@@ -51,12 +54,13 @@ HBM_PEAK_GBS = 8000.0
 EWG_COPY_BYTES = 1 << 30   # include/ewg_capi.h: the copy probe moves this many bytes each way
 
 
-def algorithmic_bytes(P, nnzb, fp32_precond=False):
+def algorithmic_bytes(P, nnzb, fp32_precond=False, fp32_operator=False):
     """Bytes each kernel must move at least once per launch (FP64 values unless stated,
     implicit structured column indices; intermediate buffers of an implementation do
-    not count)."""
+    not count). fp32_operator: BiCGStab's SpMV reads the FP32 copy of the blocks."""
     N = P.N
     blk = 32
+    oblk = 16 if fp32_operator else 32  # operator blocks read by the SpMV
     fblk = 16 if fp32_precond else 32  # preconditioner factor blocks
     fval = 4 if fp32_precond else 8    # the forward sweep's intermediate vector
     lower = (nnzb - N) // 2
@@ -66,7 +70,7 @@ def algorithmic_bytes(P, nnzb, fp32_precond=False):
         # x, K, phi, storage cache, [pe], [region] in; residual + all Jacobian blocks out
         "linearize": N * (per_cell_in + 16) + nnzb * blk,
         # all blocks + x in, y out
-        "spmv": nnzb * blk + 2 * 16 * N,
+        "spmv": nnzb * oblk + 2 * 16 * N,
         # forward: L + d in, w out; backward: U + Dinv + w in, v out
         "ilu_apply": 2 * lower * fblk + N * fblk + (16 + 2 * fval + 2 * fval + 16) * N,
         # A (lower, diagonal, upper) in; L + Dinv out (FP32: and the U copy)
@@ -111,6 +115,9 @@ FORMS = {
     "mixed": {"precond_fp32": 1, "linearize_fp32": 1},
     "learned_mixed": {"precond_fp32": 1, "linearize_fp32": 1, "warm_start": 1},
     "learned": {"precond_fp32": 1, "warm_start": 1},
+    "mixed_op": {"precond_fp32": 1, "linearize_fp32": 1, "operator_fp32": 1},
+    "learned_mixed_op": {"precond_fp32": 1, "linearize_fp32": 1, "operator_fp32": 1,
+                         "warm_start": 1},
 }
 FORM_DETAIL = {
     "fp64": "FP64 throughout",
@@ -125,12 +132,17 @@ FORM_DETAIL = {
                "from the BF16 MFMA network (saturation update), judged by the unchanged Newton "
                "test with fallback; FP64 state, fluxes, residual, Jacobian, Newton norms, "
                "Krylov recurrences and update",
+    "mixed_op": "mixed + BiCGStab's operator products read the Jacobian's blocks rounded to "
+                "FP32 (FP64 products and sums)",
+    "learned_mixed_op": "learned_mixed + BiCGStab's operator products read the Jacobian's "
+                        "blocks rounded to FP32 (FP64 products and sums)",
 }
 # the arithmetic each form computes in: FP64 state, residual, Newton norms, Krylov
 # recurrences and update everywhere; FP32 ILU(0) factors and sweeps, FP32 face fluxes and
 # the BF16 network (FP32 accumulation) where the form uses them
 DTYPE = {"fp64": "f64", "fp32_precond": "f64+f32", "fp32_flux": "f64+f32",
-         "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16", "learned": "f64+f32+bf16"}
+         "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16", "learned": "f64+f32+bf16",
+         "mixed_op": "f64+f32", "learned_mixed_op": "f64+f32+bf16"}
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 CNN_FLOP_PER_CELL = 2 * 88992  # multiply-adds of the five 3x3x3 convolutions, x2
 
@@ -352,7 +364,7 @@ def main():
         # stream (the first apply and SpMV of each batch of BiCGStab iterations, every
         # factorization and linearization); mean duration and algorithmic bandwidth.
         # Read before the sample run below, which resets them.
-        ab = algorithmic_bytes(P, ctx.nnzb, fp32)
+        ab = algorithmic_bytes(P, ctx.nnzb, fp32, bool(FORMS[form].get("operator_fp32")))
         ks = {}
         for name, kid in (("linearize", EWG_KERNEL_LINEARIZE), ("spmv", EWG_KERNEL_SPMV),
                           ("ilu_apply", EWG_KERNEL_ILU_APPLY),
@@ -592,7 +604,8 @@ def kernel_mode(args):
     ctx.time_kernel(kid, 1)
     ms = ctx.time_kernel(kid, args.reps)
     nbytes = (2 * EWG_COPY_BYTES if args.kernel == "copy" else
-              algorithmic_bytes(P, ctx.nnzb, bool(FORMS[form].get("precond_fp32")))[args.kernel])
+              algorithmic_bytes(P, ctx.nnzb, bool(FORMS[form].get("precond_fp32")),
+                                bool(FORMS[form].get("operator_fp32")))[args.kernel])
     print(json.dumps({"kernel": args.kernel, "form": form, "reps": args.reps, "cold": args.cold,
                       "mean_ms": ms,
                       "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9}), flush=True)

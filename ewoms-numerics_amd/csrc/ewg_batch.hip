@@ -1330,8 +1330,9 @@ int ewg_ensemble_create(ewg_ctx *const *members, int32_t n, ewg_ensemble **out) 
         ewg_ctx *c = members[i];
         if (!c || !c->static_set) return EWG_INVALID_STATE;
         if (c->prm.precond_fp32 && !c->L32) return EWG_INVALID_STATE;
-        // the batched and persistent kernels linearize by automatic differentiation
-        if (c->prm.local_linearizer != 0) return EWG_INVALID_ARGUMENT;
+        // the batched and persistent kernels linearize by automatic differentiation, and
+        // their operator products read the FP64 Jacobian
+        if (c->prm.local_linearizer != 0 || c->prm.operator_fp32 != 0) return EWG_INVALID_ARGUMENT;
         Member &m = host[i];
         m.lin = ctx_lin_args(c, 1.0, 0);
         m.lin.recycle = c->prm.recycle_first_iteration_storage;

@@ -96,7 +96,7 @@ void free_all(ewg_ctx *c) {
                     c->part2, c->dscalar, c->ilu_err, c->ks, c->ns, c->pref, c->fgeo, c->partAny,
                     c->wf_ticket, c->wf_ex, c->wf_ey, c->wf_order, c->wl64.L, c->wl64.U,
                     c->wl64.D, c->wl64.w, c->wl32.L, c->wl32.U, c->wl32.D, c->wl32.w,
-                    c->wl_af};
+                    c->wl_af, c->J32};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->copy_src) (void)hipFree(c->copy_src);
@@ -205,6 +205,7 @@ bool use_wavefront_layout(const ewg_ctx *c) {
 const char *check_params(const ewg_params *p) {
     if (p->ilu_schedule < 0 || p->ilu_schedule > 1) return "ilu_schedule must be 0 or 1";
     if (p->linearize_fp32 != 0 && p->linearize_fp32 != 1) return "linearize_fp32 must be 0 or 1";
+    if (p->operator_fp32 != 0 && p->operator_fp32 != 1) return "operator_fp32 must be 0 or 1";
     if (p->local_linearizer != 0 && p->local_linearizer != 1)
         return "local_linearizer must be 0 (automatic differentiation) or 1 (finite differences)";
     if (p->numeric_difference_method < -1 || p->numeric_difference_method > 1)
@@ -220,19 +221,33 @@ int capture_reference_pressure(ewg_ctx *c) {
     return EWG_OK;
 }
 
+// With params.operator_fp32 the solve's FP32 copy of the Jacobian is made here too: by the
+// wavefront layout's operand packing, which reads every block anyway, or by a rounding pass.
 void precond_factor(ewg_ctx *c, const int *gate = nullptr) {
     const bool fp32 = c->prm.precond_fp32;
+    float *op32 = c->prm.operator_fp32 ? c->J32 : nullptr;
     if (use_wavefront_layout(c)) {
         // straight into the sweeps' layout (no natural-order factors)
         if (fp32)
-            wl_factor32(c->g, c->wf, c->J, c->wl_af, c->wl32, c->ilu_err, c->stream, gate);
+            wl_factor32(c->g, c->wf, c->J, c->wl_af, c->wl32, c->ilu_err, c->stream, gate, op32);
         else
-            wl_factor64(c->g, c->wf, c->J, c->wl_af, c->wl64, c->ilu_err, c->stream, gate);
+            wl_factor64(c->g, c->wf, c->J, c->wl_af, c->wl64, c->ilu_err, c->stream, gate, op32);
         return;
     }
     launch_ilu_factor64(c->g, c->J, c->L64, c->D64, c->ilu_err, c->stream, gate);
     if (fp32)
         launch_round_factors(c->g, c->L64, c->D64, c->J, c->L32, c->D32, c->U32, c->stream, gate);
+    if (op32) launch_round_operator(c->g, c->J, op32, c->stream, gate);
+}
+
+// BiCGStab's operator product y = J x (fused dots as launch_spmv): the FP64 Jacobian, or
+// its FP32 copy of the current solve with params.operator_fp32
+void operator_apply(ewg_ctx *c, const double *x, double *y, const double *a, double *partA,
+                    const double *b, double *partB, const int *status) {
+    if (c->prm.operator_fp32)
+        launch_spmv32(c->g, c->J32, x, y, a, partA, b, partB, status, c->stream);
+    else
+        launch_spmv(c->g, c->J, x, y, a, partA, b, partB, status, c->stream);
 }
 
 // tickets_clear: a kernel queued on the stream since the last apply has cleared the
@@ -268,13 +283,13 @@ void bicg_iteration(ewg_ctx *c, const double *b, double *x, unsigned *tk, bool s
         sample_end(c, EWG_KERNEL_ILU_APPLY);
         sample_begin(c, EWG_KERNEL_SPMV);
     }
-    launch_spmv(c->g, c->J, c->ky, c->kv, b, c->part0, nullptr, nullptr, st, s);
+    operator_apply(c, c->ky, c->kv, b, c->part0, nullptr, nullptr, st);
     if (sampled) sample_end(c, EWG_KERNEL_SPMV);
     launch_ks_alpha(c->ks, c->part0, s);
     launch_hs_update(n, c->ks, c->ky, c->kv, x, c->kr, c->part1, c->partAny, s, tk);
     launch_ks_crit1(c->ks, c->part1, c->partAny, s);
     precond_apply(c, c->kr, c->kz, st, tk != nullptr);
-    launch_spmv(c->g, c->J, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st, s);
+    operator_apply(c, c->kz, c->ky, c->ky, c->part0, c->kr, c->part1, st);
     launch_ks_omega(c->ks, c->part0, c->part1, s);
     launch_xr_update(n, c->ks, c->kz, c->ky, x, c->kr, b, c->part2, c->partAny, s);
     launch_ks_crit2(c->ks, c->partAny, s);
@@ -394,6 +409,13 @@ int ensure_fp32(ewg_ctx *c) {
         HIP_TRY(c, hipMemset(c->wl32.D, 0, E * 4 * sizeof(float)));
         HIP_TRY(c, hipMemset(c->wl32.w, 0, E * 2 * sizeof(float)));
     }
+    return EWG_OK;
+}
+
+// FP32 copy of the Jacobian for BiCGStab's operator, allocated on first use
+int ensure_op32(ewg_ctx *c) {
+    if (c->J32) return EWG_OK;
+    HIP_TRY(c, dalloc(&c->J32, (size_t)c->g.nslots * c->g.N * 4));
     return EWG_OK;
 }
 
@@ -828,7 +850,8 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
         }
         c->rowptr[i + 1] = static_cast<int32_t>(c->colidx.size());
     }
-    if (c->prm.precond_fp32 && ensure_fp32(c) != EWG_OK) {
+    if ((c->prm.precond_fp32 && ensure_fp32(c) != EWG_OK) ||
+        (c->prm.operator_fp32 && ensure_op32(c) != EWG_OK)) {
         std::fprintf(stderr, "ewg_create: %s\n", c->err.c_str());
         free_all(c);
         delete c;
@@ -872,6 +895,10 @@ int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
                        "a tile-wavefront schedule needs a context created with one");
     if (p->precond_fp32) {
         int rc = ensure_fp32(c);
+        if (rc) return rc;
+    }
+    if (p->operator_fp32) {
+        int rc = ensure_op32(c);
         if (rc) return rc;
     }
     c->prm = *p;
@@ -1154,7 +1181,8 @@ int ewg_apply_operator(ewg_ctx *c, const double *x, double *y) {
     hipStream_t s = c->stream;
     const size_t bytes = 2 * (size_t)c->g.N * sizeof(double);
     HIP_TRY(c, hipMemcpyAsync(c->kp, x, bytes, hipMemcpyHostToDevice, s));
-    launch_spmv(c->g, c->J, c->kp, c->ky, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+    if (c->prm.operator_fp32) launch_round_operator(c->g, c->J, c->J32, s);
+    operator_apply(c, c->kp, c->ky, nullptr, nullptr, nullptr, nullptr, nullptr);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(s));
     HIP_TRY(c, hipMemcpy(y, c->ky, bytes, hipMemcpyDeviceToHost));
@@ -1274,6 +1302,8 @@ int ewg_time_kernel(ewg_ctx *c, int which, int reps, double *mean_ms) {
         HIP_TRY(c, hipMemsetAsync(c->ilu_err, 0, sizeof(int), s));
         precond_factor(c);
     }
+    if (which == EWG_KERNEL_SPMV && c->prm.operator_fp32)
+        launch_round_operator(c->g, c->J, c->J32, s);
     hipEvent_t e0, e1;
     HIP_TRY(c, hipEventCreate(&e0));
     HIP_TRY(c, hipEventCreate(&e1));
@@ -1292,7 +1322,7 @@ int ewg_time_kernel(ewg_ctx *c, int which, int reps, double *mean_ms) {
         switch (which) {
             case EWG_KERNEL_LINEARIZE: launch_linearize(a, s); break;
             case EWG_KERNEL_SPMV:
-                launch_spmv(c->g, c->J, c->x0, c->ky, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+                operator_apply(c, c->x0, c->ky, nullptr, nullptr, nullptr, nullptr, nullptr);
                 break;
             case EWG_KERNEL_ILU_APPLY: precond_apply(c, c->R, c->ky, nullptr, false); break;
             case EWG_KERNEL_ILU_FACTOR: precond_factor(c); break;

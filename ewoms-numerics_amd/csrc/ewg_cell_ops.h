@@ -166,13 +166,33 @@ __device__ __forceinline__ void ilu_bwd_cell(const Grid &g, int i, const int ijk
     }
 }
 
+// one 2x2 block of the slot-major Jacobian as two FP64 rows: stored in FP64 (two 16-byte
+// loads) or rounded to FP32 (one 16-byte load, widened exactly)
+__device__ __forceinline__ void load_row_blk(const double *A, size_t idx, double2 &r0,
+                                             double2 &r1) {
+    typedef double blk2 __attribute__((ext_vector_type(2)));
+    const blk2 *blk = reinterpret_cast<const blk2 *>(A + idx * 4);
+    const blk2 q0 = __builtin_nontemporal_load(blk), q1 = __builtin_nontemporal_load(blk + 1);
+    r0 = make_double2(q0.x, q0.y);
+    r1 = make_double2(q1.x, q1.y);
+}
+__device__ __forceinline__ void load_row_blk(const float *A, size_t idx, double2 &r0,
+                                             double2 &r1) {
+    typedef float blk4 __attribute__((ext_vector_type(4)));
+    const blk4 q = __builtin_nontemporal_load(reinterpret_cast<const blk4 *>(A + idx * 4));
+    r0 = make_double2(static_cast<double>(q.x), static_cast<double>(q.y));
+    r1 = make_double2(static_cast<double>(q.z), static_cast<double>(q.w));
+}
+
 // (A x)_i on the slot-major structured pattern. The Jacobian's blocks are read once per
 // product with non-temporal loads: streamed that way they do not displace the
 // preconditioner's factors from the last-level cache between the BiCGStab iterations'
-// preconditioner applies (the factors then come from that cache instead of HBM).
-__device__ __forceinline__ double2 spmv_cell(const Grid &g, int i, const double *A,
+// preconditioner applies (the factors then come from that cache instead of HBM). TB: the
+// type the blocks are stored in (the FP32 copy of the mixed-precision operator is widened
+// to FP64 on load; products and sums are FP64 either way).
+template <class TB>
+__device__ __forceinline__ double2 spmv_cell(const Grid &g, int i, const TB *A,
                                              const double *x) {
-    typedef double blk2 __attribute__((ext_vector_type(2)));
     int ijk[3];
     ijk[0] = i % g.nx;
     int r = i / g.nx;
@@ -193,9 +213,8 @@ __device__ __forceinline__ double2 spmv_cell(const Grid &g, int i, const double 
             if (ijk[ax] + 1 >= n[ax]) continue;
             off = (ax == 0 ? 1 : (ax == 1 ? g.nx : g.nxny));
         }
-        const blk2 *blk = reinterpret_cast<const blk2 *>(A + ((size_t)slot * g.N + i) * 4);
-        const blk2 q0 = __builtin_nontemporal_load(blk), q1 = __builtin_nontemporal_load(blk + 1);
-        const double2 r0 = make_double2(q0.x, q0.y), r1 = make_double2(q1.x, q1.y);
+        double2 r0, r1;
+        load_row_blk(A, (size_t)slot * g.N + i, r0, r1);
         double2 xv = reinterpret_cast<const double2 *>(x)[i + off];
         s0 += r0.x * xv.x;
         s0 += r0.y * xv.y;

@@ -49,6 +49,9 @@ struct ewg_ctx {
     // preconditioner
     double *L64 = nullptr, *D64 = nullptr;
     float *L32 = nullptr, *D32 = nullptr, *U32 = nullptr, *w32 = nullptr;
+    // the Jacobian's blocks rounded to FP32 for BiCGStab's operator products
+    // (params.operator_fp32), slot-major like J
+    float *J32 = nullptr;
     // Krylov vectors
     double *kr = nullptr, *kp = nullptr, *kv = nullptr, *ky = nullptr, *kz = nullptr,
            *kx = nullptr;

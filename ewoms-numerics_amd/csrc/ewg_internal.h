@@ -147,6 +147,13 @@ void launch_ilu_apply32(const Grid &g, const float *L, const float *Dinv, const 
 void launch_spmv(const Grid &g, const double *A, const double *x, double *y,
                  const double *a, double *partA, const double *b, double *partB,
                  const int *status, hipStream_t s);
+// the same with the blocks read from their FP32 copy (widened to FP64 on load)
+void launch_spmv32(const Grid &g, const float *A32, const double *x, double *y,
+                   const double *a, double *partA, const double *b, double *partB,
+                   const int *status, hipStream_t s);
+// A32 = every block of A rounded to FP32 (gate optional: nothing while *gate == 0)
+void launch_round_operator(const Grid &g, const double *A, float *A32, hipStream_t s,
+                           const int *gate = nullptr);
 
 // Krylov helpers (n = 2N doubles)
 // gate (optional): while *gate == 0 no solve starts (ks_init leaves the state KS_IDLE,

@@ -101,9 +101,11 @@ __device__ __forceinline__ double reduce_partials_max(const double *part) {
     return block_max(v);
 }
 
-// SpMV over the present slots in ascending column order (BCRSMatrix::mv semantics)
+// SpMV over the present slots in ascending column order (BCRSMatrix::mv semantics); TB:
+// the type the Jacobian's blocks are stored in
+template <class TB>
 __global__ void __launch_bounds__(256)
-k_spmv(Grid g, const double *A, const double *x, double *y, const double *a, double *partA,
+k_spmv(Grid g, const TB *A, const double *x, double *y, const double *a, double *partA,
        const double *b, double *partB, const int *status) {
     if (status && *status != KS_RUNNING) return;
     double accA = 0.0, accB = 0.0;
@@ -127,6 +129,19 @@ k_spmv(Grid g, const double *A, const double *x, double *y, const double *a, dou
     if (partB) {
         double t = block_sum(accB);
         if (threadIdx.x == 0) partB[blockIdx.x] = t;
+    }
+}
+
+// every 2x2 block of the Jacobian rounded to nearest FP32 (the operator of the
+// mixed-precision BiCGStab); gate: nothing while *gate == 0
+__global__ void __launch_bounds__(256)
+k_round_operator(size_t nblk, const double2 *A, float4 *A32, const int *gate) {
+    if (gate && *gate == 0) return;
+    for (size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x; b < nblk;
+         b += (size_t)gridDim.x * blockDim.x) {
+        const double2 u = A[2 * b], w = A[2 * b + 1];
+        A32[b] = make_float4(static_cast<float>(u.x), static_cast<float>(u.y),
+                             static_cast<float>(w.x), static_cast<float>(w.y));
     }
 }
 
@@ -536,8 +551,20 @@ void launch_ilu_apply32(const Grid &g, const float *L, const float *Dinv, const 
 void launch_spmv(const Grid &g, const double *A, const double *x, double *y, const double *a,
                  double *partA, const double *b, double *partB, const int *status,
                  hipStream_t s) {
-    hipLaunchKernelGGL(k_spmv, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, A, x, y, a, partA, b,
-                       partB, status);
+    hipLaunchKernelGGL(k_spmv<double>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, A, x, y, a,
+                       partA, b, partB, status);
+}
+void launch_spmv32(const Grid &g, const float *A32, const double *x, double *y, const double *a,
+                   double *partA, const double *b, double *partB, const int *status,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_spmv<float>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g, A32, x, y, a,
+                       partA, b, partB, status);
+}
+void launch_round_operator(const Grid &g, const double *A, float *A32, hipStream_t s,
+                           const int *gate) {
+    const size_t nblk = (size_t)g.nslots * g.N;
+    hipLaunchKernelGGL(k_round_operator, dim3(2048), dim3(256), 0, s, nblk,
+                       reinterpret_cast<const double2 *>(A), reinterpret_cast<float4 *>(A32), gate);
 }
 
 void launch_bicg_init(int n, const double *b, double *r, double *x, double *p, double *v,

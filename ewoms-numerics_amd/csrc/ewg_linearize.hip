@@ -115,8 +115,11 @@ __device__ __forceinline__ IQ<double> iq_of(const NbVal &v) {
 }
 
 // three workgroups' waves per SIMD (168 registers; more occupancy spills)
+#ifndef EWG_ZT_OCC
+#define EWG_ZT_OCC 3
+#endif
 template <class S, bool CONST_GEO>
-__global__ void __launch_bounds__(kZT * kZT, 3) k_linearize_zt(LinArgs a) {
+__global__ void __launch_bounds__(kZT * kZT, EWG_ZT_OCC) k_linearize_zt(LinArgs a) {
     double dt = a.dt;
     int newton_iter = a.newton_iter;
     if (a.ns) {

@@ -530,6 +530,8 @@ struct FwsArgs {
     size_t entries;
     int *err;              // singular pivot (|det| < 1e-30)
     const int *active;     // null, or skip the launch unless *active
+    float *A32;            // null, or the packing pass also stores every block of A rounded
+                           // to FP32 here, slot-major like A (BiCGStab's FP32 operator)
     WfSync sync;
 };
 
@@ -875,6 +877,19 @@ __device__ __forceinline__ void pack_factor_cell(const FwsArgs<T> &a, int c) {
     const int xb = a.nx - 1 - x, yb = a.ny - 1 - y, zb = a.nz - 1 - z;
     const int lxb = xb & 7, lyb = yb & 7;
     const size_t eb = ((size_t)((yb >> 3) * ntx + (xb >> 3)) * nsteps + zb + lxb + lyb) * 64 + lxb + 8 * lyb;
+    if (a.A32) {
+        // the cell's own seven blocks (diagonal, lower and upper slots) rounded to FP32
+        auto st32 = [&](int slot, const double2 *b) {
+            *reinterpret_cast<float4 *>(a.A32 + ((size_t)slot * N + c) * 4) =
+                make_float4(static_cast<float>(b[0].x), static_cast<float>(b[0].y),
+                            static_cast<float>(b[1].x), static_cast<float>(b[1].y));
+        };
+        st32(a.diag, v[0]);
+        for (int s = 0; s < 3; ++s) {
+            st32(s, v[1 + s]);
+            st32(a.diag + 1 + s, v[7 + s]);
+        }
+    }
     T *__restrict__ U = a.U;
     for (int s = 0; s < 3; ++s) {
         T *dst = U + s * E4 + eb * 4;
@@ -968,15 +983,18 @@ static FwsArgs<T> fws_args(const Grid &g, const WfSync &sync, const double *A, d
     f.entries = wl_entries(g, sync);
     f.err = err;
     f.active = nullptr;
+    f.A32 = nullptr;
     f.sync = sync;
     return f;
 }
 
 template <class T>
 static void wl_factor(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                      const WlPrecond<T> &wl, int *err, hipStream_t s, const int *gate) {
+                      const WlPrecond<T> &wl, int *err, hipStream_t s, const int *gate,
+                      float *A32) {
     FwsArgs<T> f = fws_args<T>(g, sync, A, Af, wl.L, wl.U, wl.D, err);
     f.active = gate;
+    f.A32 = A32;
     hipLaunchKernelGGL(k_wl_pack_factor<T>, dim3((unsigned)(sync.ntiles * ((g.nz + 3) / 4))), dim3(256), 0,
                        s, f);
     hipLaunchKernelGGL(k_sweep_reset_ticket, dim3(1), dim3(1), 0, s, sync.ticket);
@@ -984,13 +1002,15 @@ static void wl_factor(const Grid &g, const WfSync &sync, const double *A, double
 }
 
 void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<double> &wl, int *err, hipStream_t s, const int *gate) {
-    wl_factor<double>(g, sync, A, Af, wl, err, s, gate);
+                 const WlPrecond<double> &wl, int *err, hipStream_t s, const int *gate,
+                 float *A32) {
+    wl_factor<double>(g, sync, A, Af, wl, err, s, gate, A32);
 }
 
 void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<float> &wl, int *err, hipStream_t s, const int *gate) {
-    wl_factor<float>(g, sync, A, Af, wl, err, s, gate);
+                 const WlPrecond<float> &wl, int *err, hipStream_t s, const int *gate,
+                 float *A32) {
+    wl_factor<float>(g, sync, A, Af, wl, err, s, gate, A32);
 }
 
 struct WlBatch {

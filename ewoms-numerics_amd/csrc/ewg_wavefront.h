@@ -57,10 +57,14 @@ size_t wl_entries(const Grid &g, const WfSync &sync);
 // factors and upper blocks); entries without a cell keep what they hold (zeros). Af:
 // scratch for the operands in the forward layout, 7 x wl_entries x 4 doubles
 // gate (optional): a device flag; both launches return at once while it is 0
+// A32 (optional): the packing pass also stores every block of A rounded to FP32 there
+// (slot-major like A), the operator of the mixed-precision BiCGStab
 void wl_factor64(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<double> &wl, int *err, hipStream_t s, const int *gate = nullptr);
+                 const WlPrecond<double> &wl, int *err, hipStream_t s, const int *gate = nullptr,
+                 float *A32 = nullptr);
 void wl_factor32(const Grid &g, const WfSync &sync, const double *A, double *Af,
-                 const WlPrecond<float> &wl, int *err, hipStream_t s, const int *gate = nullptr);
+                 const WlPrecond<float> &wl, int *err, hipStream_t s, const int *gate = nullptr,
+                 float *A32 = nullptr);
 // v = (LU)^-1 d with one compute and one edge wavefront per tile. The forward sweep takes
 // tickets from sync.ticket[0], the backward sweep from sync.ticket[1]; tickets_clear:
 // a kernel queued on the stream after the last sweep that took tickets has set both to

@@ -67,7 +67,7 @@ class Params(C.Structure):
                 ("warm_start", C.c_int32),
                 ("local_linearizer", C.c_int32),
                 ("numeric_difference_method", C.c_int32),
-                ("reserved", C.c_int32 * 1)]
+                ("operator_fp32", C.c_int32)]
 
 
 class RestartClock(C.Structure):
@@ -117,6 +117,7 @@ def default_params(**overrides) -> Params:
     p.warm_start = 0
     p.local_linearizer = 0
     p.numeric_difference_method = 1
+    p.operator_fp32 = 0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

@@ -123,7 +123,12 @@ typedef struct ewg_params {
     int32_t numeric_difference_method; /* NumericDifferenceMethod of the finite differences:
                                           +1 forward (default), 0 central, -1 backward
                                           (fvbasefdlocallinearizer.hh:83) */
-    int32_t reserved[1];
+    int32_t operator_fp32;             /* 0: BiCGStab's operator products v = J y read the FP64
+                                          Jacobian (library default); 1: they read a copy of
+                                          the Jacobian's blocks rounded to FP32 once per linear
+                                          solve (products and sums in FP64, the recurrences
+                                          unchanged): the FP32 SpMV of the mixed-precision
+                                          form. Not available to ensemble members */
 } ewg_params;
 
 /* Counters and timers of the last calls; the split follows

@@ -228,3 +228,17 @@ def test_forced_sweep_failure_is_isolated_and_recovers():
     print(f"member 1 vs single context with the halved first step: max |dS| {dS:.2e}",
           flush=True)
     assert dS < 1e-7
+
+
+def test_members_with_the_fp32_operator_are_refused():
+    """The ensemble's operator products read each member's FP64 Jacobian: a member set up
+    for the FP32 operator copy (params.operator_fp32) is refused at creation."""
+    import ewg
+    from ewg import ensemble
+    probs = _problems()[:2]
+    ctxs = [ewg.GpuContext(probs[0], ensemble.member_params()),
+            ewg.GpuContext(probs[1], ensemble.member_params(operator_fp32=1))]
+    with pytest.raises(RuntimeError, match=f"failed: {ewg.capi.EWG_INVALID_ARGUMENT}"):
+        ensemble.Ensemble(ctxs)
+    for c in ctxs:
+        c.close()

@@ -4,7 +4,9 @@ preconditioner apply, on the GPU's own Jacobian (so each check isolates one kern
 
 Bars: FP64 results within 1e-12 relative (entry-wise with a 1e-13 * max floor); the FP32
 factors are the FP64 factors rounded to float (within one float rounding), and the FP32
-apply equals the oracle's float apply of those factors within 1e-6 relative.
+apply equals the oracle's float apply of those factors within 1e-6 relative. The FP32
+operator (params.operator_fp32) equals the oracle's FP64 product with the Jacobian's
+blocks rounded to float within 1e-12.
 Shapes include partial 8x8 tiles (x, y not multiples of 8) and the 2D lens grid.
 """
 import numpy as np
@@ -105,3 +107,48 @@ def test_fp32_preconditioner_matches_float_oracle(shape, schedule):
     dev = np.abs(vg - vo64).max() / np.abs(vo64).max()
     print(f"fp32 apply vs fp64 oracle: {dev:.3e} of max |v|", flush=True)
     assert dev < 1e-4
+
+
+@pytest.mark.parametrize("shape", SHAPES + ["lens"])
+def test_fp32_operator_apply_matches_rounded_jacobian(shape):
+    """BiCGStab's operator with params.operator_fp32: the product with the Jacobian's
+    blocks rounded once to float, in FP64 arithmetic and the FP64 product's order, i.e.
+    the oracle's SpMV of the rounded blocks."""
+    import ewg
+    P = _problem(shape)
+    print(f"fp32 operator shape {shape}", flush=True)
+    g, o, J = _system(P, ewg.default_params(operator_fp32=1), seed=9)
+    rng = np.random.default_rng(10)
+    x = rng.standard_normal(2 * P.N) * np.tile([1e4, 0.1], P.N)
+    yg = g.apply_operator(x)
+    yo = o.spmv(J.astype(np.float32).astype(np.float64), x)
+    print("bitwise:", np.array_equal(yg, yo), flush=True)
+    assert rel_err(yg, yo) < 1e-12
+    # it is not the FP64 product
+    assert not np.array_equal(yg, o.spmv(J, x))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("fp32_precond", [0, 1])
+def test_fp32_operator_solves_agree_between_schedules(shape, fp32_precond):
+    """The solve's FP32 copy of the Jacobian comes from the wavefront layout's operand
+    packing (schedule 0) or from a rounding pass (schedule 1): the two BiCGStab solves of
+    the same system are bit-identical (same copy, same preconditioner, same kernels)."""
+    import ewg
+    from ewg import problems
+    P = _problem(shape)
+    x = problems.perturbed_state(P, seed=11, dS=0.2)
+    out = []
+    for schedule in (0, 1):
+        g = ewg.GpuContext(P, ewg.default_params(operator_fp32=1, precond_fp32=fp32_precond,
+                                                 ilu_schedule=schedule))
+        g.set_solution(x, 0)
+        g.linearize(P.dt, 0, want=False)
+        rc, dx, its = g.solve()
+        print(f"shape {shape} fp32_precond {fp32_precond} schedule {schedule}: rc {rc} "
+              f"iterations {its}", flush=True)
+        assert rc == 0
+        out.append((dx, its))
+        g.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])

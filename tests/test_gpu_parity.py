@@ -224,18 +224,22 @@ def test_fp32_flux_linearization_close_to_fp64(case):
     assert rel_err(r64, ro) < 1e-12 and rel_err(j64, jo) < 1e-12
 
 
-@pytest.mark.parametrize("form", ["fp32_flux", "mixed"])
+@pytest.mark.parametrize("form", ["fp32_flux", "mixed", "mixed_operator"])
 def test_mixed_precision_steps_close_to_fp64_oracle(form):
-    """FP32 face fluxes (alone, and with the FP32 preconditioner) against the FP64 oracle,
-    both at Newton tolerance 1e-6, every time step: saturations within 1e-6, pressures
-    within 1e-6 relative, Newton iterations within one of the oracle's."""
+    """FP32 face fluxes (alone, with the FP32 preconditioner, and with the FP32
+    preconditioner and the FP32 copy of the Jacobian in BiCGStab's operator products)
+    against the FP64 oracle, both at Newton tolerance 1e-6, every time step: saturations
+    within 1e-6, pressures within 1e-6 relative, Newton iterations within one of the
+    oracle's."""
     import ewg
     from ewg import problems
     from oracle.pyoracle import OracleContext
     P = problems.heterogeneous_3d(24, 20, 12, seed=4, patch=6, steps=4)
     extra = {"linearize_fp32": 1}
-    if form == "mixed":
+    if form in ("mixed", "mixed_operator"):
         extra["precond_fp32"] = 1
+    if form == "mixed_operator":
+        extra["operator_fp32"] = 1
     g = ewg.GpuContext(P, ewg.default_params(newton_tolerance=1e-6, **extra))
     o = OracleContext(P, ewg.default_params(newton_tolerance=1e-6))
     for step in range(4):
