@@ -611,10 +611,7 @@ __global__ void __launch_bounds__(BT) k_b_ks_init(const Member *mem, Maps mp, co
                                                   const int *serr, KrylovState *ks,
                                                   const double *partMax, const double *partDot) {
     const int id = blockIdx.x;
-    if (!act[id]) {
-        if (threadIdx.x == 0) ks[id].status = KS_IDLE;
-        return;
-    }
+    if (!act[id]) return;  // another member's solve may be running: left as it is
     const Member &m = mem[id];
     const double mx = range_max(partMax, mp.coff[id], mp.coff[id + 1]);
     const double d = range_sum(partDot, mp.coff[id], mp.coff[id + 1]);
@@ -846,6 +843,31 @@ __global__ void __launch_bounds__(BT) k_b_set(int n, int *v, const int *mask) {
     if (i < n && mask[i]) v[i] = 1;
 }
 
+// v[i] = 0 where mask[i]
+__global__ void __launch_bounds__(BT) k_b_zero_masked(int n, int *v, const int *mask) {
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i < n && mask[i]) v[i] = 0;
+}
+
+// no member in a solve (the start of a run)
+__global__ void __launch_bounds__(BT) k_b_ks_idle(int n, KrylovState *ks) {
+    const int i = blockIdx.x * BT + threadIdx.x;
+    if (i < n) ks[i].status = KS_IDLE;
+}
+
+// the rows of the batched run's per-member flags (ewg_ensemble::dmask)
+enum MaskSlot {
+    M_ACCEPT,   // solution(1) <- solution(0): a (sub)step converged
+    M_RESTORE,  // solution(0), solution(1) <- the step's start: a solve failed
+    M_SAVE,     // the step's start <- solution(1): a new step
+    M_EDGE,     // tile edge buffers back to the sentinel (a wavefront wait ran out)
+    M_UPD,      // Newton update
+    M_LIN,      // linearization
+    M_FAC,      // factorization and solve start
+    M_FORCE,    // sweep error flag set by the host (test hook)
+    kMaskSlots
+};
+
 }  // namespace batch
 }  // namespace ewg
 
@@ -884,13 +906,15 @@ struct ewg_ensemble {
     // per member: set when one of its tile-wavefront waits (sweeps or factorization) ran
     // out; turned into a Krylov breakdown of that member
     int *sweep_err = nullptr;
-    int *force_mask = nullptr;           // device copy of h_force (ewg_ensemble_force_sweep_failure)
     std::vector<int> force_left;         // per member: solves still to start with the flag set
+    // the batched run's per-member flags, kMaskSlots rows of n (device copy of h_mask)
+    int *dmask = nullptr;
+    double *pres_upd = nullptr;          // per member: the Newton update's norm
     // pinned host staging of every host<->device transfer of the batched mode, n entries
-    // each; each is rewritten only after a stream synchronisation that follows its last
-    // transfer
-    int *h_act = nullptr, *h_it = nullptr, *h_sel = nullptr, *h_serr = nullptr, *h_force = nullptr;
-    double *h_dts = nullptr, *h_res = nullptr;
+    // each (h_mask: kMaskSlots x n); each is rewritten only after a stream synchronisation
+    // that follows its last transfer
+    int *h_mask = nullptr, *h_it = nullptr, *h_serr = nullptr;
+    double *h_dts = nullptr, *h_res = nullptr, *h_res_upd = nullptr;
     KrylovState *h_ks = nullptr;
     std::string err;
 };
@@ -988,16 +1012,16 @@ int batched_setup(ewg_ensemble *e) {
     ENS_TRY(e, hipMemcpy(e->lmem, lmem.data(), lmem.size() * sizeof(int), hipMemcpyHostToDevice));
     ENS_TRY(e, hipMemcpy(e->loff, loff.data(), loff.size() * sizeof(int), hipMemcpyHostToDevice));
     ENS_TRY(e, dal(&e->sweep_err, n));
-    ENS_TRY(e, dal(&e->force_mask, n));
+    ENS_TRY(e, dal(&e->dmask, kMaskSlots * n));
+    ENS_TRY(e, dal(&e->pres_upd, n));
     ENS_TRY(e, hipMemset(e->sweep_err, 0, n * sizeof(int)));
     e->force_left.assign(n, 0);
-    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_act), n * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_mask), kMaskSlots * n * sizeof(int)));
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_it), n * sizeof(int)));
-    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_sel), n * sizeof(int)));
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_serr), n * sizeof(int)));
-    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_force), n * sizeof(int)));
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_dts), n * sizeof(double)));
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_res), n * sizeof(double)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_res_upd), n * sizeof(double)));
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_ks), n * sizeof(KrylovState)));
     // batched tile-wavefront sweeps when every member carries the layout
     bool wl = true;
@@ -1095,226 +1119,294 @@ bool proceed_h(int it, double err, double lerr, const Member &m) {
     return true;
 }
 
-// one Newton solve (newtonmethod.hh:308-517) of dts[m] for every member with part[m],
-// from x0 = x1; rc[m] = 0 or the EWG_* code of the failure; err[m] = final error.
-// Every transfer goes through the ensemble's pinned staging buffers (h_*); each buffer is
-// rewritten only after the stream synchronisation that follows its previous transfer.
-int batched_newton(ewg_ensemble *e, const std::vector<int> &part, const std::vector<double> &dts,
-                   std::vector<int> &rc, std::vector<double> &err) {
+// The batched run: every member advances through its own time steps, Newton iterations
+// (newtonmethod.hh:308-517) and BiCGStab solves, independently of the others. Each round
+// of the host (one stream synchronisation) queues, for the members that need them: the
+// step bookkeeping copies, the Newton update, the linearization and its max-norm, the
+// factorization and start of a solve; then `replay` BiCGStab iterations of every member
+// whose solve runs (members in other states skip them); then reads back the norms and
+// Krylov states. The host applies each member's Newton rules to what it reads. A member's
+// operations, and their order, are those of one context stepped alone, so its results
+// are too; the rounds only decide which members' work shares a launch (a member that
+// converges early goes on to its next iteration or step instead of waiting for the
+// slowest). A failed solve is retried from the step's start with 2, 4, ... substeps
+// (fvbaseproblem.hh:507-553 halves a failed step); a member whose step still fails keeps
+// its last converged state and stops. commit = false (one step: the tuner's trial): the
+// members return to their start state afterwards.
+// After a Newton update the member's next linearization is queued in the same round,
+// before the update's finiteness is known; if the update turns out non-finite the solve
+// fails, and that linearization is neither counted nor used.
+// Every transfer goes through the ensemble's pinned staging buffers (h_*), each rewritten
+// only after the synchronisation that ends the round of its previous transfer.
+int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
     const int n = static_cast<int>(e->members.size());
-    if ((int)part.size() != n || (int)dts.size() != n || (int)rc.size() != n ||
-        (int)err.size() != n || (int)e->force_left.size() != n || !e->h_act || !e->h_ks)
-        return set_err(e, EWG_INVALID_STATE, "batched Newton: per-member arrays of the wrong size");
+    if ((int)e->force_left.size() != n || !e->h_mask || !e->h_ks || !e->h_res_upd || !e->dmask)
+        return set_err(e, EWG_INVALID_STATE, "batched run: per-member buffers missing");
+    if (!commit && steps != 1) return set_err(e, EWG_INVALID_ARGUMENT, "trial runs take one step");
     const Maps mp = maps(e);
     hipStream_t s = e->stream;
-    enum { RUN = 0, DONE = 1, FAILED = 2, OUT = 3 };
-    std::vector<int> state(n, OUT), it(n, 0), act(n, 0);
-    std::vector<double> lerr(n, 1e100);
-    for (int m = 0; m < n; ++m) {
-        rc[m] = 0;
-        err[m] = 1e100;
-        if (part[m]) state[m] = RUN;
-    }
-    auto upload = [&](const std::vector<int> &v, int *pinned, int *d) -> int {
-        std::copy(v.begin(), v.end(), pinned);
-        ENS_TRY(e, hipMemcpyAsync(d, pinned, n * sizeof(int), hipMemcpyHostToDevice, s));
-        return EWG_OK;
+    // Per-member phases. Queued in a round: NEED_LIN (linearization), NEED_FAC
+    // (factorization and solve start), NEED_UPD (update, with the next linearization when
+    // the loop goes on). Read at the round's end: LIN_PEND, SOLVING, UPD_PEND. Host-only,
+    // passed through within a round: STEP (a new time step), NSTART (a new Newton solve),
+    // HEAD (the loop head), NDONE (a solve ended with rc). IDLE: done or out.
+    enum { IDLE, STEP, NSTART, HEAD, NDONE, NEED_LIN, LIN_PEND, NEED_FAC, SOLVING, NEED_UPD, UPD_PEND };
+    struct Prog {
+        int phase = IDLE, it = 0, left = 0, level = 0, sub = 0, rc = 0;
+        bool lin = false;  // a linearization is queued beside the update
+        double err = 1e100, lerr = 1e100, dt = 0.0;
     };
-    auto fail = [&](int m, int code) {
-        state[m] = FAILED;
-        rc[m] = code;
+    std::vector<Prog> P(n);
+    int *mask = e->h_mask;  // kMaskSlots rows of n flags
+    auto mark = [&](int slot, int m) { mask[slot * n + m] = 1; };
+    auto marked = [&](int slot) {
+        for (int m = 0; m < n; ++m)
+            if (mask[slot * n + m]) return true;
+        return false;
+    };
+    auto dm = [&](int slot) -> const int * { return e->dmask + slot * n; };
+    // the host's transitions of member m up to its next device work (or its end)
+    auto advance = [&](int m) {
+        Prog &p = P[m];
+        const Member &h = e->host[m];
+        for (;;) {
+            switch (p.phase) {
+                case STEP:  // a new time step: its start state is kept for retries
+                    mark(M_SAVE, m);
+                    p.level = 0;
+                    p.sub = 0;
+                    p.phase = NSTART;
+                    break;
+                case NSTART:  // a Newton solve of the (sub)step from solution(1)
+                    p.it = 0;
+                    p.err = 1e100;  // newtonmethod.hh:197
+                    p.lerr = 1e100;
+                    p.dt = dt / (double)(1 << p.level);
+                    p.phase = HEAD;
+                    break;
+                case HEAD:  // proceed_ (newtonmethod.hh:866-884)
+                    if (proceed_h(p.it, p.err, p.lerr, h)) {
+                        p.lerr = p.err;  // beginIteration_
+                        p.phase = NEED_LIN;
+                        return;
+                    }
+                    p.rc = p.err <= h.newton_tol ? 0 : EWG_NOT_CONVERGED;
+                    p.phase = NDONE;
+                    break;
+                case NDONE:
+                    e->outcome[m].last_error = p.err;
+                    if (p.rc == 0) {
+                        mark(M_ACCEPT, m);  // solution(1) <- solution(0)
+                        if (++p.sub < (1 << p.level)) {
+                            p.phase = NSTART;
+                            break;
+                        }
+                        if (commit) e->outcome[m].steps++;
+                        if (--p.left > 0) {
+                            p.phase = STEP;
+                            break;
+                        }
+                        p.phase = IDLE;
+                        return;
+                    }
+                    mark(M_RESTORE, m);  // back to the step's start
+                    p.sub = 0;
+                    if (p.level < h.max_div) {
+                        ++p.level;
+                        e->outcome[m].divisions++;
+                        p.phase = NSTART;
+                        break;
+                    }
+                    e->outcome[m].status = p.rc;
+                    p.phase = IDLE;
+                    return;
+                default:
+                    return;
+            }
+        }
     };
     if (int r = iteration_graph(e)) return r;
-    std::copy(dts.begin(), dts.end(), e->h_dts);
-    ENS_TRY(e, hipMemcpyAsync(e->dts, e->h_dts, n * sizeof(double), hipMemcpyHostToDevice, s));
-    while (true) {
-        bool any = false;
+    std::fill(mask, mask + kMaskSlots * n, 0);
+    bool any_member = false;
+    for (int m = 0; m < n; ++m) {
+        if (e->outcome[m].status != 0) continue;
+        P[m].left = steps;
+        P[m].phase = STEP;
+        advance(m);
+        any_member = true;
+    }
+    std::vector<int> took(n, 0);  // members of this run (the trial restores them)
+    for (int m = 0; m < n; ++m) took[m] = P[m].phase != IDLE;
+    // no member starts in a solve
+    hipLaunchKernelGGL(k_b_ks_idle, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ks);
+    while (any_member) {
+        // ------------------------------------------------------------- queue this round
+        bool lin = false, fac = false, upd = false, forced = false, solving = false;
         for (int m = 0; m < n; ++m) {
-            act[m] = 0;
-            if (state[m] != RUN) continue;
-            if (!proceed_h(it[m], err[m], lerr[m], e->host[m])) {
-                state[m] = DONE;
-                continue;
+            Prog &p = P[m];
+            e->h_it[m] = p.it;
+            e->h_dts[m] = p.dt;
+            switch (p.phase) {
+                case NEED_UPD:
+                    mark(M_UPD, m);
+                    upd = true;
+                    p.phase = UPD_PEND;
+                    // the loop head after a finite update
+                    if (proceed_h(p.it + 1, p.err, p.lerr, e->host[m])) {
+                        mark(M_LIN, m);
+                        e->h_it[m] = p.it + 1;
+                        p.lin = lin = true;
+                    }
+                    break;
+                case NEED_LIN:
+                    mark(M_LIN, m);
+                    lin = true;
+                    p.phase = LIN_PEND;
+                    break;
+                case NEED_FAC:
+                    mark(M_FAC, m);
+                    fac = true;
+                    if (e->force_left[m] > 0) {
+                        --e->force_left[m];
+                        mark(M_FORCE, m);
+                        forced = true;
+                    }
+                    p.phase = SOLVING;
+                    break;
+                default:
+                    break;
             }
-            act[m] = 1;
-            any = true;
-            lerr[m] = err[m];  // beginIteration_
+            solving |= p.phase == SOLVING;
         }
-        if (!any) break;
-        if (int r = upload(act, e->h_act, e->act)) return r;
-        if (int r = upload(it, e->h_it, e->niter)) return r;
-        hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, (int)V_XCUR, (int)V_X0);
-        hipLaunchKernelGGL(k_b_lin<0>, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->niter, e->dts);
-        hipLaunchKernelGGL(k_b_lin<1>, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->niter, e->dts);
-        hipLaunchKernelGGL(k_b_maxabs, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, (int)V_RES, e->part0);
-        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(BT), 0, s, mp, e->act, e->part0, 1, e->pres);
-        ENS_TRY(e, hipMemcpyAsync(e->h_res, e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        ENS_TRY(e, hipStreamSynchronize(s));
-        bool solve = false;
-        for (int m = 0; m < n; ++m) {
-            if (!act[m]) continue;
-            e->outcome[m].newton++;
-            lerr[m] = err[m];  // preSolve_
-            err[m] = e->h_res[m];
-            act[m] = 0;
-            if (err[m] > e->host[m].newton_max_error) {
-                fail(m, EWG_NUMERICAL_ISSUE);
-            } else if (!proceed_h(it[m], err[m], lerr[m], e->host[m])) {
-                ++it[m];
+        const bool copies = marked(M_ACCEPT) || marked(M_RESTORE) || marked(M_SAVE) || marked(M_EDGE);
+        if (!lin && !fac && !upd && !solving && !copies) break;
+        ENS_TRY(e, hipMemcpyAsync(e->dmask, mask, kMaskSlots * n * sizeof(int), hipMemcpyHostToDevice, s));
+        ENS_TRY(e, hipMemcpyAsync(e->niter, e->h_it, n * sizeof(int), hipMemcpyHostToDevice, s));
+        ENS_TRY(e, hipMemcpyAsync(e->dts, e->h_dts, n * sizeof(double), hipMemcpyHostToDevice, s));
+        const dim3 gC(e->nblkC), gN((n + BT - 1) / BT), bT(BT);
+        if (marked(M_ACCEPT))
+            hipLaunchKernelGGL(k_b_copy, gC, bT, 0, s, e->dmem, mp, dm(M_ACCEPT), (int)V_X1, (int)V_X0);
+        if (marked(M_RESTORE)) {
+            hipLaunchKernelGGL(k_b_copy, gC, bT, 0, s, e->dmem, mp, dm(M_RESTORE), (int)V_X0, (int)V_XS);
+            hipLaunchKernelGGL(k_b_copy, gC, bT, 0, s, e->dmem, mp, dm(M_RESTORE), (int)V_X1, (int)V_XS);
+        }
+        if (marked(M_SAVE))
+            hipLaunchKernelGGL(k_b_copy, gC, bT, 0, s, e->dmem, mp, dm(M_SAVE), (int)V_XS, (int)V_X1);
+        // a member whose wavefront wait ran out may have left unconsumed words in its tile
+        // edge buffers: back to the sentinel before its next sweep
+        if (e->wl_py)
+            for (int m = 0; m < n; ++m)
+                if (mask[M_EDGE * n + m]) wf_init_edges(e->members[m]->wf, s);
+        if (upd) {
+            hipLaunchKernelGGL(k_b_update, gC, bT, 0, s, e->dmem, mp, dm(M_UPD), e->part0);
+            hipLaunchKernelGGL(k_b_reduce, dim3(n), bT, 0, s, mp, dm(M_UPD), e->part0, 0, e->pres_upd);
+        }
+        if (lin) {
+            hipLaunchKernelGGL(k_b_copy, gC, bT, 0, s, e->dmem, mp, dm(M_LIN), (int)V_XCUR, (int)V_X0);
+            hipLaunchKernelGGL(k_b_lin<0>, gC, bT, 0, s, e->dmem, mp, dm(M_LIN), e->niter, e->dts);
+            hipLaunchKernelGGL(k_b_lin<1>, gC, bT, 0, s, e->dmem, mp, dm(M_LIN), e->niter, e->dts);
+            hipLaunchKernelGGL(k_b_maxabs, gC, bT, 0, s, e->dmem, mp, dm(M_LIN), (int)V_RES, e->part0);
+            hipLaunchKernelGGL(k_b_reduce, dim3(n), bT, 0, s, mp, dm(M_LIN), e->part0, 1, e->pres);
+        }
+        if (fac) {
+            // e->act: the members whose factorization and solve start this round (the
+            // wavefront batch reads each member's flag there)
+            ENS_TRY(e, hipMemcpyAsync(e->act, dm(M_FAC), n * sizeof(int), hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_b_zero_masked, gN, bT, 0, s, n, e->ilu_err, (const int *)e->act);
+            hipLaunchKernelGGL(k_b_zero_masked, gN, bT, 0, s, n, e->sweep_err, (const int *)e->act);
+            if (e->wl_py) {
+                // the factorizations in one tile-wavefront launch, into the layouts
+                wl_batch_factor(e->wl_py, s);
             } else {
-                act[m] = 1;
-                solve = true;
+                for (int l = 0; l < e->lmax; ++l)
+                    hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), bT, 0, s, e->dmem, mp, e->act, l, e->ilu_err);
+                hipLaunchKernelGGL(k_b_round, gC, bT, 0, s, e->dmem, mp, e->act);
             }
-        }
-        if (!solve) continue;
-        if (int r = upload(act, e->h_act, e->act)) return r;
-        hipLaunchKernelGGL(k_b_zero, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ilu_err);
-        hipLaunchKernelGGL(k_b_zero, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->sweep_err);
-        if (e->wl_py) {
-            // every member's factorization in one tile-wavefront launch, into its layout
-            wl_batch_factor(e->wl_py, s);
-        } else {
-            for (int l = 0; l < e->lmax; ++l)
-                hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), dim3(BT), 0, s, e->dmem, mp, e->act, l, e->ilu_err);
-            hipLaunchKernelGGL(k_b_round, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act);
-        }
-        hipLaunchKernelGGL(k_b_init, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0, e->part1);
-        hipLaunchKernelGGL(k_b_ks_init, dim3(n), dim3(BT), 0, s, e->dmem, mp, e->act, e->ilu_err, e->sweep_err,
-                           e->ks, e->part0, e->part1);
-        // members whose sweeps the host makes fail in this solve (a test of the failure
-        // path): their flag is set after the factorization, before the first sweep
-        bool forced = false;
-        for (int m = 0; m < n; ++m) {
-            e->h_force[m] = act[m] && e->force_left[m] > 0;
-            if (e->h_force[m]) {
-                --e->force_left[m];
-                forced = true;
-            }
-        }
-        if (forced) {
-            ENS_TRY(e, hipMemcpyAsync(e->force_mask, e->h_force, n * sizeof(int), hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_b_set, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->sweep_err, e->force_mask);
+            hipLaunchKernelGGL(k_b_init, gC, bT, 0, s, e->dmem, mp, e->act, e->part0, e->part1);
+            hipLaunchKernelGGL(k_b_ks_init, dim3(n), bT, 0, s, e->dmem, mp, e->act, e->ilu_err,
+                               e->sweep_err, e->ks, e->part0, e->part1);
+            // members whose sweeps the host makes fail in this solve (a test of the
+            // failure path): the flag is set after the factorization, before the first
+            // sweep
+            if (forced) hipLaunchKernelGGL(k_b_set, gN, bT, 0, s, n, e->sweep_err, dm(M_FORCE));
         }
         ENS_TRY(e, hipGetLastError());
-        // iterations are replayed speculatively, the states read after 1, 2, 4, ... then
-        // every `replay` of them (members that left the running state skip the rest)
-        int batch = 1;
-        while (true) {
+        if (solving)
+            for (int q = 0; q < e->replay; ++q) ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
+        if (lin) ENS_TRY(e, hipMemcpyAsync(e->h_res, e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (upd)
+            ENS_TRY(e, hipMemcpyAsync(e->h_res_upd, e->pres_upd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (solving) {
             ENS_TRY(e, hipMemcpyAsync(e->h_ks, e->ks, n * sizeof(KrylovState), hipMemcpyDeviceToHost, s));
             ENS_TRY(e, hipMemcpyAsync(e->h_serr, e->sweep_err, n * sizeof(int), hipMemcpyDeviceToHost, s));
-            ENS_TRY(e, hipStreamSynchronize(s));
-            bool running = false;
-            for (int m = 0; m < n; ++m) running |= e->h_ks[m].status == KS_RUN;
-            if (!running) break;
-            for (int q = 0; q < batch; ++q) ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
-            batch = std::min(batch * 2, e->replay);
         }
-        // a member whose wavefront wait ran out may have left unconsumed words in its
-        // tile edge buffers: back to the sentinel before its next sweep
-        if (e->wl_py) {
-            bool reset = false;
-            for (int m = 0; m < n; ++m)
-                if (e->h_serr[m]) {
-                    wf_init_edges(e->members[m]->wf, s);
-                    reset = true;
-                }
-            if (reset) ENS_TRY(e, hipStreamSynchronize(s));
-        }
-        bool upd = false;
-        for (int m = 0; m < n; ++m) {
-            if (!act[m]) continue;
-            const KrylovState &k = e->h_ks[m];
-            e->outcome[m].linear += k.iterations;
-            e->outcome[m].solves++;
-            if (k.status == KS_CONV) {
-                upd = true;
-                continue;
-            }
-            act[m] = 0;
-            fail(m, k.status == KS_BREAK ? EWG_NUMERICAL_ISSUE : EWG_NOT_CONVERGED);
-        }
-        if (!upd) continue;
-        if (int r = upload(act, e->h_act, e->act)) return r;
-        hipLaunchKernelGGL(k_b_update, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, e->act, e->part0);
-        hipLaunchKernelGGL(k_b_reduce, dim3(n), dim3(BT), 0, s, mp, e->act, e->part0, 0, e->pres);
-        ENS_TRY(e, hipMemcpyAsync(e->h_res, e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
         ENS_TRY(e, hipStreamSynchronize(s));
+        // ------------------------------------------------ the round's results, per member
+        std::fill(mask, mask + kMaskSlots * n, 0);
         for (int m = 0; m < n; ++m) {
-            if (!act[m]) continue;
-            if (!std::isfinite(e->h_res[m])) fail(m, EWG_NUMERICAL_ISSUE);
-            else ++it[m];
-        }
-    }
-    for (int m = 0; m < n; ++m) {
-        if (state[m] == DONE && !(err[m] <= e->host[m].newton_tol)) fail(m, EWG_NOT_CONVERGED);
-        if (part[m]) e->outcome[m].last_error = err[m];
-    }
-    return EWG_OK;
-}
-
-// dst <- src for the members with sel[m] (act buffer reused)
-int batched_copy(ewg_ensemble *e, const std::vector<int> &sel, int dst, int src) {
-    const int n = static_cast<int>(e->members.size());
-    if ((int)sel.size() != n || !e->h_sel)
-        return set_err(e, EWG_INVALID_STATE, "batched copy: selection of the wrong size");
-    std::copy(sel.begin(), sel.end(), e->h_sel);
-    ENS_TRY(e, hipMemcpyAsync(e->act, e->h_sel, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
-    hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, maps(e), e->act, dst, src);
-    ENS_TRY(e, hipGetLastError());
-    ENS_TRY(e, hipStreamSynchronize(e->stream));
-    return EWG_OK;
-}
-
-// one time step of dt for every member still running; a failed solve is retried from
-// the step's start with 2, 4, ... substeps (FvBaseProblem::timeIntegration halves the
-// step, fvbaseproblem.hh:507-553); commit = 0: return to the start state afterwards
-int batched_step(ewg_ensemble *e, double dt, bool commit) {
-    const int n = static_cast<int>(e->members.size());
-    std::vector<int> in(n, 0), level(n, 0), sub(n, 0), done(n, 0), fin(n, 0), rc(n, 0);
-    std::vector<double> err(n, 0.0), dts(n, dt);
-    for (int m = 0; m < n; ++m) in[m] = e->outcome[m].status == 0;
-    if (int r = batched_copy(e, in, V_XS, V_X1)) return r;
-    while (true) {
-        std::vector<int> part(n, 0);
-        bool any = false;
-        for (int m = 0; m < n; ++m) {
-            part[m] = in[m] && !done[m];
-            any |= part[m] != 0;
-            dts[m] = dt / (double)(1 << level[m]);
-        }
-        if (!any) break;
-        if (int r = batched_newton(e, part, dts, rc, err)) return r;
-        std::vector<int> ok(n, 0), bad(n, 0);
-        for (int m = 0; m < n; ++m) {
-            if (!part[m]) continue;
-            if (rc[m] == 0) {
-                ok[m] = 1;
-                if (++sub[m] == (1 << level[m])) done[m] = 1;
-            } else {
-                bad[m] = 1;
-                sub[m] = 0;
-                if (level[m] < e->host[m].max_div) {
-                    ++level[m];
-                    e->outcome[m].divisions++;
-                } else {
-                    done[m] = 1;
-                    fin[m] = rc[m];
+            Prog &p = P[m];
+            const Member &h = e->host[m];
+            switch (p.phase) {
+                case UPD_PEND: {
+                    const bool lin_queued = p.lin;
+                    p.lin = false;
+                    if (!std::isfinite(e->h_res_upd[m])) {
+                        p.rc = EWG_NUMERICAL_ISSUE;  // "Non-finite update!"
+                        p.phase = NDONE;
+                        break;
+                    }
+                    ++p.it;
+                    if (!lin_queued) {
+                        p.phase = HEAD;
+                        break;
+                    }
+                    p.lerr = p.err;  // the loop head went on: beginIteration_
+                    [[fallthrough]];
                 }
+                case LIN_PEND:
+                    e->outcome[m].newton++;
+                    p.lerr = p.err;  // preSolve_
+                    p.err = e->h_res[m];
+                    if (p.err > h.newton_max_error) {
+                        p.rc = EWG_NUMERICAL_ISSUE;
+                        p.phase = NDONE;
+                    } else if (!proceed_h(p.it, p.err, p.lerr, h)) {
+                        ++p.it;
+                        p.phase = HEAD;
+                    } else {
+                        p.phase = NEED_FAC;
+                    }
+                    break;
+                case SOLVING: {
+                    const KrylovState &k = e->h_ks[m];
+                    if (k.status == KS_RUN) break;
+                    e->outcome[m].linear += k.iterations;
+                    e->outcome[m].solves++;
+                    if (e->h_serr[m]) mark(M_EDGE, m);
+                    if (k.status == KS_CONV) {
+                        p.phase = NEED_UPD;
+                    } else {
+                        p.rc = k.status == KS_BREAK ? EWG_NUMERICAL_ISSUE : EWG_NOT_CONVERGED;
+                        p.phase = NDONE;
+                    }
+                    break;
+                }
+                default:
+                    break;
             }
+            advance(m);
         }
-        if (int r = batched_copy(e, ok, V_X1, V_X0)) return r;   // substep accepted
-        if (int r = batched_copy(e, bad, V_X0, V_XS)) return r;  // back to the start
-        if (int r = batched_copy(e, bad, V_X1, V_XS)) return r;
     }
-    std::vector<int> back(n, 0);
-    for (int m = 0; m < n; ++m) {
-        if (!in[m]) continue;
-        if (fin[m]) e->outcome[m].status = fin[m];
-        else if (commit) e->outcome[m].steps++;
-        back[m] = fin[m] != 0 || !commit;
+    if (!commit) {
+        // the trial's members back to their start state (the step's start, kept in xs)
+        std::copy(took.begin(), took.end(), mask);
+        ENS_TRY(e, hipMemcpyAsync(e->dmask, mask, n * sizeof(int), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, dm(0), (int)V_X0, (int)V_XS);
+        hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, dm(0), (int)V_X1, (int)V_XS);
+        ENS_TRY(e, hipGetLastError());
+        ENS_TRY(e, hipStreamSynchronize(s));
+        std::fill(mask, mask + kMaskSlots * n, 0);
     }
-    if (int r = batched_copy(e, back, V_X0, V_XS)) return r;
-    if (int r = batched_copy(e, back, V_X1, V_XS)) return r;
     return EWG_OK;
 }
 
@@ -1407,11 +1499,10 @@ void ewg_ensemble_destroy(ewg_ensemble *e) {
     wl_batch_free(e->wl_rz);
     void *bufs[] = {e->cmem, e->coff, e->lmem, e->loff, e->act, e->niter, e->ilu_err, e->ks,
                     e->part0, e->part1, e->part2, e->pres, e->partAny, e->dts, e->sweep_err,
-                    e->force_mask};
+                    e->dmask, e->pres_upd};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    void *pinned[] = {e->h_act, e->h_it, e->h_sel, e->h_serr, e->h_force, e->h_dts, e->h_res,
-                      e->h_ks};
+    void *pinned[] = {e->h_mask, e->h_it, e->h_serr, e->h_dts, e->h_res, e->h_res_upd, e->h_ks};
     for (void *b : pinned)
         if (b) (void)hipHostFree(b);
     for (double *b : e->xs)
@@ -1453,7 +1544,7 @@ int ewg_ensemble_tune_modes(ewg_ensemble *e, double dt, int32_t modes, ewg_ensem
     static_assert(sizeof(cand) / sizeof(cand[0]) <= EWG_ENSEMBLE_MAX_CANDIDATES, "candidates");
     if (modes & EWG_ENSEMBLE_TUNE_BATCHED) {
         // an untimed trial first: the first batched step also captures the iteration graph
-        if (int rc = batched_step(e, dt, false)) return rc;
+        if (int rc = batched_run(e, dt, 1, false)) return rc;
         e->outcome = saved;
     }
     for (const Cand &c : cand) {
@@ -1470,7 +1561,7 @@ int ewg_ensemble_tune_modes(ewg_ensemble *e, double dt, int32_t modes, ewg_ensem
         if (batched) {
             e->replay = c.replay;
             const double t0 = now_s();
-            const int rc = batched_step(e, dt, false);
+            const int rc = batched_run(e, dt, 1, false);
             e->replay = keep_replay;
             if (rc) return rc;
             ms = 1e3 * (now_s() - t0);
@@ -1527,8 +1618,7 @@ int ewg_ensemble_run(ewg_ensemble *e, double dt, int32_t steps, double *seconds)
     if (e->mode == EWG_ENSEMBLE_BATCHED) {
         ENS_TRY(e, hipStreamSynchronize(e->stream));
         const double t0 = now_s();
-        for (int k = 0; k < steps; ++k)
-            if (int rc = batched_step(e, dt, true)) return rc;
+        if (int rc = batched_run(e, dt, steps, true)) return rc;
         if (seconds) *seconds = now_s() - t0;
         ENS_TRY(e, hipMemcpy(e->dout, e->outcome.data(), n * sizeof(Outcome), hipMemcpyHostToDevice));
         return EWG_OK;

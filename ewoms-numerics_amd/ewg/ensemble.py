@@ -168,6 +168,8 @@ def main(argv=None):
                          "long on large members)")
     ap.add_argument("--mode", choices=("batched", "persistent"), default="batched")
     ap.add_argument("--threads", type=int, default=512)
+    ap.add_argument("--replay", type=int, default=0,
+                    help="batched mode: BiCGStab iterations queued per round (0: the default)")
     ap.add_argument("--ilu-schedule", type=int, default=0, choices=(0, 1),
                     help="0: tile-wavefront sweeps in the batched mode; 1: hyperplane sweeps")
     args = ap.parse_args(argv)
@@ -180,7 +182,10 @@ def main(argv=None):
         tuning = ens.tune(args.dt)
     else:
         ens.set_mode(BATCHED if args.mode == "batched" else PERSISTENT, args.threads)
-        tuning = {"mode": args.mode, "threads": args.threads, "candidates": []}
+        if args.replay:
+            ens.set_replay(args.replay)
+        tuning = {"mode": args.mode, "threads": args.threads, "replay": args.replay,
+                  "candidates": []}
     t2 = time.perf_counter()
     sec = ens.run(args.dt, args.steps)
     st = [ens.stats(i) for i in range(len(probs))]
