@@ -41,6 +41,7 @@ struct Outcome {
     long long steps, newton, linear, solves;
     int status, divisions;
     double last_error;
+    long long retries;  // solves redone with FP64 factors (batched mode, precond_fp32 = 2)
 };
 
 constexpr double kBreakdownEps = 2.2250738585072014e-308 * 1e10;  // DBL_MIN * 1e10
@@ -864,6 +865,10 @@ enum MaskSlot {
     M_UPD,      // Newton update
     M_LIN,      // linearization
     M_FAC,      // factorization and solve start
+    M_FAC32,    // ... of the members whose solves use FP32 factors
+    M_FAC64,    // ... of the members whose solves use FP64 factors
+    M_PREC,     // each member's preconditioner precision (0 FP32, 1 FP64): the wavefront
+                // batches' sweeps read it in place
     M_FORCE,    // sweep error flag set by the host (test hook)
     kMaskSlots
 };
@@ -903,6 +908,16 @@ struct ewg_ensemble {
     // members with the tiles' wavefront layout (params.ilu_schedule = 0, 3D, all of one
     // precision): the preconditioner applications run as batched tile-wavefront sweeps
     void *wl_py = nullptr, *wl_rz = nullptr;
+    // FP32 preconditioner with FP64 retries (members with precond_fp32 = 2): a member whose
+    // FP32-preconditioned solve fails redoes it, and makes its later solves, with FP64
+    // factors (use64); the wavefront path then also launches these FP64 batches over the
+    // members' FP64 layouts (any64), factorizing those with act64 set
+    std::vector<int> retry_ok, use64;
+    void *wl_py64 = nullptr, *wl_rz64 = nullptr;
+    int *act64 = nullptr;
+    bool any64 = false;
+    bool regraph = false;   // the iteration graph must be captured again (any64 changed)
+    int *h_fp32 = nullptr;  // pinned 0 and 1: a member's Member::fp32 on the hyperplane path
     // per member: set when one of its tile-wavefront waits (sweeps or factorization) ran
     // out; turned into a Krylov breakdown of that member
     int *sweep_err = nullptr;
@@ -1012,7 +1027,15 @@ int batched_setup(ewg_ensemble *e) {
     ENS_TRY(e, hipMemcpy(e->lmem, lmem.data(), lmem.size() * sizeof(int), hipMemcpyHostToDevice));
     ENS_TRY(e, hipMemcpy(e->loff, loff.data(), loff.size() * sizeof(int), hipMemcpyHostToDevice));
     ENS_TRY(e, dal(&e->sweep_err, n));
+    ENS_TRY(e, dal(&e->act64, n));
     ENS_TRY(e, dal(&e->dmask, kMaskSlots * n));
+    ENS_TRY(e, hipMemset(e->dmask, 0, kMaskSlots * n * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_fp32), 2 * sizeof(int)));
+    e->h_fp32[0] = 0;
+    e->h_fp32[1] = 1;
+    e->retry_ok.assign(n, 0);
+    e->use64.assign(n, 0);
+    for (int m = 0; m < n; ++m) e->retry_ok[m] = e->members[m]->prm.precond_fp32 == 2;
     ENS_TRY(e, dal(&e->pres_upd, n));
     ENS_TRY(e, hipMemset(e->sweep_err, 0, n * sizeof(int)));
     e->force_left.assign(n, 0);
@@ -1047,6 +1070,8 @@ int batched_setup(ewg_ensemble *e) {
                 w.L = c->wl64.L; w.U = c->wl64.U; w.D = c->wl64.D; w.w = c->wl64.w;
             }
             w.status = &e->ks[m].status;  // device address of the member's Krylov status
+            w.prec = e->dmask + M_PREC * n + m;
+            w.want = 0;
             py[m] = w;
             py[m].A = c->J;               // the first batch also factorizes
             py[m].Af = c->wl_af;
@@ -1061,6 +1086,22 @@ int batched_setup(ewg_ensemble *e) {
         e->wl_py = wl_batch_create(py, fp32, e->stream);
         e->wl_rz = wl_batch_create(rz, fp32, e->stream);
         if (!e->wl_py || !e->wl_rz) return set_err(e, EWG_DEVICE_ERROR, "wavefront batch");
+        bool retry = false;
+        for (int m = 0; m < n; ++m) retry |= e->retry_ok[m] != 0;
+        if (fp32 && retry) {
+            // the same members over their FP64 layouts, for the members that fall back
+            for (int m = 0; m < n; ++m) {
+                const ewg_ctx *c = e->members[m];
+                for (WlMember *w : {&py[m], &rz[m]}) {
+                    w->L = c->wl64.L; w->U = c->wl64.U; w->D = c->wl64.D; w->w = c->wl64.w;
+                    w->want = 1;
+                }
+                py[m].active = e->act64 + m;
+            }
+            e->wl_py64 = wl_batch_create(py, false, e->stream);
+            e->wl_rz64 = wl_batch_create(rz, false, e->stream);
+            if (!e->wl_py64 || !e->wl_rz64) return set_err(e, EWG_DEVICE_ERROR, "FP64 wavefront batch");
+        }
     }
     return EWG_OK;
 }
@@ -1072,6 +1113,7 @@ Maps maps(const ewg_ensemble *e) { return Maps{e->cmem, e->coff, e->lmem, e->lof
 void launch_apply(ewg_ensemble *e, int src, int dst) {
     if (e->wl_py) {
         wl_batch_apply(src == V_P ? e->wl_py : e->wl_rz, e->stream);
+        if (e->any64) wl_batch_apply(src == V_P ? e->wl_py64 : e->wl_rz64, e->stream);
         return;
     }
     const Maps mp = maps(e);
@@ -1225,7 +1267,10 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
             }
         }
     };
-    if (int r = iteration_graph(e)) return r;
+    // the members' precisions at the start (a trial restores them)
+    const std::vector<int> use64_in = e->use64;
+    const bool any64_in = e->any64;
+    std::vector<int> flip(n, 0);  // hyperplane path: Member::fp32 of these to 0 next round
     std::fill(mask, mask + kMaskSlots * n, 0);
     bool any_member = false;
     for (int m = 0; m < n; ++m) {
@@ -1241,11 +1286,26 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
     hipLaunchKernelGGL(k_b_ks_idle, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ks);
     while (any_member) {
         // ------------------------------------------------------------- queue this round
-        bool lin = false, fac = false, upd = false, forced = false, solving = false;
+        if (e->regraph || !e->iter_graph) {
+            // one BiCGStab iteration of every member, now with (or without) the FP64
+            // batches' sweeps
+            if (e->iter_graph) (void)hipGraphExecDestroy(e->iter_graph);
+            e->iter_graph = nullptr;
+            e->regraph = false;
+            if (int r = iteration_graph(e)) return r;
+        }
+        for (int m = 0; m < n; ++m) {
+            if (!flip[m]) continue;
+            flip[m] = 0;
+            ENS_TRY(e, hipMemcpyAsync(&e->dmem[m].fp32, &e->h_fp32[0], sizeof(int), hipMemcpyHostToDevice, s));
+        }
+        bool lin = false, fac = false, fac32 = false, fac64 = false, upd = false, forced = false,
+             solving = false;
         for (int m = 0; m < n; ++m) {
             Prog &p = P[m];
             e->h_it[m] = p.it;
             e->h_dts[m] = p.dt;
+            mask[M_PREC * n + m] = e->use64[m];
             switch (p.phase) {
                 case NEED_UPD:
                     mark(M_UPD, m);
@@ -1265,7 +1325,9 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
                     break;
                 case NEED_FAC:
                     mark(M_FAC, m);
+                    mark(e->use64[m] ? M_FAC64 : M_FAC32, m);
                     fac = true;
+                    (e->use64[m] ? fac64 : fac32) = true;
                     if (e->force_left[m] > 0) {
                         --e->force_left[m];
                         mark(M_FORCE, m);
@@ -1309,21 +1371,26 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
             hipLaunchKernelGGL(k_b_reduce, dim3(n), bT, 0, s, mp, dm(M_LIN), e->part0, 1, e->pres);
         }
         if (fac) {
-            // e->act: the members whose factorization and solve start this round (the
-            // wavefront batch reads each member's flag there)
-            ENS_TRY(e, hipMemcpyAsync(e->act, dm(M_FAC), n * sizeof(int), hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_b_zero_masked, gN, bT, 0, s, n, e->ilu_err, (const int *)e->act);
-            hipLaunchKernelGGL(k_b_zero_masked, gN, bT, 0, s, n, e->sweep_err, (const int *)e->act);
+            hipLaunchKernelGGL(k_b_zero_masked, gN, bT, 0, s, n, e->ilu_err, dm(M_FAC));
+            hipLaunchKernelGGL(k_b_zero_masked, gN, bT, 0, s, n, e->sweep_err, dm(M_FAC));
             if (e->wl_py) {
-                // the factorizations in one tile-wavefront launch, into the layouts
-                wl_batch_factor(e->wl_py, s);
+                // the factorizations in one tile-wavefront launch per precision, into the
+                // layouts (each batch reads its members' flags in act / act64)
+                if (fac32) {
+                    ENS_TRY(e, hipMemcpyAsync(e->act, dm(M_FAC32), n * sizeof(int), hipMemcpyDeviceToDevice, s));
+                    wl_batch_factor(e->wl_py, s);
+                }
+                if (fac64) {
+                    ENS_TRY(e, hipMemcpyAsync(e->act64, dm(M_FAC64), n * sizeof(int), hipMemcpyDeviceToDevice, s));
+                    wl_batch_factor(e->wl_py64, s);
+                }
             } else {
                 for (int l = 0; l < e->lmax; ++l)
-                    hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), bT, 0, s, e->dmem, mp, e->act, l, e->ilu_err);
-                hipLaunchKernelGGL(k_b_round, gC, bT, 0, s, e->dmem, mp, e->act);
+                    hipLaunchKernelGGL(k_b_factor, dim3(e->nblkL), bT, 0, s, e->dmem, mp, dm(M_FAC), l, e->ilu_err);
+                hipLaunchKernelGGL(k_b_round, gC, bT, 0, s, e->dmem, mp, dm(M_FAC));
             }
-            hipLaunchKernelGGL(k_b_init, gC, bT, 0, s, e->dmem, mp, e->act, e->part0, e->part1);
-            hipLaunchKernelGGL(k_b_ks_init, dim3(n), bT, 0, s, e->dmem, mp, e->act, e->ilu_err,
+            hipLaunchKernelGGL(k_b_init, gC, bT, 0, s, e->dmem, mp, dm(M_FAC), e->part0, e->part1);
+            hipLaunchKernelGGL(k_b_ks_init, dim3(n), bT, 0, s, e->dmem, mp, dm(M_FAC), e->ilu_err,
                                e->sweep_err, e->ks, e->part0, e->part1);
             // members whose sweeps the host makes fail in this solve (a test of the
             // failure path): the flag is set after the factorization, before the first
@@ -1381,11 +1448,24 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
                     const KrylovState &k = e->h_ks[m];
                     if (k.status == KS_RUN) break;
                     e->outcome[m].linear += k.iterations;
-                    e->outcome[m].solves++;
                     if (e->h_serr[m]) mark(M_EDGE, m);
                     if (k.status == KS_CONV) {
+                        e->outcome[m].solves++;
                         p.phase = NEED_UPD;
+                    } else if (e->retry_ok[m] && !e->use64[m]) {
+                        // the FP32 factors failed: this solve again, and the member's later
+                        // ones, with FP64 factors
+                        e->use64[m] = 1;
+                        e->outcome[m].retries++;
+                        if (e->wl_py) {
+                            e->regraph |= !e->any64;
+                            e->any64 = true;
+                        } else {
+                            flip[m] = 1;
+                        }
+                        p.phase = NEED_FAC;
                     } else {
+                        e->outcome[m].solves++;
                         p.rc = k.status == KS_BREAK ? EWG_NUMERICAL_ISSUE : EWG_NOT_CONVERGED;
                         p.phase = NDONE;
                     }
@@ -1398,11 +1478,21 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
         }
     }
     if (!commit) {
-        // the trial's members back to their start state (the step's start, kept in xs)
+        // the trial's members back to their start state (the step's start, kept in xs) and
+        // their preconditioner precisions
         std::copy(took.begin(), took.end(), mask);
         ENS_TRY(e, hipMemcpyAsync(e->dmask, mask, n * sizeof(int), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, dm(0), (int)V_X0, (int)V_XS);
         hipLaunchKernelGGL(k_b_copy, dim3(e->nblkC), dim3(BT), 0, s, e->dmem, mp, dm(0), (int)V_X1, (int)V_XS);
+        for (int m = 0; m < n; ++m) {
+            if (e->use64[m] == use64_in[m]) continue;
+            e->use64[m] = use64_in[m];
+            if (!e->wl_py)
+                ENS_TRY(e, hipMemcpyAsync(&e->dmem[m].fp32, &e->h_fp32[use64_in[m] ? 0 : 1],
+                                          sizeof(int), hipMemcpyHostToDevice, s));
+        }
+        e->regraph |= e->any64 != any64_in;
+        e->any64 = any64_in;
         ENS_TRY(e, hipGetLastError());
         ENS_TRY(e, hipStreamSynchronize(s));
         std::fill(mask, mask + kMaskSlots * n, 0);
@@ -1497,12 +1587,15 @@ void ewg_ensemble_destroy(ewg_ensemble *e) {
     if (e->iter_graph) (void)hipGraphExecDestroy(e->iter_graph);
     wl_batch_free(e->wl_py);
     wl_batch_free(e->wl_rz);
+    wl_batch_free(e->wl_py64);
+    wl_batch_free(e->wl_rz64);
     void *bufs[] = {e->cmem, e->coff, e->lmem, e->loff, e->act, e->niter, e->ilu_err, e->ks,
                     e->part0, e->part1, e->part2, e->pres, e->partAny, e->dts, e->sweep_err,
-                    e->dmask, e->pres_upd};
+                    e->dmask, e->pres_upd, e->act64};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    void *pinned[] = {e->h_mask, e->h_it, e->h_serr, e->h_dts, e->h_res, e->h_res_upd, e->h_ks};
+    void *pinned[] = {e->h_mask, e->h_it, e->h_serr, e->h_dts, e->h_res, e->h_res_upd, e->h_ks,
+                      e->h_fp32};
     for (void *b : pinned)
         if (b) (void)hipHostFree(b);
     for (double *b : e->xs)
@@ -1686,6 +1779,7 @@ int ewg_ensemble_stats(const ewg_ensemble *e, int32_t i, ewg_ensemble_member_sta
     out->status = o.status;
     out->step_divisions = o.divisions;
     out->last_error = o.last_error;
+    out->precond_retries = static_cast<int32_t>(o.retries);
     return EWG_OK;
 }
 

@@ -204,6 +204,7 @@ bool use_wavefront_layout(const ewg_ctx *c) {
 // the parameter values the library accepts beyond the reference's own ranges
 const char *check_params(const ewg_params *p) {
     if (p->ilu_schedule < 0 || p->ilu_schedule > 1) return "ilu_schedule must be 0 or 1";
+    if (p->precond_fp32 < 0 || p->precond_fp32 > 2) return "precond_fp32 must be 0, 1 or 2";
     if (p->linearize_fp32 != 0 && p->linearize_fp32 != 1) return "linearize_fp32 must be 0 or 1";
     if (p->operator_fp32 != 0 && p->operator_fp32 != 1) return "operator_fp32 must be 0 or 1";
     if (p->local_linearizer != 0 && p->local_linearizer != 1)
@@ -223,8 +224,11 @@ int capture_reference_pressure(ewg_ctx *c) {
 
 // With params.operator_fp32 the solve's FP32 copy of the Jacobian is made here too: by the
 // wavefront layout's operand packing, which reads every block anyway, or by a rounding pass.
+// the current solve's preconditioner precision
+bool solve_fp32(const ewg_ctx *c) { return c->prm.precond_fp32 != 0 && !c->fp64_latched; }
+
 void precond_factor(ewg_ctx *c, const int *gate = nullptr) {
-    const bool fp32 = c->prm.precond_fp32;
+    const bool fp32 = solve_fp32(c);
     float *op32 = c->prm.operator_fp32 ? c->J32 : nullptr;
     if (use_wavefront_layout(c)) {
         // straight into the sweeps' layout (no natural-order factors)
@@ -255,13 +259,13 @@ void operator_apply(ewg_ctx *c, const double *x, double *y, const double *a, dou
 void precond_apply(ewg_ctx *c, const double *d, double *v, const int *status,
                    bool tickets_clear) {
     if (use_wavefront_layout(c)) {
-        if (c->prm.precond_fp32)
+        if (solve_fp32(c))
             wl_apply32(c->g, c->wf, c->wl32, d, v, status, c->stream, tickets_clear);
         else
             wl_apply64(c->g, c->wf, c->wl64, d, v, status, c->stream, tickets_clear);
         return;
     }
-    if (c->prm.precond_fp32)
+    if (solve_fp32(c))
         launch_ilu_apply32(c->g, c->L32, c->D32, c->U32, d, c->w32, v, status, c->stream);
     else
         launch_ilu_apply64(c->g, c->L64, c->D64, c->J, d, v, status, c->stream);
@@ -379,6 +383,24 @@ int bicgstab(ewg_ctx *c, const double *b, double *x, int *iters) {
     return bicgstab_end(c, b, x, iters);
 }
 
+// precond_fp32 = 2: a solve that failed with the FP32 factors (rc > 0: not converged or a
+// numerical issue) is redone from x0 = 0 with FP64 factors, which the context keeps for its
+// later solves; the redone solve's iterations are added to *iters. The FP32 factors'
+// rounding can stall BiCGStab on strongly heterogeneous permeability fields where the FP64
+// ones converge.
+int fp64_retry(ewg_ctx *c, const double *b, double *x, int rc, int *iters) {
+    if (rc <= 0 || c->prm.precond_fp32 != 2 || c->fp64_latched) return rc;
+    c->fp64_latched = true;
+    c->stats.precond_retries++;
+    bicgstab_begin(c, b, x, nullptr);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int its = 0;
+    rc = bicgstab_end(c, b, x, &its);
+    *iters += its;
+    return rc;
+}
+
 int linearize_device(ewg_ctx *c, double dt, int newton_iter) {
     LinArgs a = lin_args(c, dt, newton_iter);
     sample_begin(c, EWG_KERNEL_LINEARIZE);
@@ -487,7 +509,7 @@ int newton(ewg_ctx *c, double dt) {
             HIP_TRY(c, hipStreamSynchronize(s));
         }
         int its = 0;
-        const int rc = bicgstab_end(c, c->R, c->kx, &its);
+        const int rc = fp64_retry(c, c->R, c->kx, bicgstab_end(c, c->R, c->kx, &its), &its);
         const double t2 = now_s();
         c->stats.time_solve_s += t2 - t1;
         c->last_linear_iterations = its;
@@ -902,6 +924,7 @@ int ewg_set_params(ewg_ctx *c, const ewg_params *p) {
         if (rc) return rc;
     }
     c->prm = *p;
+    c->fp64_latched = false;
     return EWG_OK;
 }
 
@@ -1049,6 +1072,7 @@ int ewg_solve(ewg_ctx *c, double *dx, int32_t *iterations) {
     if (!c->have_system) return set_err(c, EWG_INVALID_STATE, "no linearization");
     int its = 0;
     int rc = bicgstab(c, c->R, c->kx, &its);
+    rc = fp64_retry(c, c->R, c->kx, rc, &its);
     c->last_linear_iterations = its;
     if (iterations) *iterations = its;
     if (dx && rc >= 0) {

@@ -49,6 +49,9 @@ struct ewg_ctx {
     // preconditioner
     double *L64 = nullptr, *D64 = nullptr;
     float *L32 = nullptr, *D32 = nullptr, *U32 = nullptr, *w32 = nullptr;
+    // params.precond_fp32 = 2: a solve failed with the FP32 factors, and the context's
+    // solves use FP64 factors from then on (until the next ewg_set_params)
+    bool fp64_latched = false;
     // the Jacobian's blocks rounded to FP32 for BiCGStab's operator products
     // (params.operator_fp32), slot-major like J
     float *J32 = nullptr;

@@ -151,6 +151,8 @@ struct WsArgs {
     T *w;                 // forward result (forward layout): written forward, read backward
     double *out;          // backward: preconditioned vector, natural order
     const int *status;
+    const int *prec;      // null, or skip the launch unless *prec == want (the ensemble's
+    int want;             // per-member preconditioner precision: 0 FP32, 1 FP64)
     WfSync sync;
 };
 
@@ -205,6 +207,7 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     __shared__ int progress;              // compute steps completed (per PD steps)
     __shared__ unsigned s_tk;
     if (a.status && *a.status != KS_RUNNING) return;
+    if (a.prec && *a.prec != a.want) return;
     const int tid = threadIdx.x;
     if (tid == 0) s_tk = atomicAdd(a.sync.ticket, 1u);
     __syncthreads();
@@ -1044,6 +1047,7 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
         a.blk[0] = L; a.blk[1] = L + entries * 4; a.blk[2] = L + 2 * entries * 4;
         a.Dinv = nullptr; a.w = static_cast<T *>(m[i].w); a.out = nullptr;
         a.status = m[i].status; a.sync = m[i].sync;
+        a.prec = m[i].prec; a.want = m[i].want;
         f[i] = a;
         WsArgs<T> c = a;
         c.d = nullptr;

@@ -77,7 +77,8 @@ void wl_apply32(const Grid &g, const WfSync &sync, const WlPrecond<float> &wl, c
 // Several realizations' preconditioner applications v_i = (L_i U_i)^-1 d_i in one pair
 // of sweep launches (each realization with its own layout, tickets and edge buffers; its
 // blocks take tickets of their own realization, so the schedule inside each is the
-// single launch's). Realizations whose status is not running skip.
+// single launch's). Realizations whose status is not running skip, and so do those whose
+// precision flag (prec) differs from the batch's (want).
 struct WlMember {
     Grid g;
     WfSync sync;
@@ -90,6 +91,8 @@ struct WlMember {
     double *Af = nullptr;          // the realization's operand scratch (see wl_factor64)
     int *err = nullptr;            // the realization's factorization error flag
     const int *active = nullptr;   // factorize only when *active
+    const int *prec = nullptr;     // sweep only when *prec == want (null: always)
+    int want = 0;
 };
 void *wl_batch_create(const std::vector<WlMember> &members, bool fp32, hipStream_t s);
 void wl_batch_apply(void *batch, hipStream_t s);

@@ -87,7 +87,8 @@ class Stats(C.Structure):
                 ("last_newton_error", C.c_double),
                 ("time_linearize_s", C.c_double),
                 ("time_solve_s", C.c_double),
-                ("time_update_s", C.c_double)]
+                ("time_update_s", C.c_double),
+                ("precond_retries", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -27,7 +27,7 @@ class MemberStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("newton_iterations", C.c_int64),
                 ("linear_iterations", C.c_int64), ("linear_solves", C.c_int64),
                 ("status", C.c_int32), ("step_divisions", C.c_int32),
-                ("last_error", C.c_double)]
+                ("last_error", C.c_double), ("precond_retries", C.c_int32)]
 
 
 MAXC = 8
@@ -78,10 +78,11 @@ def _lib():
 
 
 def member_params(**overrides):
-    """Parameters of an ensemble member: FP32 preconditioner by default; with
-    ilu_schedule = 0 (default) the batched mode sweeps the members' tile-wavefront
+    """Parameters of an ensemble member: FP32 preconditioner by default, with a failed
+    solve redone, and the member's later solves made, with FP64 factors (precond_fp32 = 2);
+    with ilu_schedule = 0 (default) the batched mode sweeps the members' tile-wavefront
     layouts, with ilu_schedule = 1 (no tile buffers) their hyperplanes."""
-    kw = dict(newton_tolerance=1e-6, linear_tolerance=1e-3, precond_fp32=1, ilu_schedule=0)
+    kw = dict(newton_tolerance=1e-6, linear_tolerance=1e-3, precond_fp32=2, ilu_schedule=0)
     kw.update(overrides)
     return default_params(**kw)
 

@@ -96,7 +96,11 @@ typedef struct ewg_params {
     double min_time_step_size;         /* MinTimeStepSize = 0 */
     double max_time_step_size;         /* MaxTimeStepSize = inf */
     int32_t precond_fp32;              /* 0: FP64 ILU0 (library default); 1: FP32 ILU0 factors
-                                          and preconditioner inside FP64 BiCGStab */
+                                          and preconditioner inside FP64 BiCGStab; 2: the same,
+                                          and a linear solve that fails with them (not
+                                          converged, breakdown) is redone with FP64 factors,
+                                          which the context then keeps for its later solves
+                                          (until the next ewg_set_params) */
     int32_t ilu_schedule;              /* ILU(0) schedule on 3D grids (2D grids always use
                                           hyperplane launches); both give the same results:
                                           0 tile wavefront, 8x8 tiles with an edge wavefront
@@ -145,6 +149,7 @@ typedef struct ewg_stats {
     double time_linearize_s;
     double time_solve_s;
     double time_update_s;
+    int64_t precond_retries;        /* linear solves redone with FP64 factors (precond_fp32 = 2) */
 } ewg_stats;
 
 typedef struct ewg_ctx ewg_ctx;

@@ -51,6 +51,8 @@ typedef struct ewg_ensemble_member_stats {
     int32_t status;            /* 0 running; else the EWG_* code of the failed solve */
     int32_t step_divisions;    /* time steps halved after a failed solve */
     double last_error;         /* Newton error of the last solve */
+    int32_t precond_retries;   /* batched mode, params.precond_fp32 = 2: solves redone with
+                                  FP64 factors (the member's later solves use them too) */
 } ewg_ensemble_member_stats;
 
 #define EWG_ENSEMBLE_MAX_CANDIDATES 8

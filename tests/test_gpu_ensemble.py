@@ -242,3 +242,66 @@ def test_members_with_the_fp32_operator_are_refused():
         ensemble.Ensemble(ctxs)
     for c in ctxs:
         c.close()
+
+
+def _rough_member():
+    """Realization 21 of the seed-3 ensemble family (96x96x48, sigma_lnK 1.87, correlation
+    length 4.7 cells): with FP32 factors one of its 40 solves fails (BiCGStab stalls at
+    1,000 iterations), with FP64 factors none does."""
+    from ewg import problems
+    return problems.ensemble_member(21, 3, steps=40)
+
+
+def test_fp64_retry_of_a_failed_fp32_solve_single_context():
+    """precond_fp32 = 2: the solve that fails with FP32 factors is redone with FP64 factors
+    (kept for the later solves), and the 40 steps need no halving; with precond_fp32 = 1
+    the same failure halves a step."""
+    import ewg
+    from ewg import ensemble
+    P = _rough_member()
+    out = {}
+    for level in (1, 2):
+        c = ewg.GpuContext(P, ensemble.member_params(max_time_step_size=P.dt, precond_fp32=level))
+        n = c.run(0.0, 40 * P.dt, P.dt)
+        out[level] = (n, c.stats())
+        print(f"precond_fp32={level}: {n} steps, {out[level][1]}", flush=True)
+        c.close()
+    assert out[1][1]["failed_steps"] >= 1 and out[1][1]["precond_retries"] == 0
+    assert out[2][0] == 40 and out[2][1]["failed_steps"] == 0
+    assert out[2][1]["precond_retries"] == 1
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_fp64_retry_in_the_batched_ensemble(schedule):
+    """A member whose FP32-preconditioned solve fails redoes it, and its later solves, with
+    FP64 factors (the FP64 wavefront batches, or its hyperplane factors unrounded): no
+    halving, and it stays within the Newton tolerance's reach of a single context with
+    precond_fp32 = 2 (their BiCGStab dot products sum in different orders, and on this
+    field the FP32-preconditioned iterations amplify that); the other members match their
+    single contexts as without retries."""
+    import ewg
+    from ewg import ensemble
+    steps = 40
+    probs = [_rough_member()] + _problems(steps=steps)[:2]
+    prm = ensemble.member_params(precond_fp32=2, ilu_schedule=schedule)
+    ctxs = [ewg.GpuContext(P, prm) for P in probs]
+    ens = ensemble.Ensemble(ctxs)
+    ens.set_mode(ensemble.BATCHED, 512)
+    ens.run(probs[0].dt, steps)
+    st = [ens.stats(i) for i in range(len(probs))]
+    print(st, flush=True)
+    ref = _single_runs(probs, prm, steps)
+    for i, (its, kits, x) in enumerate(ref):
+        dS = np.abs(ctxs[i].solution(0)[1::2] - x[1::2]).max()
+        print(f"member {i}: newton {st[i]['newton_iterations']} vs {its}, bicgstab "
+              f"{st[i]['linear_iterations']} vs {kits}, max |dS| {dS:.2e}", flush=True)
+        assert st[i]["status"] == 0 and st[i]["steps"] == steps and st[i]["step_divisions"] == 0
+        if i == 0:
+            assert abs(st[i]["newton_iterations"] - its) <= 3
+            assert dS < 1e-5
+        else:
+            assert st[i]["newton_iterations"] == its
+            assert abs(st[i]["linear_iterations"] - kits) <= 2
+            assert dS < 1e-7
+    assert st[0]["precond_retries"] == 1
+    assert st[1]["precond_retries"] == 0 and st[2]["precond_retries"] == 0
