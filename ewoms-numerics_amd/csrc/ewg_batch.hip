@@ -917,6 +917,7 @@ struct ewg_ensemble {
     int *act64 = nullptr;
     bool any64 = false;
     bool regraph = false;   // the iteration graph must be captured again (any64 changed)
+    bool use_graph = true;  // replay the iteration graph (else launch the iteration's kernels)
     int *h_fp32 = nullptr;  // pinned 0 and 1: a member's Member::fp32 on the hyperplane path
     // per member: set when one of its tile-wavefront waits (sweeps or factorization) ran
     // out; turned into a Krylov breakdown of that member
@@ -1123,13 +1124,10 @@ void launch_apply(ewg_ensemble *e, int src, int dst) {
         hipLaunchKernelGGL(k_b_bwd, dim3(e->nblkL), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, l, dst);
 }
 
-// one BiCGStab iteration of every member (bicgstabsolver.hh:191-322), captured once
-int iteration_graph(ewg_ensemble *e) {
-    if (e->iter_graph) return EWG_OK;
+// one BiCGStab iteration (bicgstabsolver.hh:191-322) of every member whose solve runs
+void enqueue_iteration(ewg_ensemble *e) {
     const int n = static_cast<int>(e->members.size());
     const Maps mp = maps(e);
-    hipGraph_t gr;
-    ENS_TRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_b_ks_beta, dim3(n), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2);
     hipLaunchKernelGGL(k_b_p_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks);
     launch_apply(e, V_P, V_Y);
@@ -1146,6 +1144,14 @@ int iteration_graph(ewg_ensemble *e) {
     hipLaunchKernelGGL(k_b_xr_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2, e->partAny);
     hipLaunchKernelGGL(k_b_ks_crit2, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->partAny,
                        e->sweep_err);
+}
+
+// the same, captured once in a HIP graph
+int iteration_graph(ewg_ensemble *e) {
+    if (e->iter_graph) return EWG_OK;
+    hipGraph_t gr;
+    ENS_TRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_iteration(e);
     hipError_t h = hipStreamEndCapture(e->stream, &gr);
     ENS_TRY(e, h);
     h = hipGraphInstantiate(&e->iter_graph, gr, nullptr, nullptr, 0);
@@ -1286,7 +1292,7 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
     hipLaunchKernelGGL(k_b_ks_idle, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, e->ks);
     while (any_member) {
         // ------------------------------------------------------------- queue this round
-        if (e->regraph || !e->iter_graph) {
+        if (e->use_graph && (e->regraph || !e->iter_graph)) {
             // one BiCGStab iteration of every member, now with (or without) the FP64
             // batches' sweeps
             if (e->iter_graph) (void)hipGraphExecDestroy(e->iter_graph);
@@ -1399,7 +1405,12 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
         }
         ENS_TRY(e, hipGetLastError());
         if (solving)
-            for (int q = 0; q < e->replay; ++q) ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
+            for (int q = 0; q < e->replay; ++q) {
+                if (e->use_graph)
+                    ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
+                else
+                    enqueue_iteration(e);
+            }
         if (lin) ENS_TRY(e, hipMemcpyAsync(e->h_res, e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
         if (upd)
             ENS_TRY(e, hipMemcpyAsync(e->h_res_upd, e->pres_upd, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1759,6 +1770,12 @@ int ewg_ensemble_set_mode(ewg_ensemble *e, int32_t mode, int32_t threads) {
         return set_err(e, EWG_INVALID_ARGUMENT, "mode / threads");
     e->mode = mode;
     e->threads = threads;
+    return EWG_OK;
+}
+
+int ewg_ensemble_set_graph(ewg_ensemble *e, int32_t on) {
+    if (!e || (on != 0 && on != 1)) return EWG_INVALID_ARGUMENT;
+    e->use_graph = on != 0;
     return EWG_OK;
 }
 

@@ -63,6 +63,7 @@ def _lib():
             "ewg_ensemble_tune": (C.c_int, [_vp, C.c_double, C.POINTER(Tuning)]),
             "ewg_ensemble_tune_modes": (C.c_int, [_vp, C.c_double, C.c_int32, C.POINTER(Tuning)]),
             "ewg_ensemble_set_replay": (C.c_int, [_vp, C.c_int32]),
+            "ewg_ensemble_set_graph": (C.c_int, [_vp, C.c_int32]),
             "ewg_ensemble_set_mode": (C.c_int, [_vp, C.c_int32, C.c_int32]),
             "ewg_ensemble_run": (C.c_int, [_vp, C.c_double, C.c_int32,
                                            C.POINTER(C.c_double)]),
@@ -136,6 +137,11 @@ class Ensemble:
     def set_mode(self, mode, threads=512):
         self._check(self.lib.ewg_ensemble_set_mode(self.h, mode, threads))
 
+    def set_graph(self, on):
+        """Batched mode: replay the BiCGStab iteration from a HIP graph (True, default) or
+        launch its kernels each time (False)."""
+        self._check(self.lib.ewg_ensemble_set_graph(self.h, 1 if on else 0))
+
     def set_replay(self, depth):
         """Batched mode: most BiCGStab iterations queued between state reads."""
         self._check(self.lib.ewg_ensemble_set_replay(self.h, depth))
@@ -171,6 +177,9 @@ def main(argv=None):
     ap.add_argument("--threads", type=int, default=512)
     ap.add_argument("--replay", type=int, default=0,
                     help="batched mode: BiCGStab iterations queued per round (0: the default)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched mode: launch the BiCGStab iteration's kernels instead of "
+                         "replaying its graph (for kernel tracers)")
     ap.add_argument("--ilu-schedule", type=int, default=0, choices=(0, 1),
                     help="0: tile-wavefront sweeps in the batched mode; 1: hyperplane sweeps")
     args = ap.parse_args(argv)
@@ -185,6 +194,8 @@ def main(argv=None):
         ens.set_mode(BATCHED if args.mode == "batched" else PERSISTENT, args.threads)
         if args.replay:
             ens.set_replay(args.replay)
+        if args.no_graph:
+            ens.set_graph(False)
         tuning = {"mode": args.mode, "threads": args.threads, "replay": args.replay,
                   "candidates": []}
     t2 = time.perf_counter()

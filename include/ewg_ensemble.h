@@ -93,9 +93,13 @@ int ewg_ensemble_tune_modes(ewg_ensemble *ens, double dt, int32_t modes,
                             ewg_ensemble_tuning *out);
 /* Choose the mode without tuning (threads: 256, 512 or 1024 for the persistent mode). */
 int ewg_ensemble_set_mode(ewg_ensemble *ens, int32_t mode, int32_t threads);
-/* The batched mode's BiCGStab replay depth (1 to 64; default 8): results do not depend on
- * it, only the number of host reads and of iterations queued past convergence. */
+/* The batched mode's BiCGStab replay depth (1 to 64; default 8): the iterations queued
+ * between two host reads of the members' states. Results do not depend on it. */
 int ewg_ensemble_set_replay(ewg_ensemble *ens, int32_t depth);
+/* The batched mode replays its BiCGStab iteration from a HIP graph (on = 1, the default)
+ * or launches the iteration's kernels each time (on = 0). Results do not depend on it;
+ * the second form is for tools that follow each launch. */
+int ewg_ensemble_set_graph(ewg_ensemble *ens, int32_t on);
 int ewg_ensemble_run(ewg_ensemble *ens, double dt, int32_t steps, double *seconds);
 int ewg_ensemble_stats(const ewg_ensemble *ens, int32_t i,
                        ewg_ensemble_member_stats *out);
@@ -103,8 +107,9 @@ int ewg_ensemble_stats(const ewg_ensemble *ens, int32_t i,
 /* Test hook of the failure path (no counterpart in the reference): the next `solves`
  * linear solves of member i in the batched mode start their preconditioner sweeps with
  * the member's wavefront-wait error flag already set, as if one of its waits had run
- * out. The member's solve then breaks down (EWG_NUMERICAL_ISSUE, retried with a halved
- * step), its tile edge buffers are reset, and the other members are unaffected. */
+ * out. The member's solve then breaks down (EWG_NUMERICAL_ISSUE; redone with FP64 factors
+ * when the member has params.precond_fp32 = 2, else retried with a halved step), its tile
+ * edge buffers are reset, and the other members are unaffected. */
 int ewg_ensemble_force_sweep_failure(ewg_ensemble *ens, int32_t i, int32_t solves);
 
 #ifdef __cplusplus

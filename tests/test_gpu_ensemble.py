@@ -200,7 +200,7 @@ def test_forced_sweep_failure_is_isolated_and_recovers():
     import ewg
     from ewg import ensemble
     probs = _problems()
-    prm = ensemble.member_params()
+    prm = ensemble.member_params(precond_fp32=1)
     ctxs = [ewg.GpuContext(P, prm) for P in probs]
     ens = ensemble.Ensemble(ctxs)
     ens.set_mode(ensemble.BATCHED, 512)
@@ -305,3 +305,47 @@ def test_fp64_retry_in_the_batched_ensemble(schedule):
             assert dS < 1e-7
     assert st[0]["precond_retries"] == 1
     assert st[1]["precond_retries"] == 0 and st[2]["precond_retries"] == 0
+
+
+def test_forced_sweep_failure_is_redone_with_fp64_factors():
+    """With precond_fp32 = 2 the forced breakdown of member 1's first solve is redone with
+    FP64 factors (kept for its later solves) instead of halving the step: two whole steps,
+    one retry, and the member close to a single context with FP64 factors from the start
+    (its later solves are the same; its first solve stops at another iterate)."""
+    import ewg
+    from ewg import ensemble
+    probs = _problems()
+    prm = ensemble.member_params()
+    ctxs = [ewg.GpuContext(P, prm) for P in probs]
+    ens = ensemble.Ensemble(ctxs)
+    ens.set_mode(ensemble.BATCHED, 512)
+    ens.force_sweep_failure(1, 1)
+    ens.run(probs[0].dt, 2)
+    st = [ens.stats(i) for i in range(4)]
+    print(st, flush=True)
+    assert st[1]["status"] == 0 and st[1]["steps"] == 2 and st[1]["step_divisions"] == 0
+    assert st[1]["precond_retries"] == 1
+    assert all(st[i]["precond_retries"] == 0 for i in (0, 2, 3))
+    (its, _, x), = _single_runs([probs[1]], ensemble.member_params(precond_fp32=0), 2)
+    dS = np.abs(ctxs[1].solution(0)[1::2] - x[1::2]).max()
+    print(f"member 1 vs the FP64-preconditioned single context: max |dS| {dS:.2e}", flush=True)
+    assert dS < 1e-6
+
+
+def test_graph_and_direct_launches_agree():
+    """The batched run replaying its iteration graph and launching the iteration's kernels
+    directly give identical states and counts."""
+    import ewg
+    from ewg import ensemble
+    probs = _problems()
+    out = []
+    for graph in (True, False):
+        ctxs = [ewg.GpuContext(P, ensemble.member_params()) for P in probs]
+        ens = ensemble.Ensemble(ctxs)
+        ens.set_mode(ensemble.BATCHED, 512)
+        ens.set_graph(graph)
+        ens.run(probs[0].dt, 2)
+        out.append(([ens.stats(i) for i in range(4)], [c.solution(0) for c in ctxs]))
+    for i in range(4):
+        assert out[0][0][i] == out[1][0][i]
+        assert np.array_equal(out[0][1][i], out[1][1][i])
