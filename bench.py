@@ -505,11 +505,12 @@ def main():
 
 def ensemble_leg(args):
     """The config-3 ensemble: realizations of the same physics in the sizes of
-    problems.ENSEMBLE_SIZES (seed 3), FP64 state with the FP32 preconditioner, Newton
-    tolerance 1e-6, dt 500 s, `steps` steps, all members time-stepped together in the
-    batched mode; the tuner picks the batched mode's BiCGStab replay depth on one trial
-    step first (the persistent mode's trials, one step of every member in one workgroup
-    each, take seconds apiece and are left to `python -m ewg.ensemble --tune`)."""
+    problems.ENSEMBLE_SIZES (seed 3), FP64 state with the FP32 preconditioner (a solve that
+    fails with it is redone, and the member's later solves made, with FP64 factors),
+    Newton tolerance 1e-6, dt 500 s, `steps` steps, all members time-stepped in the batched
+    mode, each at its own pace; the tuner picks the batched mode's BiCGStab replay depth on
+    one trial step first (the persistent mode's trials, one step of every member in one
+    workgroup each, take seconds apiece and are left to `python -m ewg.ensemble --tune`)."""
     import numpy as np
     from ewg import ensemble, problems
     t0 = time.perf_counter()
@@ -529,6 +530,7 @@ def ensemble_leg(args):
            "seconds": sec, "completed_steps": int(done.sum()),
            "failed_members": int(sum(1 for x in st if x["status"] != 0)),
            "step_divisions": int(sum(x["step_divisions"] for x in st)),
+           "precond_retries": int(sum(x["precond_retries"] for x in st)),
            "newton_iterations": int(sum(x["newton_iterations"] for x in st)),
            "linear_iterations": int(sum(x["linear_iterations"] for x in st)),
            "sizes": {f"{a}x{b}x{c}": int(sum(1 for P in probs if (P.nx, P.ny, P.nz) == (a, b, c)))
@@ -536,7 +538,8 @@ def ensemble_leg(args):
            "mode": "batched", "tuning": tuning, "setup_s": round(t1 - t0, 2),
            "tune_s": round(t2 - t1, 2),
            "workload": f"{len(probs)} realizations, {int(cells.sum())} cells, FP64 state, FP32 "
-                       f"ILU(0) preconditioner, Newton tol 1e-6, dt {dt:g} s, seed 3"}
+                       f"ILU(0) preconditioner (a failed solve redone with FP64 factors), "
+                       f"Newton tol 1e-6, dt {dt:g} s, seed 3"}
     states = {i: ctxs[i].solution(0) for i in range(len(probs))}
     ens.close()
     for c in ctxs:
