@@ -44,8 +44,8 @@ int set_err(ewg_ctx *c, int code, const std::string &msg) {
 // loads the library holds its own copy until the process ends), split into fixed regions
 // so that the edge words always live in the edge region:
 //   [0, 256)                 the two tile tickets of the sweeps
-//   [256, 256 + 16 MiB)      both edge buffers of the tile sweeps (the sentinel between
-//                            calls; the sweeps restore it word by word)
+//   [256, 256 + 16 MiB)      both edge buffers of the tile sweeps (each call fills the
+//                            part its grid uses with the sentinel first)
 //   [.., .. + 32 MiB)        the forward sweep's intermediate vector
 // Host code never passes `scratch` itself to a kernel: it is a host-side stand-in there;
 // the device address comes from hipGetSymbolAddress once per device (noalloc_setup).
@@ -326,9 +326,10 @@ void bicgstab_begin(ewg_ctx *c, const double *b, double *x, const int *gate) {
 }
 
 // Second part, after the host has waited for the first: the remaining iterations are
-// queued speculatively and the state read back after 2, 4, then every 8 of them (the
+// queued speculatively and the state read back after 1, 2, 4, then every 8 of them (the
 // iterations queued past convergence are no-ops, so the count kept on the device is the
-// one of a solver that stops exactly). Returns EWG_OK, EWG_NOT_CONVERGED or
+// one of a solver that stops exactly; most solves of a time step after the first end
+// within their second iteration, so the first batch is a single one). Returns EWG_OK, EWG_NOT_CONVERGED or
 // EWG_NUMERICAL_ISSUE; *iters as BiCGStab's report.
 int bicgstab_end(ewg_ctx *c, const double *b, double *x, int *iters) {
     hipStream_t s = c->stream;
@@ -346,7 +347,7 @@ int bicgstab_end(ewg_ctx *c, const double *b, double *x, int *iters) {
     if (c->prm.linear_max_iterations <= 0)
         return set_err(c, EWG_NOT_CONVERGED, "linear solver did not converge");
     unsigned *tk = layout_tickets(c);
-    int it = 1, batch = 2;
+    int it = 1, batch = 1;
     while (true) {
         if (c->err_host[2]) {
             // a tile-wavefront wait ran out: its edge buffers may hold unconsumed words
@@ -587,7 +588,7 @@ int newton_step(ewg_ctx *c, double dt) {
 size_t wl_entry(const Grid &g, const WfSync &w, int x, int y, int z) {
     const int lx = x & 7, ly = y & 7;
     const size_t tile = (size_t)(x >> 3) + (size_t)w.ntx * (y >> 3);
-    return (tile * (size_t)(g.nz + 14) + (size_t)(z + lx + ly)) * 64 + (size_t)(lx + 8 * ly);
+    return (tile * (size_t)wl_nsteps(g.nz) + (size_t)(z + lx + ly)) * 64 + (size_t)(lx + 8 * ly);
 }
 
 // The factors the sweeps read, widened to double, as natural-order slot arrays:
@@ -785,7 +786,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
             w.ntiles = w.ntx * w.nty;
             ALLOC(ticket, 2);
             ALLOC(dorder, w.ntiles);
-            w.edge_words = (size_t)w.ntiles * T * g.nz * kWfMaxWords;
+            w.edge_words = (size_t)w.ntiles * T * wf_edge_slots(g.nz) * kWfMaxWords;
             ALLOC(ex, w.edge_words);
             ALLOC(ey, w.edge_words);
             std::vector<int2> order;
@@ -1168,6 +1169,9 @@ int ewg_apply_preconditioner_noalloc(ewg_ctx *c, const double *d, double *v) {
         w.ey = w.ex + c->wf.edge_words;
         w.err = c->ilu_err;
         launch_flag_clear(c->ilu_err, s);
+        // the sweeps leave values in the spare slots of their edge lanes, which another
+        // grid's edge lanes may use as value slots: the sentinel first, in the part used
+        wf_init_edges(w, s);
         if (fp32) {
             WlPrecond<float> wl = c->wl32;
             wl.w = reinterpret_cast<float *>(base + kScratchTickets + kScratchEdges);

@@ -14,13 +14,16 @@
 namespace ewg {
 namespace cell {
 
+// The products and sums of dune's dense loops (each entry: a_i0 b_0j, then + a_i1 b_1j)
+// without their initial zero: 0 + x equals x except that it turns -0 into +0, so the
+// results are the same numbers, the sign of a zero aside.
 template <class T>
 __device__ __forceinline__ void bmul(const T *a, const T *b, T *out) {
-    T r0 = T(0), r1 = T(0), r2 = T(0), r3 = T(0);
-    r0 += a[0] * b[0]; r0 += a[1] * b[2];
-    r1 += a[0] * b[1]; r1 += a[1] * b[3];
-    r2 += a[2] * b[0]; r2 += a[3] * b[2];
-    r3 += a[2] * b[1]; r3 += a[3] * b[3];
+    T r0 = a[0] * b[0], r1 = a[0] * b[1], r2 = a[2] * b[0], r3 = a[2] * b[1];
+    r0 += a[1] * b[2];
+    r1 += a[1] * b[3];
+    r2 += a[3] * b[2];
+    r3 += a[3] * b[3];
     out[0] = r0; out[1] = r1; out[2] = r2; out[3] = r3;
 }
 
@@ -33,10 +36,10 @@ __device__ __forceinline__ void bmmv(const T *a, const T *x, T *y) {  // y -= a 
 }
 
 template <class T>
-__device__ __forceinline__ void bmv(const T *a, const T *x, T *y) {  // y = a x
-    T s0 = T(0), s1 = T(0);
-    s0 += a[0] * x[0]; s0 += a[1] * x[1];
-    s1 += a[2] * x[0]; s1 += a[3] * x[1];
+__device__ __forceinline__ void bmv(const T *a, const T *x, T *y) {  // y = a x (as bmul)
+    T s0 = a[0] * x[0], s1 = a[2] * x[0];
+    s0 += a[1] * x[1];
+    s1 += a[3] * x[1];
     y[0] = s0; y[1] = s1;
 }
 

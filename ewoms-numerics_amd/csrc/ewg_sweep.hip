@@ -11,9 +11,13 @@
 //     predication of memory operations (this needs N * 32 bytes plus the skew margin
 //     below 2^31, which the kWfMaxCells limit guarantees);
 //   - no predication of the block products either: blocks of absent neighbours are
-//     stored as zero, and the neighbour values that enter them are selected to zero;
-//   - the in-tile exchange of step results through a double-buffered LDS array
-//     (one store and two loads per step).
+//     stored as zero, and the neighbour values that enter them are finite (the backward
+//     sweep's -z value of a column's first cell is selected to zero);
+//   - no test on the edge stores: the producing lanes store every step's result, the
+//     steps outside their column into spare slots of the edge buffers;
+//   - the in-tile exchange of step results through LDS (one store and two loads per
+//     step), and one comparison per edge direction to see whether a step's upstream
+//     values are in place.
 // The arithmetic per row is the one of the hyperplane kernels (ascending columns, no
 // fused multiply-add), so both schedules agree to the last bit up to the sign of zeros.
 #include <vector>
@@ -95,17 +99,19 @@ __device__ __forceinline__ EW<T> lde(__amdgpu_buffer_rsrc_t r, unsigned off) {
     }
     return e;
 }
+// soff: a wave-uniform part of the offset (the scalar offset operand)
 template <class T>
-__device__ __forceinline__ void ste(__amdgpu_buffer_rsrc_t r, unsigned off, const EW<T> &e) {
+__device__ __forceinline__ void ste(__amdgpu_buffer_rsrc_t r, unsigned off, const EW<T> &e,
+                                    unsigned soff = 0) {
     if constexpr (sizeof(T) == 8) {
         u32x4 v;
         v.x = (unsigned)e.w[0]; v.y = (unsigned)(e.w[0] >> 32);
         v.z = (unsigned)e.w[1]; v.w = (unsigned)(e.w[1] >> 32);
-        __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, kCoh);
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, off, (int)soff, kCoh);
     } else {
         u32x2 v;
         v.x = (unsigned)e.w[0]; v.y = (unsigned)(e.w[0] >> 32);
-        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kCoh);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, (int)soff, kCoh);
     }
 }
 template <class T>
@@ -143,7 +149,7 @@ __device__ __forceinline__ void mmv(const T *m, P2<T> v, T &r0, T &r1) {
 template <class T>
 struct WsArgs {
     int nx, ny, nz, nxny, N;
-    int nsteps;           // local steps per tile: 14 + nz
+    int nsteps;           // local steps per tile: wl_nsteps(nz), a multiple of kRing
     unsigned wl_bytes_blk, wl_bytes_vec;  // byte sizes of one layout array of blocks / pairs
     const double *d;      // forward: right-hand side, natural order
     const T *blk[3];      // forward: L (-z,-y,-x); backward: U (+x,+y,+z)
@@ -183,6 +189,7 @@ struct In {
 // tile hand-over 1-2 us slower: the outbox round trip and that wave's polling.)
 // ----------------------------------------------------------------------------------
 constexpr int kRing = 16;  // ring slots per edge lane (z modulo kRing)
+constexpr int kSweepThreads = 128;  // compute wave and edge wave
 constexpr unsigned kLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
 
 __device__ __forceinline__ int lds_ld(const int *p) {
@@ -191,6 +198,15 @@ __device__ __forceinline__ int lds_ld(const int *p) {
 __device__ __forceinline__ void lds_st(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+
+// Buffer descriptors kept opaque to the compiler: every descriptor of a launch has the
+// same flags word and several share their size word, and the compiler otherwise keeps
+// only one copy of such words and rebuilds the four-register descriptors before each use
+// (several scalar moves per step of the sweeps and the factorization).
+__device__ __forceinline__ void pin(__amdgpu_buffer_rsrc_t &r) { asm volatile("" : "+s"(r)); }
+
+// value of the edge wave's counts that never holds up a step
+constexpr int kReady = 1 << 30;
 
 template <class T, bool FWD, int PD>
 __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
@@ -201,9 +217,13 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     // tile write the next step's slot of their +x / +y neighbour (the last column and row
     // write into spare columns), the edge wave the slots of the first column / row.
     static_assert(kRing % PD == 0 && PD < kRing, "prefetch depth must divide the ring");
+    static_assert(kRing == 16, "wl_nsteps rounds the tile's steps to the unroll of 16");
     __shared__ P2<T> xv[kRing][NT + 1];
     __shared__ P2<T> yv[kRing][NT + TX];
-    __shared__ int filled[16];            // values of each edge lane in place (z < filled)
+    // per edge lane e (x edge: lane (0, e); y edge: lane (e - 8, 0)): the count of its
+    // values in place plus e & 7 (the skew of the lane that uses them), kReady once all
+    // are; [16]: kReady, read by the lanes off the first column / row
+    __shared__ int filled[17];
     __shared__ int progress;              // compute steps completed (per PD steps)
     __shared__ unsigned s_tk;
     if (a.status && *a.status != KS_RUNNING) return;
@@ -215,32 +235,37 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     if (tk >= (unsigned)a.sync.ntiles) return;
     const int2 tile = a.sync.order[tk];
     const int tx = tile.x, ty = tile.y;
-    const int nz = a.nz;
+    const int nz = a.nz, nzp = wf_edge_slots(nz);
     const int tileId = ty * a.sync.ntx + tx;
     // slots without an upstream tile read zeros for good (the matching blocks are zero as
     // well) and their edge lanes count as complete
-    for (int i = tid; i < kRing * (NT + 1); i += 2 * NT) (&xv[0][0])[i] = P2<T>{T(0), T(0)};
-    for (int i = tid; i < kRing * (NT + TX); i += 2 * NT) (&yv[0][0])[i] = P2<T>{T(0), T(0)};
+    for (int i = tid; i < kRing * (NT + 1); i += kSweepThreads) (&xv[0][0])[i] = P2<T>{T(0), T(0)};
+    for (int i = tid; i < kRing * (NT + TX); i += kSweepThreads) (&yv[0][0])[i] = P2<T>{T(0), T(0)};
     if (tid < 16) {
         const int dir = tid >> 3, e = tid & 7;
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + e < a.ny) : (ty > 0 && tx * TX + e < a.nx);
-        filled[tid] = valid ? 0 : nz;
+        filled[tid] = valid ? e : kReady;
     }
+    if (tid == 16) filled[16] = kReady;
     if (tid == 0) progress = 0;
     __syncthreads();
     constexpr unsigned zw = kWfMaxWords * 8;  // bytes per z of one edge lane
-    const __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
-    const __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
+    __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
+    __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
+    pin(rex);
+    pin(rey);
 
     if (tid >= NT) {
         // ---------------------------------------------------------------- edge wave
+        // lanes e + 16 j (j = 0..3) serve edge lane e: e < 8 the x edge (the -x tile's
+        // last column), e >= 8 the y edge (the -y tile's last row)
         const int l = tid - NT, e = l & 15, j = l >> 4;
         const int dir = e >> 3, idx = e & 7;
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + idx < a.ny) : (ty > 0 && tx * TX + idx < a.nx);
         const unsigned gbase =
             !valid ? 0u
-            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nz) * zw)
-                       : (unsigned)((((size_t)(tileId - a.sync.ntx) * TX + idx) * nz) * zw);
+            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nzp + kWfEdgeLo) * zw)
+                       : (unsigned)((((size_t)(tileId - a.sync.ntx) * TX + idx) * nzp + kWfEdgeLo) * zw);
         // value z of this edge lane is used by lane (0, idx) (x edge) or (idx, 0) (y edge)
         // at its step z + idx
         P2<T> *const ring = dir == 0 ? &xv[0][TX * idx] : &yv[0][idx];
@@ -280,7 +305,7 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 __builtin_amdgcn_s_waitcnt(kLgkm0);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                if (advance) lds_st(&filled[e], zmin);
+                if (advance) lds_st(&filled[e], zmin >= nz ? kReady : zmin + idx);
                 published = zmin > published ? zmin : published;
             }
             if (__builtin_amdgcn_ballot_w64(deliver) != 0ull) {
@@ -316,7 +341,7 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
         dz = -a.nxny;
     }
     const int cs0 = c0 - (lx + ly) * dz;  // cell of local step 0
-    const int nsteps = a.nsteps;
+    const int nsteps = a.nsteps;          // a multiple of kRing
     constexpr unsigned bB = 4 * sizeof(T), vB = 2 * sizeof(T);
     // wavefront-layout arrays: this lane's entry of local step 0, then 64 entries on
     const unsigned wl0 = (unsigned)((tileId * nsteps) * 64 + tid);
@@ -326,10 +351,10 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     unsigned pnat = inplane ? (unsigned)(cs0 * 16) : kFar;
     const unsigned snat = inplane ? (unsigned)(dz * 16) : 0u;
     unsigned onat = pnat;
-    // forward result: written in the forward layout by the forward sweep; the backward
-    // sweep reads entry (forward tile, z + lxf + lyf, forward lane) of its cell, which
-    // moves back by one row of 64 per step
-    unsigned ow = wl0 * vB;  // forward: store offset
+    // forward result: written in the forward layout by the forward sweep (every step the
+    // loop runs has its own entry); the backward sweep reads entry (forward tile,
+    // z + lxf + lyf, forward lane) of its cell, which moves back by one row of 64 per step
+    unsigned ow = wl0 * vB;  // forward: store offset of the chunk's first step
     unsigned pw = kFar;      // backward: load offset
     if (!FWD && inplane) {
         const int x = a.nx - 1 - xs, y = a.ny - 1 - ys;  // natural coordinates
@@ -338,15 +363,26 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
         const int tf = (y >> 3) * a.sync.ntx + (x >> 3);
         pw = (unsigned)(((tf * nsteps + z0 + lxf + lyf) * 64 + lxf + 8 * lyf) * (int)vB);
     }
-    constexpr unsigned sw = 64 * vB;  // backward: the forward entry moves back by sw
-    const __amdgpu_buffer_rsrc_t rb0 = rsrc(a.blk[0], a.wl_bytes_blk),
-                                 rb1 = rsrc(a.blk[1], a.wl_bytes_blk),
-                                 rb2 = rsrc(a.blk[2], a.wl_bytes_blk);
-    const __amdgpu_buffer_rsrc_t rdi = rsrc(a.Dinv, FWD ? 0 : a.wl_bytes_blk);
-    const __amdgpu_buffer_rsrc_t rw = rsrc(a.w, a.wl_bytes_vec);
+    constexpr unsigned sw = 64 * vB;  // the forward entry moves by sw per step
+    // this lane's edge slot of local step 0 (z = -lx - ly): the last column's lanes store
+    // to edge lane ly of the x buffer, the last row's to edge lane lx of the y buffer; the
+    // slot of step tau is tau slots on (the other lanes: out of range)
+    const unsigned eX = prodX ? (unsigned)((((size_t)tileId * 8 + ly) * nzp + kWfEdgeLo - lx - ly) * zw) : kFar;
+    const unsigned eY = prodY ? (unsigned)((((size_t)tileId * 8 + lx) * nzp + kWfEdgeLo - lx - ly) * zw) : kFar;
+    __amdgpu_buffer_rsrc_t rb0 = rsrc(a.blk[0], a.wl_bytes_blk), rb1 = rsrc(a.blk[1], a.wl_bytes_blk),
+                           rb2 = rsrc(a.blk[2], a.wl_bytes_blk);
+    __amdgpu_buffer_rsrc_t rdi = rsrc(a.Dinv, FWD ? 0 : a.wl_bytes_blk);
+    __amdgpu_buffer_rsrc_t rw = rsrc(a.w, a.wl_bytes_vec);
     const size_t N = (size_t)a.N;
-    const __amdgpu_buffer_rsrc_t rd = rsrc(a.d, FWD ? N * 16 : 0);
-    const __amdgpu_buffer_rsrc_t rout = rsrc(a.out, FWD ? 0 : N * 16);
+    __amdgpu_buffer_rsrc_t rd = rsrc(a.d, FWD ? N * 16 : 0);
+    __amdgpu_buffer_rsrc_t rout = rsrc(a.out, FWD ? 0 : N * 16);
+    pin(rb0);
+    pin(rb1);
+    pin(rb2);
+    pin(rdi);
+    pin(rw);
+    pin(rd);
+    pin(rout);
 
     In<T, FWD> ring[PD];
     auto load = [&](In<T, FWD> &in) {
@@ -373,28 +409,25 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     }
 
     P2<T> prev = P2<T>{T(0), T(0)};
-    // this tile's words in the edge buffers: its last column (edge lane ly) and last row
-    // (edge lane lx), z-value words consecutive
-    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nz) * zw);
-    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nz) * zw);
     // this lane's exchange columns, and the columns it writes (its +x / +y neighbour's, or
     // a spare column)
     const P2<T> *const rx = &xv[0][tid];
     const P2<T> *const ry = &yv[0][tid];
     P2<T> *const wx = lx < TX - 1 ? &xv[0][tid + 1] : &xv[0][NT];
     P2<T> *const wy = ly < TY - 1 ? &yv[0][tid + TX] : &yv[0][NT + lx];
-    const int *const fX = &filled[ly];
-    const int *const fY = &filled[8 + lx];
-    const bool edgeX = lx == 0, edgeY = ly == 0;
+    const int *const fX = lx == 0 ? &filled[ly] : &filled[16];
+    const int *const fY = ly == 0 ? &filled[8 + lx] : &filled[16];
 
     // The counts and the neighbour values of a step are read at the end of the step
     // before, so that their LDS latency overlaps that step's tail and the check; the
     // counts come first (a ring value read after a count that covers it is in place),
-    // and the rare wait re-reads the values.
+    // and the rare wait re-reads the values. Value z = tau - lx - ly of a first-column
+    // lane's edge lane is in place once that count exceeds tau (z < 0: from the start).
     int cntX = lds_ld(fX), cntY = lds_ld(fY);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     P2<T> nx_ = rx[0], ny_ = ry[0];
     for (int tau0 = 0; tau0 < nsteps; tau0 += kRing) {
+        const unsigned esoff = (unsigned)tau0 * zw;
 #pragma unroll
         for (int k = 0; k < kRing; ++k) {
             const int tau = tau0 + k;
@@ -405,17 +438,15 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 if (tid == 0) lds_st(&progress, tau);
             }
-            const int z = tau - lx - ly;
-            const bool zin = (unsigned)z < (unsigned)nz;
-            // this step's edge values must be in place: value z of the lane's edge
-            bool wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
-            if (__builtin_amdgcn_ballot_w64(wait) != 0ull) {
+            if ((__builtin_amdgcn_ballot_w64(cntX <= tau) | __builtin_amdgcn_ballot_w64(cntY <= tau)) !=
+                0ull) {
                 unsigned spins = 0;
+                bool wait;
                 do {
                     __builtin_amdgcn_s_sleep(1);
                     cntX = lds_ld(fX);
                     cntY = lds_ld(fY);
-                    wait = (edgeX & zin & (cntX <= z)) | (edgeY & zin & (cntY <= z));
+                    wait = cntX <= tau || cntY <= tau;
                     ++spins;
                     if ((spins & 63) == 0 &&
                         __hip_atomic_load(a.sync.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
@@ -426,9 +457,18 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 nx_ = rx[k * (NT + 1)];
                 ny_ = ry[k * (NT + TX)];
             }
-            P2<T> nz_;
-            nz_.x = z > 0 ? prev.x : T(0);
-            nz_.y = z > 0 ? prev.y : T(0);
+            // the -z neighbour: the lane's previous result. Forward: the steps before the
+            // column's first cell load a zero right-hand side (out of range) and zero
+            // blocks (entries without a cell), so their results are zeros and the first
+            // cell gets a zero -z value without a test (the sign of a zero aside).
+            // Backward: those steps read other entries of the forward result, so the
+            // first cell (z = 0: tau = lx + ly) takes an explicit zero.
+            P2<T> nz_ = prev;
+            if (!FWD) {
+                const bool zpos = tau > lx + ly;
+                nz_.x = zpos ? prev.x : T(0);
+                nz_.y = zpos ? prev.y : T(0);
+            }
             const In<T, FWD> &c = ring[k % PD];
             T r0 = c.rhs.x, r1 = c.rhs.y;
             P2<T> res;
@@ -441,18 +481,16 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 mmv(c.b[0], nx_, r0, r1);
                 mmv(c.b[1], ny_, r0, r1);
                 mmv(c.b[2], nz_, r0, r1);
-                T s0 = T(0), s1 = T(0);
-                s0 += c.di[0] * r0; s0 += c.di[1] * r1;
-                s1 += c.di[2] * r0; s1 += c.di[3] * r1;
+                T s0 = c.di[0] * r0, s1 = c.di[2] * r0;
+                s0 += c.di[1] * r1;
+                s1 += c.di[3] * r1;
                 res = P2<T>{s0, s1};
             }
             // the downstream tiles' values, then the result vector
-            ste<T>(rex, prodX && zin ? gX + (unsigned)z * zw : kFar, val_ew(res));
-            ste<T>(rey, prodY && zin ? gY + (unsigned)z * zw : kFar, val_ew(res));
+            ste<T>(rex, eX + (unsigned)k * zw, val_ew(res), esoff);
+            ste<T>(rey, eY + (unsigned)k * zw, val_ew(res), esoff);
             if (FWD) {
-                // the last block's steps past nsteps would land in the next tile's rows
-                stp<T>(rw, tau < nsteps ? ow : kFar, res);
-                ow += sw;
+                stp<T>(rw, ow + (unsigned)k * sw, res);
             } else {
                 stp<double>(rout, onat,
                             P2<double>{static_cast<double>(res.x), static_cast<double>(res.y)});
@@ -473,18 +511,19 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
             load(ring[k % PD]);
             __builtin_amdgcn_sched_barrier(0);
         }
+        if (FWD) ow += kRing * sw;
     }
 }
 
 template <class T, bool FWD, int PD>
-__global__ void __launch_bounds__(128) k_sweep_ws(WsArgs<T> a) {
+__global__ void __launch_bounds__(kSweepThreads) k_sweep_ws(WsArgs<T> a) {
     sweep_ws_body<T, FWD, PD>(a);
 }
 
 // the same sweep for several realizations in one launch: block -> realization by tmem,
 // each block then takes a tile ticket of its realization as the single launch does
 template <class T, bool FWD, int PD>
-__global__ void __launch_bounds__(128) k_sweep_ws_batch(const WsArgs<T> *args, const int *tmem) {
+__global__ void __launch_bounds__(kSweepThreads) k_sweep_ws_batch(const WsArgs<T> *args, const int *tmem) {
     const WsArgs<T> a = args[tmem[blockIdx.x]];
     sweep_ws_body<T, FWD, PD>(a);
 }
@@ -523,7 +562,7 @@ __global__ void k_reset_ticket_pair(unsigned *ticket) {
 template <class T>
 struct FwsArgs {
     int nx, ny, nz, nxny, N, diag;
-    int nsteps;            // local steps per tile: 14 + nz
+    int nsteps;            // local steps per tile: wl_nsteps(nz)
     const double *A;       // Jacobian, slot-major [2 diag + 1][N][4]
     double *Af;            // forward layout, 7 arrays of entries x 4: A_ii, A_ij (-z, -y,
                            // -x), A_ji (the -z, -y, -x neighbour's block towards the cell)
@@ -596,7 +635,7 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     if (tk >= (unsigned)a.sync.ntiles) return;
     const int2 tile = a.sync.order[tk];
     const int tx = tile.x, ty = tile.y;
-    const int nz = a.nz, nsteps = a.nsteps;
+    const int nz = a.nz, nsteps = a.nsteps, nzp = wf_edge_slots(nz);
     const int ntx = a.sync.ntx;
     const int tileId = ty * ntx + tx;
     for (int i = tid; i < 2 * 8 * kRing; i += 2 * NT) (&ering[0][0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
@@ -610,8 +649,10 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     __syncthreads();
     constexpr unsigned zw = kWfMaxWords * 8;
     static_assert(kWfMaxWords == 4, "a Dinv block takes four edge words");
-    const __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
-    const __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
+    __amdgpu_buffer_rsrc_t rex = rsrc(a.sync.ex, a.sync.edge_words * 8);
+    __amdgpu_buffer_rsrc_t rey = rsrc(a.sync.ey, a.sync.edge_words * 8);
+    pin(rex);
+    pin(rey);
     constexpr unsigned bB = 4 * sizeof(T);
     const unsigned blk_bytes = (unsigned)(a.entries * bB);
     const int wave = tid >> 6, l = tid & 63;
@@ -636,8 +677,8 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + idx < a.ny) : (ty > 0 && tx * TX + idx < a.nx);
         const unsigned gbase =
             !valid ? 0u
-            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nz) * zw)
-                       : (unsigned)((((size_t)(tileId - ntx) * TX + idx) * nz) * zw);
+            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nzp + kWfEdgeLo) * zw)
+                       : (unsigned)((((size_t)(tileId - ntx) * TX + idx) * nzp + kWfEdgeLo) * zw);
         B4 *ring = &ering[dir][idx][0];
         const unsigned long long sent[4] = {kSentinel, kSentinel, kSentinel, kSentinel};
         int z = valid ? j : nz;
@@ -698,15 +739,25 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     const bool prodY = inplane && ly == TY - 1 && ty + 1 < a.sync.nty;
     const size_t E4 = a.entries * 4;
     const unsigned af_bytes = (unsigned)(a.entries * 32);
-    const __amdgpu_buffer_rsrc_t rAii = rsrc(a.Af, af_bytes);
-    const __amdgpu_buffer_rsrc_t rAij0 = rsrc(a.Af + E4, af_bytes), rAij1 = rsrc(a.Af + 2 * E4, af_bytes),
-                                 rAij2 = rsrc(a.Af + 3 * E4, af_bytes);
-    const __amdgpu_buffer_rsrc_t rAji0 = rsrc(a.Af + 4 * E4, af_bytes),
-                                 rAji1 = rsrc(a.Af + 5 * E4, af_bytes),
-                                 rAji2 = rsrc(a.Af + 6 * E4, af_bytes);
-    const __amdgpu_buffer_rsrc_t rL0 = rsrc(a.L, blk_bytes), rL1 = rsrc(a.L + a.entries * 4, blk_bytes),
-                                 rL2 = rsrc(a.L + 2 * a.entries * 4, blk_bytes);
-    const __amdgpu_buffer_rsrc_t rD = rsrc(a.D, blk_bytes);
+    __amdgpu_buffer_rsrc_t rAii = rsrc(a.Af, af_bytes);
+    __amdgpu_buffer_rsrc_t rAij0 = rsrc(a.Af + E4, af_bytes), rAij1 = rsrc(a.Af + 2 * E4, af_bytes),
+                           rAij2 = rsrc(a.Af + 3 * E4, af_bytes);
+    __amdgpu_buffer_rsrc_t rAji0 = rsrc(a.Af + 4 * E4, af_bytes), rAji1 = rsrc(a.Af + 5 * E4, af_bytes),
+                           rAji2 = rsrc(a.Af + 6 * E4, af_bytes);
+    __amdgpu_buffer_rsrc_t rL0 = rsrc(a.L, blk_bytes), rL1 = rsrc(a.L + a.entries * 4, blk_bytes),
+                           rL2 = rsrc(a.L + 2 * a.entries * 4, blk_bytes);
+    __amdgpu_buffer_rsrc_t rD = rsrc(a.D, blk_bytes);
+    pin(rAii);
+    pin(rAij0);
+    pin(rAij1);
+    pin(rAij2);
+    pin(rAji0);
+    pin(rAji1);
+    pin(rAji2);
+    pin(rL0);
+    pin(rL1);
+    pin(rL2);
+    pin(rD);
     const unsigned wl0 = (unsigned)((tileId * nsteps) * 64 + l);
 
     FIn ring[PD];
@@ -731,8 +782,8 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     B4 prev = B4{{0.0, 0.0, 0.0, 0.0}};
     const bool hasL = lx > 0, hasD = ly > 0;
     // this tile's words in the edge buffers (last column: edge lane ly; last row: lx)
-    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nz) * zw);
-    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nz) * zw);
+    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nzp + kWfEdgeLo) * zw);
+    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nzp + kWfEdgeLo) * zw);
     const B4 *const eX = &ering[0][ly][0];
     const B4 *const eY = &ering[1][lx][0];
     const int *const fX = &filled[ly];
@@ -780,8 +831,8 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
             const double *nb[3] = {prev.a, ny_.a, nx_.a};
             // Branch-free: an absent neighbour's blocks are zero in the packed operands
             // (and so are all blocks of an entry without a cell), and every neighbour
-            // value is finite, so its L_ij is bmul's +0 and D - (+0) is D (also for D =
-            // -0): the same bits as skipping the neighbour.
+            // value is finite, so its L_ij is a zero and D minus it is D (a zero entry of D
+            // may change its sign): the same numbers as skipping the neighbour.
             double D[4] = {c.Aii[0], c.Aii[1], c.Aii[2], c.Aii[3]};
             double Lv[3][4];
 #pragma unroll
@@ -941,7 +992,7 @@ k_wl_pack_factor_batch(const FwsArgs<T> *args, const int *pmem, const int *poff)
 }  // namespace
 
 size_t wl_entries(const Grid &g, const WfSync &sync) {
-    return (size_t)sync.ntiles * (size_t)(g.nz + 14) * 64;
+    return (size_t)sync.ntiles * (size_t)wl_nsteps(g.nz) * 64;
 }
 
 template <class T>
@@ -950,21 +1001,21 @@ static void wl_apply(const Grid &g, const WfSync &sync, const WlPrecond<T> &wl, 
     const size_t entries = wl_entries(g, sync);
     WsArgs<T> f{};
     f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N;
-    f.nsteps = g.nz + 14;
+    f.nsteps = wl_nsteps(g.nz);
     f.wl_bytes_blk = (unsigned)(entries * 4 * sizeof(T));
     f.wl_bytes_vec = (unsigned)(entries * 2 * sizeof(T));
     f.d = d;
     f.blk[0] = wl.L; f.blk[1] = wl.L + entries * 4; f.blk[2] = wl.L + 2 * entries * 4;
     f.Dinv = nullptr; f.w = wl.w; f.out = nullptr; f.status = status; f.sync = sync;
     if (!tickets_clear) hipLaunchKernelGGL(k_reset_ticket_pair, dim3(1), dim3(1), 0, s, sync.ticket);
-    hipLaunchKernelGGL((k_sweep_ws<T, true, kSweepPD>), dim3(sync.ntiles), dim3(128), 0, s, f);
+    hipLaunchKernelGGL((k_sweep_ws<T, true, kSweepPD>), dim3(sync.ntiles), dim3(kSweepThreads), 0, s, f);
     WsArgs<T> b = f;
     b.d = nullptr;
     b.blk[0] = wl.U; b.blk[1] = wl.U + entries * 4; b.blk[2] = wl.U + 2 * entries * 4;
     b.Dinv = wl.D;
     b.out = v;
     b.sync.ticket = sync.ticket + 1;
-    hipLaunchKernelGGL((k_sweep_ws<T, false, kSweepPD>), dim3(sync.ntiles), dim3(128), 0, s, b);
+    hipLaunchKernelGGL((k_sweep_ws<T, false, kSweepPD>), dim3(sync.ntiles), dim3(kSweepThreads), 0, s, b);
 }
 
 void wl_apply64(const Grid &g, const WfSync &sync, const WlPrecond<double> &wl, const double *d,
@@ -981,7 +1032,7 @@ static FwsArgs<T> fws_args(const Grid &g, const WfSync &sync, const double *A, d
                            T *U, T *D, int *err) {
     FwsArgs<T> f{};
     f.nx = g.nx; f.ny = g.ny; f.nz = g.nz; f.nxny = g.nxny; f.N = g.N; f.diag = g.diag;
-    f.nsteps = g.nz + 14;
+    f.nsteps = wl_nsteps(g.nz);
     f.A = A; f.Af = Af; f.L = L; f.U = U; f.D = D;
     f.entries = wl_entries(g, sync);
     f.err = err;
@@ -1039,7 +1090,7 @@ static int wl_batch_fill(WlBatch &b, const std::vector<WlMember> &m, hipStream_t
         const size_t entries = wl_entries(g, m[i].sync);
         WsArgs<T> a{};
         a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nxny = g.nxny; a.N = g.N;
-        a.nsteps = g.nz + 14;
+        a.nsteps = wl_nsteps(g.nz);
         a.wl_bytes_blk = (unsigned)(entries * 4 * sizeof(T));
         a.wl_bytes_vec = (unsigned)(entries * 2 * sizeof(T));
         a.d = m[i].d;
@@ -1110,17 +1161,17 @@ void wl_batch_apply(void *h, hipStream_t s) {
     const dim3 rg((b->n + 255) / 256), rb(256);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_sweep_ws_batch<float, true, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, true, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
                            static_cast<const WsArgs<float> *>(b->fwd), b->tmem);
     else
-        hipLaunchKernelGGL((k_sweep_ws_batch<double, true, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, true, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
                            static_cast<const WsArgs<double> *>(b->fwd), b->tmem);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_sweep_ws_batch<float, false, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, false, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
                            static_cast<const WsArgs<float> *>(b->bwd), b->tmem);
     else
-        hipLaunchKernelGGL((k_sweep_ws_batch<double, false, kSweepPD>), dim3(b->ntiles), dim3(128), 0, s,
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, false, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
                            static_cast<const WsArgs<double> *>(b->bwd), b->tmem);
 }
 

@@ -20,6 +20,15 @@ constexpr int kSweepPD = EWG_SWEEP_PD;  // prefetch depth of the triangular swee
 constexpr int kSweepPE = 1;     // prefetch depth of their cross-tile words
 
 constexpr int kWfMaxWords = 4;  // 64-bit words per exchanged value (FP64 2x2 block)
+// The edge buffers hold, per tile and edge lane (8 for its last column, 8 for its last
+// row), one slot of kWfMaxWords words per z-value, with kWfEdgeLo slots before z = 0 and
+// kWfEdgeHi after z = nz - 1: the producing lanes store every step's result without a
+// test, and the steps outside their column land in these spare slots.
+constexpr int kWfEdgeLo = 14, kWfEdgeHi = 22;
+inline __host__ __device__ int wf_edge_slots(int nz) { return nz + kWfEdgeLo + kWfEdgeHi; }
+// local steps per tile in the wavefront layout: the tile's skew of 14 steps plus nz,
+// rounded up to the sweeps' unroll of 16 (every step the sweeps run has its own row)
+inline __host__ __device__ int wl_nsteps(int nz) { return (nz + 14 + 15) & ~15; }
 // largest grid the tile-wavefront kernels take: their 32-bit buffer offsets (32 bytes per
 // cell and array, plus a skew margin and the out-of-range base 2^31) must not wrap
 constexpr long long kWfMaxCells = 1LL << 25;
@@ -31,8 +40,8 @@ constexpr long long kWfMaxCells = 1LL << 25;
 struct WfSync {
     unsigned *ticket;          // two counters (the warp-specialized apply uses both)
     const int2 *order;         // [ntiles] tiles (tx, ty) in anti-diagonal order
-    unsigned long long *ex;    // [ntiles][TY][nz][kWfMaxWords] last-column values
-    unsigned long long *ey;    // [ntiles][TX][nz][kWfMaxWords] last-row values
+    unsigned long long *ex;    // [ntiles][TY][wf_edge_slots(nz)][kWfMaxWords] last-column values
+    unsigned long long *ey;    // [ntiles][TX][wf_edge_slots(nz)][kWfMaxWords] last-row values
     size_t edge_words;         // words in each of ex, ey
     int ntx, nty, ntiles;
     int *err;                  // set when a wait exceeded its spin limit
@@ -42,7 +51,7 @@ struct WfSync {
 void wf_init_edges(const WfSync &sync, hipStream_t s);
 
 // The ILU(0) factors in the wavefront layout of the 8x8 tiles of sync: entry
-// ((tile * (nz + 14) + step) * 64 + lane) holds the 2x2 block of the cell that lane of
+// ((tile * wl_nsteps(nz) + step) * 64 + lane) holds the 2x2 block of the cell that lane of
 // that tile reaches at that local step (L in forward order, U and Dinv in backward
 // order), so that each step of a tile reads one contiguous row of 64 blocks.
 template <class T>
