@@ -648,11 +648,14 @@ __device__ __forceinline__ void b_criterion(KrylovState &k, double err, int stag
     else if (failed) k.status = KS_FAIL;
 }
 
-__global__ void __launch_bounds__(BT) k_b_ks_beta(const Member *mem, Maps mp, KrylovState *ks, const double *partRho) {
+// rho_i: the solve start's r.r (k_b_ks_init), then the r_hat.r of the iteration before
+// (k_b_ks_crit2), each reduced in the launches that produced its partials, so that
+// successive iterations may run through different block maps
+__global__ void __launch_bounds__(BT) k_b_ks_beta(const Member *mem, KrylovState *ks) {
     const int id = blockIdx.x;
     KrylovState &k = ks[id];
     if (k.status != KS_RUN) return;
-    const double rho_i = k.iterations == 0 ? k.rho_i : range_sum(partRho, mp.coff[id], mp.coff[id + 1]);
+    const double rho_i = k.rho_i;
     if (threadIdx.x == 0) {
         if (k.iterations >= mem[id].lin_max_it) {
             k.status = KS_MAXIT;
@@ -796,8 +799,8 @@ __global__ void __launch_bounds__(BT) k_b_xr_update(const Member *mem, Maps mp, 
     }
 }
 
-__global__ void __launch_bounds__(BT) k_b_ks_crit2(Maps mp, KrylovState *ks, const int *partAny,
-                                                   const int *serr) {
+__global__ void __launch_bounds__(BT) k_b_ks_crit2(Maps mp, KrylovState *ks, const double *partRho,
+                                                   const int *partAny, const int *serr) {
     const int id = blockIdx.x;
     KrylovState &k = ks[id];
     if (k.status != KS_RUN) return;
@@ -806,10 +809,14 @@ __global__ void __launch_bounds__(BT) k_b_ks_crit2(Maps mp, KrylovState *ks, con
         return;
     }
     const int any = range_any(partAny, mp.coff[id], mp.coff[id + 1]);
+    const double rho_i = range_sum(partRho, mp.coff[id], mp.coff[id + 1]);
     if (threadIdx.x == 0) {
         // the residual checked here is still s (the object r is updated afterwards)
         b_criterion(k, k.resid_err, any == 0);
-        if (k.status == KS_RUN) k.iterations += 1;
+        if (k.status == KS_RUN) {
+            k.iterations += 1;
+            k.rho_i = rho_i;  // the next iteration's r_hat.r
+        }
     }
 }
 
@@ -890,7 +897,7 @@ struct ewg_ensemble {
     int mode = 1;  // 0: persistent (one workgroup per member), 1: batched launches
     int threads = 512;
     bool largest_first = true;
-    int replay = 8;  // batched mode: most BiCGStab iterations queued between state reads
+    int replay = 4;  // batched mode: most BiCGStab iterations queued between state reads
     hipGraphExec_t graph = nullptr;
     int graph_steps = -1, graph_threads = 0;
     bool graph_largest = true;
@@ -932,6 +939,19 @@ struct ewg_ensemble {
     int *h_mask = nullptr, *h_it = nullptr, *h_serr = nullptr;
     double *h_dts = nullptr, *h_res = nullptr, *h_res_upd = nullptr;
     KrylovState *h_ks = nullptr;
+    // Compact rounds (tile-wavefront path): when the members whose solve runs hold at most
+    // half of the cell blocks, a round's BiCGStab iterations launch over their blocks and
+    // tiles only, through maps of these members (device cc_*, pinned host h_cc_*, rebuilt
+    // when the set changes) instead of the full maps and the iteration graph, whose
+    // launches also dispatch every block of the members that are done.
+    int compact = 1;                     // 0: never, 1: at most half of the blocks, 2: always
+    std::vector<int> nblk_of, ntile_of;  // per member: cell blocks, tiles
+    int ntile_all = 0;
+    int *cc_mem = nullptr, *cc_off = nullptr, *cc_t32 = nullptr, *cc_t64 = nullptr;
+    int *h_cc_mem = nullptr, *h_cc_off = nullptr, *h_cc_t32 = nullptr, *h_cc_t64 = nullptr;
+    std::vector<int> cc_set;              // the members of the current compact maps
+    std::vector<int> cc_use64;            // their precisions when the maps were built
+    int cc_nblk = 0, cc_n32 = 0, cc_n64 = 0;
     std::string err;
 };
 
@@ -1047,6 +1067,22 @@ int batched_setup(ewg_ensemble *e) {
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_res), n * sizeof(double)));
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_res_upd), n * sizeof(double)));
     ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_ks), n * sizeof(KrylovState)));
+    e->nblk_of.assign(n, 0);
+    e->ntile_of.assign(n, 0);
+    e->ntile_all = 0;
+    for (int m = 0; m < n; ++m) {
+        e->nblk_of[m] = coff[m + 1] - coff[m];
+        e->ntile_of[m] = e->members[m]->wf.ntiles;
+        e->ntile_all += e->ntile_of[m];
+    }
+    ENS_TRY(e, dal(&e->cc_mem, e->nblkC));
+    ENS_TRY(e, dal(&e->cc_off, n + 1));
+    ENS_TRY(e, dal(&e->cc_t32, e->ntile_all));
+    ENS_TRY(e, dal(&e->cc_t64, e->ntile_all));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_cc_mem), std::max(e->nblkC, 1) * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_cc_off), (n + 1) * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_cc_t32), std::max(e->ntile_all, 1) * sizeof(int)));
+    ENS_TRY(e, hipHostMalloc(reinterpret_cast<void **>(&e->h_cc_t64), std::max(e->ntile_all, 1) * sizeof(int)));
     // batched tile-wavefront sweeps when every member carries the layout
     bool wl = true;
     for (int m = 0; m < n; ++m) {
@@ -1111,8 +1147,16 @@ Maps maps(const ewg_ensemble *e) { return Maps{e->cmem, e->coff, e->lmem, e->lof
 
 // out = (LU)^-1 in for all members whose Krylov status runs: tile-wavefront sweeps over
 // the members' wavefront layouts, else hyperplane by hyperplane
-void launch_apply(ewg_ensemble *e, int src, int dst) {
+// (compact: over the tiles of the compact maps' members only)
+void launch_apply(ewg_ensemble *e, int src, int dst, bool compact) {
     if (e->wl_py) {
+        if (compact) {
+            wl_batch_apply_tiles(src == V_P ? e->wl_py : e->wl_rz, e->cc_t32, e->cc_n32, e->stream);
+            if (e->any64)
+                wl_batch_apply_tiles(src == V_P ? e->wl_py64 : e->wl_rz64, e->cc_t64, e->cc_n64,
+                                     e->stream);
+            return;
+        }
         wl_batch_apply(src == V_P ? e->wl_py : e->wl_rz, e->stream);
         if (e->any64) wl_batch_apply(src == V_P ? e->wl_py64 : e->wl_rz64, e->stream);
         return;
@@ -1125,25 +1169,66 @@ void launch_apply(ewg_ensemble *e, int src, int dst) {
 }
 
 // one BiCGStab iteration (bicgstabsolver.hh:191-322) of every member whose solve runs
-void enqueue_iteration(ewg_ensemble *e) {
+// (compact: through the compact maps, over their members' blocks and tiles; the other
+// members' solves are not running then)
+void enqueue_iteration(ewg_ensemble *e, bool compact = false) {
     const int n = static_cast<int>(e->members.size());
-    const Maps mp = maps(e);
-    hipLaunchKernelGGL(k_b_ks_beta, dim3(n), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2);
-    hipLaunchKernelGGL(k_b_p_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks);
-    launch_apply(e, V_P, V_Y);
-    hipLaunchKernelGGL(k_b_spmv, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks,
+    const Maps mp = compact ? Maps{e->cc_mem, e->cc_off, e->lmem, e->loff} : maps(e);
+    const dim3 gC(compact ? e->cc_nblk : e->nblkC);
+    if (gC.x == 0) return;
+    hipLaunchKernelGGL(k_b_ks_beta, dim3(n), dim3(BT), 0, e->stream, e->dmem, e->ks);
+    hipLaunchKernelGGL(k_b_p_update, gC, dim3(BT), 0, e->stream, e->dmem, mp, e->ks);
+    launch_apply(e, V_P, V_Y, compact);
+    hipLaunchKernelGGL(k_b_spmv, gC, dim3(BT), 0, e->stream, e->dmem, mp, e->ks,
                        (int)V_Y, (int)V_V, (int)V_RES, -1, e->part0, static_cast<double *>(nullptr));
     hipLaunchKernelGGL(k_b_ks_alpha, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part0);
-    hipLaunchKernelGGL(k_b_hs_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part1, e->partAny);
+    hipLaunchKernelGGL(k_b_hs_update, gC, dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part1, e->partAny);
     hipLaunchKernelGGL(k_b_ks_crit1, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part1, e->partAny,
                        e->sweep_err);
-    launch_apply(e, V_R, V_Z);
-    hipLaunchKernelGGL(k_b_spmv, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks,
+    launch_apply(e, V_R, V_Z, compact);
+    hipLaunchKernelGGL(k_b_spmv, gC, dim3(BT), 0, e->stream, e->dmem, mp, e->ks,
                        (int)V_Z, (int)V_Y, (int)V_Y, (int)V_R, e->part0, e->part1);
     hipLaunchKernelGGL(k_b_ks_omega, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part0, e->part1);
-    hipLaunchKernelGGL(k_b_xr_update, dim3(e->nblkC), dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2, e->partAny);
-    hipLaunchKernelGGL(k_b_ks_crit2, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->partAny,
+    hipLaunchKernelGGL(k_b_xr_update, gC, dim3(BT), 0, e->stream, e->dmem, mp, e->ks, e->part2, e->partAny);
+    hipLaunchKernelGGL(k_b_ks_crit2, dim3(n), dim3(BT), 0, e->stream, mp, e->ks, e->part2, e->partAny,
                        e->sweep_err);
+}
+
+// The compact maps of the members in `set` (ascending): block -> member and each member's
+// first block (members outside the set: empty ranges), and the FP32 and FP64 batches'
+// tile lists of the set's members of that precision. Queued on the stream when the set or
+// a member's precision changed (the staging is rewritten only after the round's
+// synchronisation, which follows its copies).
+int compact_maps(ewg_ensemble *e, const std::vector<int> &set) {
+    const int n = static_cast<int>(e->members.size());
+    std::vector<int> p64(set.size());
+    for (size_t k = 0; k < set.size(); ++k) p64[k] = e->use64[set[k]];
+    if (set == e->cc_set && p64 == e->cc_use64) return EWG_OK;
+    int nb = 0, n32 = 0, n64 = 0, k = 0;
+    for (int m = 0; m < n; ++m) {
+        e->h_cc_off[m] = nb;
+        if (k < (int)set.size() && set[k] == m) {
+            ++k;
+            if (nb + e->nblk_of[m] > e->nblkC) return set_err(e, EWG_INVALID_STATE, "compact maps: blocks");
+            for (int b = 0; b < e->nblk_of[m]; ++b) e->h_cc_mem[nb++] = m;
+            int *t = e->use64[m] ? e->h_cc_t64 : e->h_cc_t32;
+            int &nt = e->use64[m] ? n64 : n32;
+            if (nt + e->ntile_of[m] > e->ntile_all) return set_err(e, EWG_INVALID_STATE, "compact maps: tiles");
+            for (int q = 0; q < e->ntile_of[m]; ++q) t[nt++] = m;
+        }
+    }
+    e->h_cc_off[n] = nb;
+    hipStream_t s = e->stream;
+    if (nb > 0) ENS_TRY(e, hipMemcpyAsync(e->cc_mem, e->h_cc_mem, nb * sizeof(int), hipMemcpyHostToDevice, s));
+    ENS_TRY(e, hipMemcpyAsync(e->cc_off, e->h_cc_off, (n + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    if (n32 > 0) ENS_TRY(e, hipMemcpyAsync(e->cc_t32, e->h_cc_t32, n32 * sizeof(int), hipMemcpyHostToDevice, s));
+    if (n64 > 0) ENS_TRY(e, hipMemcpyAsync(e->cc_t64, e->h_cc_t64, n64 * sizeof(int), hipMemcpyHostToDevice, s));
+    e->cc_set = set;
+    e->cc_use64 = p64;
+    e->cc_nblk = nb;
+    e->cc_n32 = n32;
+    e->cc_n64 = n64;
+    return EWG_OK;
 }
 
 // the same, captured once in a HIP graph
@@ -1404,13 +1489,27 @@ int batched_run(ewg_ensemble *e, double dt, int steps, bool commit) {
             if (forced) hipLaunchKernelGGL(k_b_set, gN, bT, 0, s, n, e->sweep_err, dm(M_FORCE));
         }
         ENS_TRY(e, hipGetLastError());
-        if (solving)
+        if (solving) {
+            std::vector<int> set;
+            int blocks = 0;
+            for (int m = 0; m < n; ++m)
+                if (P[m].phase == SOLVING) {
+                    set.push_back(m);
+                    blocks += e->nblk_of[m];
+                }
+            const bool compact =
+                e->wl_py && (e->compact == 2 || (e->compact == 1 && 2 * blocks <= e->nblkC));
+            if (compact)
+                if (int r = compact_maps(e, set)) return r;
             for (int q = 0; q < e->replay; ++q) {
-                if (e->use_graph)
+                if (compact)
+                    enqueue_iteration(e, true);
+                else if (e->use_graph)
                     ENS_TRY(e, hipGraphLaunch(e->iter_graph, s));
                 else
                     enqueue_iteration(e);
             }
+        }
         if (lin) ENS_TRY(e, hipMemcpyAsync(e->h_res, e->pres, n * sizeof(double), hipMemcpyDeviceToHost, s));
         if (upd)
             ENS_TRY(e, hipMemcpyAsync(e->h_res_upd, e->pres_upd, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1602,11 +1701,11 @@ void ewg_ensemble_destroy(ewg_ensemble *e) {
     wl_batch_free(e->wl_rz64);
     void *bufs[] = {e->cmem, e->coff, e->lmem, e->loff, e->act, e->niter, e->ilu_err, e->ks,
                     e->part0, e->part1, e->part2, e->pres, e->partAny, e->dts, e->sweep_err,
-                    e->dmask, e->pres_upd, e->act64};
+                    e->dmask, e->pres_upd, e->act64, e->cc_mem, e->cc_off, e->cc_t32, e->cc_t64};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     void *pinned[] = {e->h_mask, e->h_it, e->h_serr, e->h_dts, e->h_res, e->h_res_upd, e->h_ks,
-                      e->h_fp32};
+                      e->h_fp32, e->h_cc_mem, e->h_cc_off, e->h_cc_t32, e->h_cc_t64};
     for (void *b : pinned)
         if (b) (void)hipHostFree(b);
     for (double *b : e->xs)
@@ -1776,6 +1875,12 @@ int ewg_ensemble_set_mode(ewg_ensemble *e, int32_t mode, int32_t threads) {
 int ewg_ensemble_set_graph(ewg_ensemble *e, int32_t on) {
     if (!e || (on != 0 && on != 1)) return EWG_INVALID_ARGUMENT;
     e->use_graph = on != 0;
+    return EWG_OK;
+}
+
+int ewg_ensemble_set_compact(ewg_ensemble *e, int32_t on) {
+    if (!e || on < 0 || on > 2) return EWG_INVALID_ARGUMENT;
+    e->compact = on;
     return EWG_OK;
 }
 

@@ -1156,23 +1156,29 @@ void *wl_batch_create(const std::vector<WlMember> &members, bool fp32, hipStream
     return b;
 }
 
-void wl_batch_apply(void *h, hipStream_t s) {
+void wl_batch_apply_tiles(void *h, const int *tmem, int ntiles, hipStream_t s) {
     auto *b = static_cast<WlBatch *>(h);
+    if (ntiles <= 0) return;
     const dim3 rg((b->n + 255) / 256), rb(256);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_sweep_ws_batch<float, true, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
-                           static_cast<const WsArgs<float> *>(b->fwd), b->tmem);
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, true, kSweepPD>), dim3(ntiles), dim3(kSweepThreads), 0, s,
+                           static_cast<const WsArgs<float> *>(b->fwd), tmem);
     else
-        hipLaunchKernelGGL((k_sweep_ws_batch<double, true, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
-                           static_cast<const WsArgs<double> *>(b->fwd), b->tmem);
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, true, kSweepPD>), dim3(ntiles), dim3(kSweepThreads), 0, s,
+                           static_cast<const WsArgs<double> *>(b->fwd), tmem);
     hipLaunchKernelGGL(k_reset_tickets, rg, rb, 0, s, b->tickets, b->n);
     if (b->fp32)
-        hipLaunchKernelGGL((k_sweep_ws_batch<float, false, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
-                           static_cast<const WsArgs<float> *>(b->bwd), b->tmem);
+        hipLaunchKernelGGL((k_sweep_ws_batch<float, false, kSweepPD>), dim3(ntiles), dim3(kSweepThreads), 0, s,
+                           static_cast<const WsArgs<float> *>(b->bwd), tmem);
     else
-        hipLaunchKernelGGL((k_sweep_ws_batch<double, false, kSweepPD>), dim3(b->ntiles), dim3(kSweepThreads), 0, s,
-                           static_cast<const WsArgs<double> *>(b->bwd), b->tmem);
+        hipLaunchKernelGGL((k_sweep_ws_batch<double, false, kSweepPD>), dim3(ntiles), dim3(kSweepThreads), 0, s,
+                           static_cast<const WsArgs<double> *>(b->bwd), tmem);
+}
+
+void wl_batch_apply(void *h, hipStream_t s) {
+    auto *b = static_cast<WlBatch *>(h);
+    wl_batch_apply_tiles(h, b->tmem, b->ntiles, s);
 }
 
 void wl_batch_factor(void *h, hipStream_t s) {

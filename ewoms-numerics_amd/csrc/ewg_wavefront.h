@@ -105,6 +105,9 @@ struct WlMember {
 };
 void *wl_batch_create(const std::vector<WlMember> &members, bool fp32, hipStream_t s);
 void wl_batch_apply(void *batch, hipStream_t s);
+// the same over the tiles that tmem lists (tile -> realization, device array of ntiles
+// entries: every tile of each listed realization, a realization's tiles together)
+void wl_batch_apply_tiles(void *batch, const int *tmem, int ntiles, hipStream_t s);
 void wl_batch_factor(void *batch, hipStream_t s);
 void wl_batch_free(void *batch);
 

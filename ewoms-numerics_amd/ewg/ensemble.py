@@ -64,6 +64,7 @@ def _lib():
             "ewg_ensemble_tune_modes": (C.c_int, [_vp, C.c_double, C.c_int32, C.POINTER(Tuning)]),
             "ewg_ensemble_set_replay": (C.c_int, [_vp, C.c_int32]),
             "ewg_ensemble_set_graph": (C.c_int, [_vp, C.c_int32]),
+            "ewg_ensemble_set_compact": (C.c_int, [_vp, C.c_int32]),
             "ewg_ensemble_set_mode": (C.c_int, [_vp, C.c_int32, C.c_int32]),
             "ewg_ensemble_run": (C.c_int, [_vp, C.c_double, C.c_int32,
                                            C.POINTER(C.c_double)]),
@@ -142,6 +143,13 @@ class Ensemble:
         launch its kernels each time (False)."""
         self._check(self.lib.ewg_ensemble_set_graph(self.h, 1 if on else 0))
 
+    def set_compact(self, on):
+        """Batched mode: a round whose running solves belong to members holding at most
+        half of the cells launches its BiCGStab iterations over those members only (True or
+        1, default), every round does ("always" or 2), or none does (False or 0)."""
+        v = 2 if on == "always" else int(on)
+        self._check(self.lib.ewg_ensemble_set_compact(self.h, v))
+
     def set_replay(self, depth):
         """Batched mode: most BiCGStab iterations queued between state reads."""
         self._check(self.lib.ewg_ensemble_set_replay(self.h, depth))
@@ -180,6 +188,9 @@ def main(argv=None):
     ap.add_argument("--no-graph", action="store_true",
                     help="batched mode: launch the BiCGStab iteration's kernels instead of "
                          "replaying its graph (for kernel tracers)")
+    ap.add_argument("--no-compact", action="store_true",
+                    help="batched mode: launch every round's BiCGStab iterations over all "
+                         "members")
     ap.add_argument("--ilu-schedule", type=int, default=0, choices=(0, 1),
                     help="0: tile-wavefront sweeps in the batched mode; 1: hyperplane sweeps")
     args = ap.parse_args(argv)
@@ -196,8 +207,10 @@ def main(argv=None):
             ens.set_replay(args.replay)
         if args.no_graph:
             ens.set_graph(False)
+        if args.no_compact:
+            ens.set_compact(False)
         tuning = {"mode": args.mode, "threads": args.threads, "replay": args.replay,
-                  "candidates": []}
+                  "compact": not args.no_compact, "candidates": []}
     t2 = time.perf_counter()
     sec = ens.run(args.dt, args.steps)
     st = [ens.stats(i) for i in range(len(probs))]

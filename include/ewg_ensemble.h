@@ -93,13 +93,19 @@ int ewg_ensemble_tune_modes(ewg_ensemble *ens, double dt, int32_t modes,
                             ewg_ensemble_tuning *out);
 /* Choose the mode without tuning (threads: 256, 512 or 1024 for the persistent mode). */
 int ewg_ensemble_set_mode(ewg_ensemble *ens, int32_t mode, int32_t threads);
-/* The batched mode's BiCGStab replay depth (1 to 64; default 8): the iterations queued
+/* The batched mode's BiCGStab replay depth (1 to 64; default 4): the iterations queued
  * between two host reads of the members' states. Results do not depend on it. */
 int ewg_ensemble_set_replay(ewg_ensemble *ens, int32_t depth);
 /* The batched mode replays its BiCGStab iteration from a HIP graph (on = 1, the default)
  * or launches the iteration's kernels each time (on = 0). Results do not depend on it;
  * the second form is for tools that follow each launch. */
 int ewg_ensemble_set_graph(ewg_ensemble *ens, int32_t on);
+/* Compact rounds of the batched mode (members with the tile-wavefront layout): with
+ * on = 1 (the default), while the members whose linear solve runs hold at most half of the
+ * ensemble's cells, a round's BiCGStab iterations launch over those members' cells and
+ * tiles only (direct launches) instead of over every member's (graph replays); on = 2:
+ * every round compact; on = 0: none. Results do not depend on it. */
+int ewg_ensemble_set_compact(ewg_ensemble *ens, int32_t on);
 int ewg_ensemble_run(ewg_ensemble *ens, double dt, int32_t steps, double *seconds);
 int ewg_ensemble_stats(const ewg_ensemble *ens, int32_t i,
                        ewg_ensemble_member_stats *out);

@@ -349,3 +349,28 @@ def test_graph_and_direct_launches_agree():
     for i in range(4):
         assert out[0][0][i] == out[1][0][i]
         assert np.array_equal(out[0][1][i], out[1][1][i])
+
+
+def test_compact_rounds_agree_with_full_rounds():
+    """Rounds launched over the running members only (compact maps, direct launches) and
+    over all members (the iteration graph) give identical states and counts: every round
+    compact, the default (compact while the running members hold at most half of the
+    cells; the members' sizes differ by a factor of 40) and none."""
+    import ewg
+    from ewg import ensemble, problems
+    probs = [problems.heterogeneous_3d(nx, ny, nz, seed=s, patch=4, steps=2)
+             for nx, ny, nz, s in ((40, 32, 24, 61), (8, 8, 12, 62), (12, 8, 8, 63), (8, 16, 8, 64))]
+    out = []
+    for compact in ("always", 1, 0):
+        print(f"compact {compact}", flush=True)
+        ctxs = [ewg.GpuContext(P, ensemble.member_params()) for P in probs]
+        ens = ensemble.Ensemble(ctxs)
+        ens.set_mode(ensemble.BATCHED, 512)
+        ens.set_compact(compact)
+        ens.run(probs[0].dt, 2)
+        out.append(([ens.stats(i) for i in range(4)], [c.solution(0) for c in ctxs]))
+        assert all(s["status"] == 0 and s["steps"] == 2 for s in out[-1][0])
+    for k in (1, 2):
+        for i in range(4):
+            assert out[0][0][i] == out[k][0][i], (k, i)
+            assert np.array_equal(out[0][1][i], out[k][1][i]), (k, i)
