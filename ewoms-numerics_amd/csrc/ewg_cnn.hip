@@ -48,6 +48,8 @@ constexpr int kConvPF = 4;              // convolution operand prefetch distance
 // get the whole-grid prediction bit for bit)
 constexpr double kActiveEps = 1e-8;
 constexpr int kActiveMargin = 8;
+// the crop of a prediction on the device: origin, extent, cells
+enum { kGeoX0 = 0, kGeoY0, kGeoZ0, kGeoCx, kGeoCy, kGeoCz, kGeoN, kGeoLen };
 
 struct Layer {
     int ci, co, cip, cipb;  // real in/out channels, K channels forward / backward data
@@ -87,6 +89,8 @@ enum Mode { FWD_FIRST = 0, FWD_RES = 1, FWD_OUT = 2, BWD_DATA = 3 };
 struct ConvArgs {
     int nx, ny, nz;        // one sample
     int ncells;            // cells of the batch
+    const int *geo;        // null, or a crop on the device (CropGeo): nx, ny, nz and ncells
+                           // from there (the launch covers the largest crop)
     const unsigned short *in;    // [ncells][CIP]
     const unsigned short *w;     // [28][32][CIP]
     const float *bias;           // [32] or nullptr
@@ -116,6 +120,14 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     constexpr int NSTEP = CIP == 32 ? 54 : 14;  // K = 27 x 32 or 28 x 8, 16 per step
     constexpr int WELEM = kTapsPad * 32 * CIP;
     __shared__ __attribute__((aligned(16))) unsigned short wl[WELEM];
+    if (a.geo) {
+        a.nx = a.geo[kGeoCx];
+        a.ny = a.geo[kGeoCy];
+        a.nz = a.geo[kGeoCz];
+        a.ncells = a.geo[kGeoN];
+    }
+    // workgroups past the cells (a launch sized for the largest crop) do nothing
+    if (blockIdx.x * 4 * 64 >= a.ncells) return;
     {
         const u32x4 *src = reinterpret_cast<const u32x4 *>(a.w);
         u32x4 *dst = reinterpret_cast<u32x4 *>(wl);
@@ -473,22 +485,6 @@ __global__ void k_features(const double *x, const double *K, const double *phi, 
               f2bf(f4), 0u};
 }
 
-// guess = old + (dp, -dS_n) from the network output, saturation kept in [0, 1]
-__global__ void k_guess(const double *xold, const float *y, int n, ewg_cnn_norm nm,
-                        double *xg) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const float2 yv = reinterpret_cast<const float2 *>(y)[c];
-    double dp = (double)yv.x * nm.dp_scale, dsn = (double)yv.y * nm.ds_scale;
-    dp = fabs(dp) >= nm.dp_min ? dp : 0.0;
-    dsn = fabs(dsn) >= nm.ds_min ? dsn : 0.0;
-    const double p = xold[2 * c] + dp;
-    double s = xold[2 * c + 1] - dsn;
-    s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
-    xg[2 * c] = p;
-    xg[2 * c + 1] = s;
-}
-
 // Inference on the active region: the bounding box (x0, x1, y0, y1, z0, z1) of the cells
 // with a non-wetting saturation above eps, merged into box by min / max atomics (one set
 // per block that has such a cell)
@@ -528,12 +524,56 @@ k_active_box(const double *x, int nx, int ny, int n, double eps, int *box) {
     }
 }
 
+// The crop of the prediction on the device (k_crop_geo): origin, extent, cells
+struct SrcBox {
+    int b[6];   // box of the cells next to boundaries that inject the non-wetting phase
+    int valid;
+};
+// k_crop_geo: from the active box (x0, x1, y0, y1, z0, z1) merged with the injection
+// box, grown by kActiveMargin and clipped, x extent a multiple of 16 (nx is one); no
+// active cell: 0 cells (the guess is the old state); more than half of the grid: the
+// whole grid
+__global__ void k_crop_geo(const int *box, SrcBox src, int nx, int ny, int nz, int *geo) {
+    if (threadIdx.x != 0) return;
+    int b[6];
+    for (int k = 0; k < 6; ++k) b[k] = box[k];
+    if (src.valid)
+        for (int k = 0; k < 6; ++k) b[k] = (k & 1) ? max(b[k], src.b[k]) : min(b[k], src.b[k]);
+    if (b[1] < b[0]) {
+        for (int k = 0; k < kGeoLen; ++k) geo[k] = 0;
+        return;
+    }
+    const int M = kActiveMargin;
+    int x0 = max(0, b[0] - M);
+    const int x1 = min(nx - 1, b[1] + M);
+    const int y0 = max(0, b[2] - M), y1 = min(ny - 1, b[3] + M);
+    const int z0 = max(0, b[4] - M), z1 = min(nz - 1, b[5] + M);
+    x0 = (x0 / 16) * 16;
+    const int cx = ((x1 - x0 + 1 + 15) / 16) * 16;
+    if (x0 + cx > nx) x0 = nx - cx;
+    const int cy = y1 - y0 + 1, cz = z1 - z0 + 1;
+    const long long nc = (long long)nx * ny * nz, ncrop = (long long)cx * cy * cz;
+    if (2 * ncrop > nc) {
+        geo[kGeoX0] = 0; geo[kGeoY0] = 0; geo[kGeoZ0] = 0;
+        geo[kGeoCx] = nx; geo[kGeoCy] = ny; geo[kGeoCz] = nz;
+        geo[kGeoN] = (int)nc;
+        return;
+    }
+    geo[kGeoX0] = x0; geo[kGeoY0] = y0; geo[kGeoZ0] = z0;
+    geo[kGeoCx] = cx; geo[kGeoCy] = cy; geo[kGeoCz] = cz;
+    geo[kGeoN] = (int)ncrop;
+}
+
 // features of the cells of the crop [x0, x0 + cx) x [y0, y0 + cy) x [z0, ...) of the grid
+// (the crop from geo; a launch sized for the largest crop)
 __global__ void k_features_crop(const double *x, const double *K, const double *phi, int nx,
-                                int ny, int x0, int y0, int z0, int cx, int cy, int n, double dt,
-                                ewg_cnn_norm nm, unsigned short *feat) {
+                                int ny, const int *geo, double dt, ewg_cnn_norm nm,
+                                unsigned short *feat) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = geo[kGeoN];
     if (q >= n) return;
+    const int x0 = geo[kGeoX0], y0 = geo[kGeoY0], z0 = geo[kGeoZ0], cx = geo[kGeoCx],
+              cy = geo[kGeoCy];
     const int i = q % cx, r = q / cx, j = r % cy, k = r / cy;
     const size_t c = (size_t)(x0 + i) + (size_t)nx * ((size_t)(y0 + j) + (size_t)ny * (z0 + k));
     const double p = x[2 * c], sw = x[2 * c + 1];
@@ -547,11 +587,15 @@ __global__ void k_features_crop(const double *x, const double *K, const double *
               f2bf(f4), 0u};
 }
 
-// guess at the crop's cells (k_guess's arithmetic); the others keep the old state
-__global__ void k_guess_crop(const double *xold, const float *y, int nx, int ny, int x0, int y0,
-                             int z0, int cx, int cy, int n, ewg_cnn_norm nm, double *xg) {
+// guess = old + (dp, -dS_n) from the network output at the crop's cells, saturation kept
+// in [0, 1]; the others keep the old state
+__global__ void k_guess_crop(const double *xold, const float *y, int nx, int ny, const int *geo,
+                             ewg_cnn_norm nm, double *xg) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = geo[kGeoN];
     if (q >= n) return;
+    const int x0 = geo[kGeoX0], y0 = geo[kGeoY0], z0 = geo[kGeoZ0], cx = geo[kGeoCx],
+              cy = geo[kGeoCy];
     const int i = q % cx, r = q / cx, j = r % cy, k = r / cy;
     const size_t c = (size_t)(x0 + i) + (size_t)nx * ((size_t)(y0 + j) + (size_t)ny * (z0 + k));
     const float2 yv = reinterpret_cast<const float2 *>(y)[q];
@@ -588,9 +632,11 @@ struct ewg_cnn {
     float *y = nullptr, *g[2] = {};
     unsigned short *gz = nullptr, *gzT = nullptr, *actT = nullptr, *feat = nullptr;
     unsigned short *bfeat = nullptr;  // gathered batch (training)
-    int *box_d = nullptr, *box_h = nullptr;  // active-region box (device, pinned host)
+    int *box_d = nullptr;                    // active-region box (device)
+    int *geo_d = nullptr, *geo_h = nullptr;  // the last prediction's crop (device, pinned host)
+    hipEvent_t geo_ev = nullptr;             // recorded after the copy of geo_h
+    bool geo_pending = false;
     float *bpart = nullptr;                   // bias-gradient partitions [32][kBiasParts]
-    int64_t last_crop_cells = 0;
     float *btarget = nullptr;
     float *part = nullptr;
     double *lpart = nullptr;
@@ -629,10 +675,12 @@ void free_net(ewg_cnn *n) {
         ptrs.push_back(n->u[l]);
     }
     ptrs.push_back(n->box_d);
+    ptrs.push_back(n->geo_d);
     ptrs.push_back(n->bpart);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    if (n->box_h) (void)hipHostFree(n->box_h);
+    if (n->geo_h) (void)hipHostFree(n->geo_h);
+    if (n->geo_ev) (void)hipEventDestroy(n->geo_ev);
     if (n->ev0) (void)hipEventDestroy(n->ev0);
     if (n->ev1) (void)hipEventDestroy(n->ev1);
     if (n->stream) (void)hipStreamDestroy(n->stream);
@@ -670,13 +718,14 @@ ConvArgs conv_args(const ewg_cnn *n, int ncells) {
 
 // forward pass over `ncells` cells of feat; keeps h[0..3] (and u[1..3] when training)
 void forward(ewg_cnn *n, const unsigned short *feat, int ncells, bool keep_u, hipStream_t s,
-             bool timed, int gx = 0, int gy = 0, int gz = 0) {
+             bool timed, int gx = 0, int gy = 0, int gz = 0, const int *geo = nullptr) {
     ConvArgs a = conv_args(n, ncells);
     if (gx > 0) {  // samples of another grid (a crop of the network's grid)
         a.nx = gx;
         a.ny = gy;
         a.nz = gz;
     }
+    a.geo = geo;  // a crop on the device (ncells: the largest crop)
     if (timed) (void)hipEventRecord(n->ev0, s);
     a.in = feat;
     a.w = n->wf[0];
@@ -818,57 +867,42 @@ namespace cnn {
 int predict_guess(ewg_cnn *n, const double *xold, const double *K, const double *phi,
                   double dt, double *xguess, hipStream_t s, const int *src_box) {
     const int nc = static_cast<int>(n->ncell);
-    const int nx = n->nx, ny = n->ny, nz = n->nz;
+    const int nx = n->nx, ny = n->ny;
     // the active region: cells with non-wetting phase, and the cells next to rate
-    // boundaries that inject it
+    // boundaries that inject it; the crop is derived on the device, and the launches
+    // after it are sized for the whole grid (their workgroups past the crop return at
+    // once), so that the prediction is queued without a wait for the box
+    SrcBox src{};
+    src.valid = src_box && src_box[1] >= src_box[0];
+    if (src.valid)
+        for (int k = 0; k < 6; ++k) src.b[k] = src_box[k];
     hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, s, n->box_d);
     hipLaunchKernelGGL(k_active_box, dim3((nc + 255) / 256), dim3(256), 0, s, xold, nx, ny, nc,
                        kActiveEps, n->box_d);
-    if (hipMemcpyAsync(n->box_h, n->box_d, 6 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
+    hipLaunchKernelGGL(k_crop_geo, dim3(1), dim3(64), 0, s, n->box_d, src, nx, ny, n->nz, n->geo_d);
+    if (!n->geo_ev && hipEventCreateWithFlags(&n->geo_ev, hipEventDisableTiming) != hipSuccess)
         return EWG_DEVICE_ERROR;
-    int b[6];
-    for (int k = 0; k < 6; ++k) b[k] = n->box_h[k];
-    if (src_box && src_box[1] >= src_box[0])
-        for (int k = 0; k < 6; ++k) b[k] = (k & 1) ? std::max(b[k], src_box[k]) : std::min(b[k], src_box[k]);
-    if (b[1] < b[0]) {  // nothing active: the prediction would be discarded everywhere
-        n->last_crop_cells = 0;
-        return hipMemcpyAsync(xguess, xold, 2 * n->ncell * sizeof(double), hipMemcpyDeviceToDevice,
-                              s) == hipSuccess ? EWG_OK : EWG_DEVICE_ERROR;
-    }
-    // grow by the margin, clip; x extent a multiple of 16 (nx is one)
-    const int M = kActiveMargin;
-    int x0 = std::max(0, b[0] - M), x1 = std::min(nx - 1, b[1] + M);
-    const int y0 = std::max(0, b[2] - M), y1 = std::min(ny - 1, b[3] + M);
-    const int z0 = std::max(0, b[4] - M), z1 = std::min(nz - 1, b[5] + M);
-    x0 = (x0 / 16) * 16;
-    int cx = ((x1 - x0 + 1 + 15) / 16) * 16;
-    if (x0 + cx > nx) x0 = nx - cx;
-    const int cy = y1 - y0 + 1, cz = z1 - z0 + 1;
-    const long long ncrop = (long long)cx * cy * cz;
-    if (2 * ncrop > nc) {  // most of the grid: the whole grid
-        n->last_crop_cells = nc;
-        hipLaunchKernelGGL(k_features, dim3((nc + 255) / 256), dim3(256), 0, s, xold, K, phi, nc,
-                           dt, n->norm, n->feat);
-        forward(n, n->feat, nc, false, s, false);
-        hipLaunchKernelGGL(k_guess, dim3((nc + 255) / 256), dim3(256), 0, s, xold, n->y, nc,
-                           n->norm, xguess);
-        return hipGetLastError() == hipSuccess ? EWG_OK : EWG_DEVICE_ERROR;
-    }
-    const int nq = static_cast<int>(ncrop);
-    n->last_crop_cells = nq;
-    hipLaunchKernelGGL(k_features_crop, dim3((nq + 255) / 256), dim3(256), 0, s, xold, K, phi, nx,
-                       ny, x0, y0, z0, cx, cy, nq, dt, n->norm, n->feat);
-    forward(n, n->feat, nq, false, s, false, cx, cy, cz);
+    if (hipMemcpyAsync(n->geo_h, n->geo_d, kGeoLen * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipEventRecord(n->geo_ev, s) != hipSuccess)
+        return EWG_DEVICE_ERROR;
+    n->geo_pending = true;
+    hipLaunchKernelGGL(k_features_crop, dim3((nc + 255) / 256), dim3(256), 0, s, xold, K, phi, nx,
+                       ny, n->geo_d, dt, n->norm, n->feat);
+    forward(n, n->feat, nc, false, s, false, 0, 0, 0, n->geo_d);
     if (hipMemcpyAsync(xguess, xold, 2 * n->ncell * sizeof(double), hipMemcpyDeviceToDevice, s) !=
         hipSuccess)
         return EWG_DEVICE_ERROR;
-    hipLaunchKernelGGL(k_guess_crop, dim3((nq + 255) / 256), dim3(256), 0, s, xold, n->y, nx, ny,
-                       x0, y0, z0, cx, cy, nq, n->norm, xguess);
+    hipLaunchKernelGGL(k_guess_crop, dim3((nc + 255) / 256), dim3(256), 0, s, xold, n->y, nx, ny,
+                       n->geo_d, n->norm, xguess);
     return hipGetLastError() == hipSuccess ? EWG_OK : EWG_DEVICE_ERROR;
 }
 
-int64_t last_inference_cells(const ewg_cnn *n) { return n ? n->last_crop_cells : 0; }
+// the cells of the last prediction's crop (waits for its copy to the host)
+int64_t last_inference_cells(const ewg_cnn *n) {
+    if (!n || !n->geo_pending) return 0;
+    if (hipEventSynchronize(n->geo_ev) != hipSuccess) return 0;
+    return n->geo_h[kGeoN];
+}
 
 bool fits_grid(const ewg_cnn *n, int nx, int ny, int nz) {
     return n && n->nx == nx && n->ny == ny && n->nz == nz && n->max_batch >= 1;
@@ -926,9 +960,11 @@ int ewg_cnn_create(int32_t nx, int32_t ny, int32_t nz, int32_t max_batch, ewg_cn
     CNN_ALLOC(n->actT, V * 3 * 32);
     CNN_ALLOC(n->feat, ncell * 8);
     CNN_ALLOC(n->box_d, 6);
+    CNN_ALLOC(n->geo_d, kGeoLen);
     CNN_ALLOC(n->bpart, 32 * kBiasParts);
-    if ((e = hipHostMalloc(reinterpret_cast<void **>(&n->box_h), 6 * sizeof(int))) != hipSuccess)
+    if ((e = hipHostMalloc(reinterpret_cast<void **>(&n->geo_h), kGeoLen * sizeof(int))) != hipSuccess)
         return fail(e);
+    for (int k = 0; k < kGeoLen; ++k) n->geo_h[k] = 0;
     CNN_ALLOC(n->part, (size_t)kWgParts * 27 * 1024);
     CNN_ALLOC(n->lpart, (V + 255) / 256);
     CNN_ALLOC(n->bfeat, V * 8);
