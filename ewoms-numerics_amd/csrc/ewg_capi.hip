@@ -44,8 +44,8 @@ int set_err(ewg_ctx *c, int code, const std::string &msg) {
 // loads the library holds its own copy until the process ends), split into fixed regions
 // so that the edge words always live in the edge region:
 //   [0, 256)                 the two tile tickets of the sweeps
-//   [256, 256 + 16 MiB)      both edge buffers of the tile sweeps (each call fills the
-//                            part its grid uses with the sentinel first)
+//   [256, 256 + 16 MiB)      both edge buffers of the tile sweeps (the sentinel between
+//                            calls; the sweeps restore it word by word)
 //   [.., .. + 32 MiB)        the forward sweep's intermediate vector
 // Host code never passes `scratch` itself to a kernel: it is a host-side stand-in there;
 // the device address comes from hipGetSymbolAddress once per device (noalloc_setup).
@@ -786,7 +786,7 @@ int ewg_create(const ewg_grid_desc *grid, const ewg_fluid_desc *fluid,
             w.ntiles = w.ntx * w.nty;
             ALLOC(ticket, 2);
             ALLOC(dorder, w.ntiles);
-            w.edge_words = (size_t)w.ntiles * T * wf_edge_slots(g.nz) * kWfMaxWords;
+            w.edge_words = (size_t)w.ntiles * T * g.nz * kWfMaxWords;
             ALLOC(ex, w.edge_words);
             ALLOC(ey, w.edge_words);
             std::vector<int2> order;
@@ -1169,9 +1169,6 @@ int ewg_apply_preconditioner_noalloc(ewg_ctx *c, const double *d, double *v) {
         w.ey = w.ex + c->wf.edge_words;
         w.err = c->ilu_err;
         launch_flag_clear(c->ilu_err, s);
-        // the sweeps leave values in the spare slots of their edge lanes, which another
-        // grid's edge lanes may use as value slots: the sentinel first, in the part used
-        wf_init_edges(w, s);
         if (fp32) {
             WlPrecond<float> wl = c->wl32;
             wl.w = reinterpret_cast<float *>(base + kScratchTickets + kScratchEdges);

@@ -13,8 +13,6 @@
 //   - no predication of the block products either: blocks of absent neighbours are
 //     stored as zero, and the neighbour values that enter them are finite (the backward
 //     sweep's -z value of a column's first cell is selected to zero);
-//   - no test on the edge stores: the producing lanes store every step's result, the
-//     steps outside their column into spare slots of the edge buffers;
 //   - the in-tile exchange of step results through LDS (one store and two loads per
 //     step), and one comparison per edge direction to see whether a step's upstream
 //     values are in place.
@@ -99,19 +97,17 @@ __device__ __forceinline__ EW<T> lde(__amdgpu_buffer_rsrc_t r, unsigned off) {
     }
     return e;
 }
-// soff: a wave-uniform part of the offset (the scalar offset operand)
 template <class T>
-__device__ __forceinline__ void ste(__amdgpu_buffer_rsrc_t r, unsigned off, const EW<T> &e,
-                                    unsigned soff = 0) {
+__device__ __forceinline__ void ste(__amdgpu_buffer_rsrc_t r, unsigned off, const EW<T> &e) {
     if constexpr (sizeof(T) == 8) {
         u32x4 v;
         v.x = (unsigned)e.w[0]; v.y = (unsigned)(e.w[0] >> 32);
         v.z = (unsigned)e.w[1]; v.w = (unsigned)(e.w[1] >> 32);
-        __builtin_amdgcn_raw_buffer_store_b128(v, r, off, (int)soff, kCoh);
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, kCoh);
     } else {
         u32x2 v;
         v.x = (unsigned)e.w[0]; v.y = (unsigned)(e.w[0] >> 32);
-        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, (int)soff, kCoh);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kCoh);
     }
 }
 template <class T>
@@ -235,7 +231,7 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     if (tk >= (unsigned)a.sync.ntiles) return;
     const int2 tile = a.sync.order[tk];
     const int tx = tile.x, ty = tile.y;
-    const int nz = a.nz, nzp = wf_edge_slots(nz);
+    const int nz = a.nz;
     const int tileId = ty * a.sync.ntx + tx;
     // slots without an upstream tile read zeros for good (the matching blocks are zero as
     // well) and their edge lanes count as complete
@@ -264,8 +260,8 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + idx < a.ny) : (ty > 0 && tx * TX + idx < a.nx);
         const unsigned gbase =
             !valid ? 0u
-            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nzp + kWfEdgeLo) * zw)
-                       : (unsigned)((((size_t)(tileId - a.sync.ntx) * TX + idx) * nzp + kWfEdgeLo) * zw);
+            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nz) * zw)
+                       : (unsigned)((((size_t)(tileId - a.sync.ntx) * TX + idx) * nz) * zw);
         // value z of this edge lane is used by lane (0, idx) (x edge) or (idx, 0) (y edge)
         // at its step z + idx
         P2<T> *const ring = dir == 0 ? &xv[0][TX * idx] : &yv[0][idx];
@@ -364,11 +360,10 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
         pw = (unsigned)(((tf * nsteps + z0 + lxf + lyf) * 64 + lxf + 8 * lyf) * (int)vB);
     }
     constexpr unsigned sw = 64 * vB;  // the forward entry moves by sw per step
-    // this lane's edge slot of local step 0 (z = -lx - ly): the last column's lanes store
-    // to edge lane ly of the x buffer, the last row's to edge lane lx of the y buffer; the
-    // slot of step tau is tau slots on (the other lanes: out of range)
-    const unsigned eX = prodX ? (unsigned)((((size_t)tileId * 8 + ly) * nzp + kWfEdgeLo - lx - ly) * zw) : kFar;
-    const unsigned eY = prodY ? (unsigned)((((size_t)tileId * 8 + lx) * nzp + kWfEdgeLo - lx - ly) * zw) : kFar;
+    // this tile's words in the edge buffers: its last column (edge lane ly) and last row
+    // (edge lane lx), z-value words consecutive
+    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nz) * zw);
+    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nz) * zw);
     __amdgpu_buffer_rsrc_t rb0 = rsrc(a.blk[0], a.wl_bytes_blk), rb1 = rsrc(a.blk[1], a.wl_bytes_blk),
                            rb2 = rsrc(a.blk[2], a.wl_bytes_blk);
     __amdgpu_buffer_rsrc_t rdi = rsrc(a.Dinv, FWD ? 0 : a.wl_bytes_blk);
@@ -427,7 +422,6 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     P2<T> nx_ = rx[0], ny_ = ry[0];
     for (int tau0 = 0; tau0 < nsteps; tau0 += kRing) {
-        const unsigned esoff = (unsigned)tau0 * zw;
 #pragma unroll
         for (int k = 0; k < kRing; ++k) {
             const int tau = tau0 + k;
@@ -486,9 +480,12 @@ __device__ __forceinline__ void sweep_ws_body(const WsArgs<T> &a) {
                 s1 += c.di[3] * r1;
                 res = P2<T>{s0, s1};
             }
-            // the downstream tiles' values, then the result vector
-            ste<T>(rex, eX + (unsigned)k * zw, val_ew(res), esoff);
-            ste<T>(rey, eY + (unsigned)k * zw, val_ew(res), esoff);
+            // the downstream tiles' values (the cells of the column only: unconsumed words
+            // would stay in the buffers), then the result vector
+            const int z = tau - lx - ly;
+            const bool zin = (unsigned)z < (unsigned)nz;
+            ste<T>(rex, prodX && zin ? gX + (unsigned)z * zw : kFar, val_ew(res));
+            ste<T>(rey, prodY && zin ? gY + (unsigned)z * zw : kFar, val_ew(res));
             if (FWD) {
                 stp<T>(rw, ow + (unsigned)k * sw, res);
             } else {
@@ -635,7 +632,7 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     if (tk >= (unsigned)a.sync.ntiles) return;
     const int2 tile = a.sync.order[tk];
     const int tx = tile.x, ty = tile.y;
-    const int nz = a.nz, nsteps = a.nsteps, nzp = wf_edge_slots(nz);
+    const int nz = a.nz, nsteps = a.nsteps;
     const int ntx = a.sync.ntx;
     const int tileId = ty * ntx + tx;
     for (int i = tid; i < 2 * 8 * kRing; i += 2 * NT) (&ering[0][0][0])[i] = B4{{0.0, 0.0, 0.0, 0.0}};
@@ -677,8 +674,8 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
         const bool valid = dir == 0 ? (tx > 0 && ty * TY + idx < a.ny) : (ty > 0 && tx * TX + idx < a.nx);
         const unsigned gbase =
             !valid ? 0u
-            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nzp + kWfEdgeLo) * zw)
-                       : (unsigned)((((size_t)(tileId - ntx) * TX + idx) * nzp + kWfEdgeLo) * zw);
+            : dir == 0 ? (unsigned)((((size_t)(tileId - 1) * TY + idx) * nz) * zw)
+                       : (unsigned)((((size_t)(tileId - ntx) * TX + idx) * nz) * zw);
         B4 *ring = &ering[dir][idx][0];
         const unsigned long long sent[4] = {kSentinel, kSentinel, kSentinel, kSentinel};
         int z = valid ? j : nz;
@@ -782,8 +779,8 @@ __device__ __forceinline__ void factor_ws_body(const FwsArgs<T> &a) {
     B4 prev = B4{{0.0, 0.0, 0.0, 0.0}};
     const bool hasL = lx > 0, hasD = ly > 0;
     // this tile's words in the edge buffers (last column: edge lane ly; last row: lx)
-    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nzp + kWfEdgeLo) * zw);
-    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nzp + kWfEdgeLo) * zw);
+    const unsigned gX = (unsigned)((((size_t)tileId * 8 + ly) * nz) * zw);
+    const unsigned gY = (unsigned)((((size_t)tileId * 8 + lx) * nz) * zw);
     const B4 *const eX = &ering[0][ly][0];
     const B4 *const eY = &ering[1][lx][0];
     const int *const fX = &filled[ly];

@@ -20,12 +20,6 @@ constexpr int kSweepPD = EWG_SWEEP_PD;  // prefetch depth of the triangular swee
 constexpr int kSweepPE = 1;     // prefetch depth of their cross-tile words
 
 constexpr int kWfMaxWords = 4;  // 64-bit words per exchanged value (FP64 2x2 block)
-// The edge buffers hold, per tile and edge lane (8 for its last column, 8 for its last
-// row), one slot of kWfMaxWords words per z-value, with kWfEdgeLo slots before z = 0 and
-// kWfEdgeHi after z = nz - 1: the producing lanes store every step's result without a
-// test, and the steps outside their column land in these spare slots.
-constexpr int kWfEdgeLo = 14, kWfEdgeHi = 22;
-inline __host__ __device__ int wf_edge_slots(int nz) { return nz + kWfEdgeLo + kWfEdgeHi; }
 // local steps per tile in the wavefront layout: the tile's skew of 14 steps plus nz,
 // rounded up to the sweeps' unroll of 16 (every step the sweeps run has its own row)
 inline __host__ __device__ int wl_nsteps(int nz) { return (nz + 14 + 15) & ~15; }
@@ -40,8 +34,8 @@ constexpr long long kWfMaxCells = 1LL << 25;
 struct WfSync {
     unsigned *ticket;          // two counters (the warp-specialized apply uses both)
     const int2 *order;         // [ntiles] tiles (tx, ty) in anti-diagonal order
-    unsigned long long *ex;    // [ntiles][TY][wf_edge_slots(nz)][kWfMaxWords] last-column values
-    unsigned long long *ey;    // [ntiles][TX][wf_edge_slots(nz)][kWfMaxWords] last-row values
+    unsigned long long *ex;    // [ntiles][TY][nz][kWfMaxWords] last-column values
+    unsigned long long *ey;    // [ntiles][TX][nz][kWfMaxWords] last-row values
     size_t edge_words;         // words in each of ex, ey
     int ntx, nty, ntiles;
     int *err;                  // set when a wait exceeded its spin limit

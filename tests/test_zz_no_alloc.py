@@ -122,7 +122,7 @@ def test_noalloc_apply_rejects_a_grid_beyond_the_scratch():
     import ewg
     from ewg.capi import EWG_INVALID_ARGUMENT
     fn = _bind()
-    P = _problem((72, 72, 512))      # tile edges 22.7 MB, intermediate 43.8 MB
+    P = _problem((72, 72, 512))      # tile edges 21.2 MB, intermediate 43.8 MB
     g = ewg.GpuContext(P, ewg.default_params())
     rc = fn(g.h, C.c_void_p(1), C.c_void_p(1))   # rejected before any device access
     assert rc == EWG_INVALID_ARGUMENT
