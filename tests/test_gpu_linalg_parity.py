@@ -7,7 +7,8 @@ factors are the FP64 factors rounded to float (within one float rounding), and t
 apply equals the oracle's float apply of those factors within 1e-6 relative. The FP32
 operator (params.operator_fp32) equals the oracle's FP64 product with the Jacobian's
 blocks rounded to float within 1e-12.
-Shapes include partial 8x8 tiles (x, y not multiples of 8) and the 2D lens grid.
+Shapes include partial 8x8 tiles (x, y not multiples of 8), thin grids (one z-layer,
+three cells across) and the 2D lens grid.
 """
 import numpy as np
 import pytest
@@ -16,7 +17,7 @@ from helpers import rel_err
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(20, 13, 7), (32, 24, 16), (37, 21, 5)]
+SHAPES = [(20, 13, 7), (32, 24, 16), (37, 21, 5), (17, 4, 1), (3, 11, 2)]
 
 
 def _system(P, params, seed):

@@ -164,13 +164,15 @@ def test_fp32_preconditioner_close_to_fp64():
     assert dS < 1e-6
 
 
-@pytest.mark.parametrize("shape", [(20, 13, 7), (32, 24, 16), (8, 8, 3), (37, 21, 5)])
+@pytest.mark.parametrize("shape", [(20, 13, 7), (32, 24, 16), (8, 8, 3), (37, 21, 5), (9, 3, 1),
+                                   (3, 11, 2), (40, 3, 9)])
 @pytest.mark.parametrize("fp32", [0, 1])
 def test_wavefront_ilu_matches_hyperplane_schedule(shape, fp32):
     """The tile-wavefront ILU(0) schedule (one persistent launch per factorization and
     sweep, factors in the tiles' wavefront layout) and the one-launch-per-hyperplane
     schedule run the same arithmetic in the same order: the whole BiCGStab solve is
-    bit-identical (partial tiles included)."""
+    bit-identical (partial tiles included, and thin grids: one z-layer, a single row of
+    tiles in x or y, three cells across)."""
     import ewg
     from ewg import problems
     nx, ny, nz = shape
