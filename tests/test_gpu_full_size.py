@@ -9,8 +9,13 @@ comparisons are direct, with the bars of the small cases:
   once, the FP32 apply within 1e-6 of the oracle's float apply of those factors;
 * the first time step from the initial state (its 85 BiCGStab iterations are most of the
   benchmark's 20-step run): the same Newton iterations, saturations within 1e-8 and
-  BiCGStab iterations within 2 % (the dot products sum in another order).
+  BiCGStab iterations within 2 % (the dot products sum in another order);
+* the reduced-precision forms against the same FP64 oracle: FP32 face fluxes within 1e-5
+  of the largest residual and Jacobian entry, and the first time step of FP32 fluxes with
+  the FP32 preconditioner with saturations within 1e-6 and Newton iterations within one.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -24,6 +29,24 @@ SHAPE = (128, 128, 64)
 def _problem():
     from ewg import problems
     return problems.heterogeneous_3d(*SHAPE)  # the benchmark's case (its default seed)
+
+
+def _step_params(**extra):
+    import ewg
+    return ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3,
+                              max_time_step_size=_problem().dt, **extra)
+
+
+@functools.lru_cache(maxsize=1)
+def _oracle_first_step():
+    """The oracle's first time step of the case (FP64): Newton and BiCGStab iterations and
+    the state after it (computed once per session)."""
+    from oracle.pyoracle import OracleContext
+    P = _problem()
+    o = OracleContext(P, _step_params())
+    rc, it, _ = o.newton(P.dt)
+    assert rc == 0
+    return it, o.stats()["linear_iterations"], o.solution(0)
 
 
 @pytest.mark.parametrize("newton_iter", [0, 1])
@@ -83,20 +106,56 @@ def test_full_size_ilu0_and_spmv_match_oracle(fp32):
 
 def test_full_size_first_time_step_matches_oracle():
     import ewg
-    from oracle.pyoracle import OracleContext
     P = _problem()
     print(f"first time step {SHAPE}", flush=True)
-    prm = ewg.default_params(newton_tolerance=1e-6, linear_tolerance=1e-3,
-                             max_time_step_size=P.dt)
-    g, o = ewg.GpuContext(P, prm), OracleContext(P, prm)
+    g = ewg.GpuContext(P, _step_params())
     rg, itg, _ = g.newton(P.dt)
-    ro, ito, _ = o.newton(P.dt)
-    sg, so = g.stats(), o.stats()
-    print("newton", itg, ito, "bicgstab", sg["linear_iterations"], so["linear_iterations"],
-          flush=True)
-    assert rg == 0 and ro == 0
+    ito, lito, xo = _oracle_first_step()
+    litg = g.stats()["linear_iterations"]
+    print("newton", itg, ito, "bicgstab", litg, lito, flush=True)
+    assert rg == 0
     assert itg == ito
-    assert abs(sg["linear_iterations"] - so["linear_iterations"]) <= 0.02 * so["linear_iterations"]
-    dS = np.abs(g.solution(0)[1::2] - o.solution(0)[1::2]).max()
+    assert abs(litg - lito) <= 0.02 * lito
+    dS = np.abs(g.solution(0)[1::2] - xo[1::2]).max()
     print(f"max |dS| {dS:.3e}", flush=True)
     assert dS < 1e-8
+
+
+def test_full_size_fp32_flux_linearization_close_to_fp64():
+    import ewg
+    from ewg import problems
+    from oracle.pyoracle import OracleContext
+    P = _problem()
+    print(f"fp32 flux linearization {SHAPE}", flush=True)
+    g32, ora = ewg.GpuContext(P, ewg.default_params(linearize_fp32=1)), OracleContext(P)
+    x = problems.perturbed_state(P, seed=35)
+    x1 = problems.perturbed_state(P, seed=36, dS=0.1)
+    for c in (g32, ora):
+        c.set_solution(x1, 1)
+        c.set_solution(x1, 0)
+        c.linearize(P.dt, 0, want=False)
+        c.set_solution(x, 0)
+    r32, j32 = g32.linearize(P.dt, 1)
+    ro, jo = ora.linearize(P.dt, 1)
+    er = np.abs(r32 - ro).max() / np.abs(ro).max()
+    ej = np.abs(j32 - jo).max() / np.abs(jo).max()
+    print(f"fp32 flux: residual {er:.3e}, jacobian {ej:.3e} (of the largest entry)", flush=True)
+    assert er < 1e-5 and ej < 1e-5
+    assert er > 0.0 or ej > 0.0  # the FP32 form is really taken
+
+
+def test_full_size_mixed_first_time_step_close_to_fp64_oracle():
+    import ewg
+    P = _problem()
+    print(f"first time step, FP32 fluxes and FP32 preconditioner {SHAPE}", flush=True)
+    g = ewg.GpuContext(P, _step_params(linearize_fp32=1, precond_fp32=1))
+    rg, itg, _ = g.newton(P.dt)
+    ito, lito, xo = _oracle_first_step()
+    xg = g.solution(0)
+    dS = np.abs(xg[1::2] - xo[1::2]).max()
+    dp = np.abs(xg[0::2] - xo[0::2]).max() / np.abs(xo[0::2]).max()
+    print(f"newton {itg} vs {ito}, bicgstab {g.stats()['linear_iterations']} vs {lito}, "
+          f"max |dS| {dS:.3e}, max |dp| / max |p| {dp:.3e}", flush=True)
+    assert rg == 0
+    assert abs(itg - ito) <= 1
+    assert dS < 1e-6
