@@ -92,6 +92,39 @@ def test_fp64_members_match_the_oracle(mode, schedule):
         assert dS < 1e-8
 
 
+def test_benchmark_size_members_match_the_oracle():
+    """Members of the benchmark ensemble's largest and smallest sizes (96x96x48: 144 tiles
+    of the wavefront, and 32x32x32) in one batched ensemble, FP64 preconditioner, Newton
+    tolerance 1e-8, two steps: identical Newton iterations and saturations within 1e-8 of
+    the oracle, as the small members."""
+    import ewg
+    from ewg import ensemble, problems
+    from oracle.pyoracle import OracleContext
+    probs = [problems.ensemble_member(0, 7, sizes=((96, 96, 48),), steps=2),
+             problems.ensemble_member(1, 7, sizes=((32, 32, 32),), steps=2)]
+    prm = ensemble.member_params(precond_fp32=0, newton_tolerance=1e-8)
+    ctxs = [ewg.GpuContext(P, prm) for P in probs]
+    ens = ensemble.Ensemble(ctxs)
+    ens.set_mode(ensemble.BATCHED, 0)
+    print("ensemble of", [(P.nx, P.ny, P.nz) for P in probs], flush=True)
+    ens.run(probs[0].dt, 2)
+    for i, P in enumerate(probs):
+        o = OracleContext(P, ewg.default_params(newton_tolerance=1e-8))
+        its = 0
+        for _ in range(2):
+            rc, it, _ = o.newton(P.dt)
+            assert rc == 0
+            its += it
+            o.advance_time_level()
+        st = ens.stats(i)
+        dS = np.abs(ctxs[i].solution(0)[1::2] - o.solution(0)[1::2]).max()
+        print(f"member {i} {(P.nx, P.ny, P.nz)}: ensemble newton {st['newton_iterations']}, "
+              f"oracle {its}; max |dS| {dS:.2e}", flush=True)
+        assert st["status"] == 0 and st["steps"] == 2
+        assert st["newton_iterations"] == its
+        assert dS < 1e-8
+
+
 @pytest.mark.parametrize("mode,schedule", [("batched", 0), ("batched", 1), ("persistent", 0)])
 def test_fp32_preconditioner_members_close_to_the_fp64_oracle(mode, schedule):
     """The ensemble's own form (FP32 ILU(0) inside FP64 BiCGStab, Newton tolerance 1e-6)
